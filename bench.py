@@ -1,0 +1,286 @@
+"""Benchmark of the cuMat hot path on one MI355X.
+
+Headline (value): Jacobi-PCG iterations per second on the 2048x2048 2D
+Poisson system (n = 4,194,304, nnz = 20,963,328, FP32), the largest
+single-GPU bandwidth configuration; a step is one CG iteration of the fused
+solver (tol = 0, so exactly K iterations run in the timed region).
+
+roofline: the CSR SpMV kernel on the same matrix, timed with HIP events on
+the stream it runs on, with four rotating copies of (matrix, x, y) so that
+successive launches cannot be served from the 256 MB last-level cache.
+Algorithmic bytes per launch = 8*nnz + 12*n.
+
+Also reported (bounded, after the timed region): the linear combination of 8
+vectors of 2^26 floats, dot of 2^26, row/column/batch sums of 1024x1024x64,
+the 128^2 plain CG solve (iters/s, time to tol 1e-6), and the CPU baseline:
+the reference's vendored Eigen (oracle/_ref) on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+class HipTimer:
+    """hipEvent pair on a given stream (the library's context stream)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.e0, self.e1 = C.c_void_p(), C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(self.e0)) == 0
+        assert self.hip.hipEventCreate(C.byref(self.e1)) == 0
+
+    def start(self, stream):
+        self.hip.hipEventRecord(self.e0, C.c_void_p(stream))
+
+    def stop_ms(self, stream):
+        self.hip.hipEventRecord(self.e1, C.c_void_p(stream))
+        self.hip.hipEventSynchronize(self.e1)
+        ms = C.c_float()
+        self.hip.hipEventElapsedTime(C.byref(ms), self.e0, self.e1)
+        return float(ms.value)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
+              "described in DESIGN.md and not run.", flush=True)
+        sys.exit(2)
+
+    import cumat_amd as cm
+    from cumat_amd import capi
+    from cumat_amd.problems import poisson2d
+
+    info = cm.device_info(0)
+    log("device:", info)
+    ctx = cm.Context(0)
+    stream = ctx.stream()
+    timer = HipTimer()
+
+    # ------------------------------------------------------------ workload
+    g = 2048
+    JA, IA, V = poisson2d(g)
+    n, nnz = g * g, IA.size
+    rng = np.random.default_rng(20261202)
+    b = rng.uniform(-1, 1, n).astype(np.float32)
+    A = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
+    bd = cm.DeviceArray.from_numpy(b, ctx=ctx)
+    xd = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
+    cg = cm.ConjugateGradient(A, path=capi.CG_MULTIKERNEL).setTolerance(0.0)
+
+    # ------------------------------------------------------------ timed CG
+    cg.setMaxIterations(args.warmup)
+    cg.solve(bd, xd, ctx=ctx)
+    ctx.sync()
+    cg.setMaxIterations(args.steps)
+    ctx.sync()
+    t0 = time.perf_counter()
+    cg.solve(bd, xd, ctx=ctx)
+    ctx.sync()
+    t1 = time.perf_counter()
+    assert cg.iterations() == args.steps, cg.iterations()
+    sec = t1 - t0
+    iters_per_s = args.steps / sec
+    ms_per_step = 1000.0 * sec / args.steps
+    cg_bytes = 8 * nnz + 56 * n
+    log(f"CG 2048^2: {iters_per_s:.1f} it/s, {ms_per_step * 1000:.1f} us/it, "
+        f"{cg_bytes / (ms_per_step * 1e-3) / 1e9:.0f} GB/s (fused-minimum bytes)")
+
+    # ------------------------------------------------------------ SpMV roofline
+    copies = []
+    for k in range(4):
+        Ak = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
+        xk = cm.DeviceArray.from_numpy(rng.uniform(-1, 1, n).astype(np.float32), ctx=ctx)
+        yk = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
+        copies.append((Ak, xk, yk))
+    for k in range(8):
+        Ak, xk, yk = copies[k % 4]
+        Ak.matvec(xk, yk, ctx=ctx)
+    ctx.sync()
+    reps = 200
+    timer.start(stream)
+    for k in range(reps):
+        Ak, xk, yk = copies[k % 4]
+        Ak.matvec(xk, yk, ctx=ctx)
+    spmv_ms = timer.stop_ms(stream) / reps
+    spmv_bytes = 8 * nnz + 12 * n
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    log(f"SpMV 2048^2: {spmv_ms * 1000:.1f} us/launch, {spmv_gbs:.0f} GB/s ({spmv_gbs / HBM_PEAK_GBS:.1%})")
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "spmv_poisson2048_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+
+    extras = {}
+    if not args.no_extras:
+        extras = run_extras(cm, capi, ctx, stream, timer, rng)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(JA, IA, V, b, n, nnz)
+
+    line = {
+        "metric": "CG iters/s + SpMV HBM GB/s (% of peak); batched/mixed vs plain at rel-res <=1e-6",
+        "value": round(iters_per_s, 2),
+        "unit": "CG iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (2D 5-point Poisson, b ~ U[-1,1] seed 20261202)",
+        "config": {"workload": "Jacobi-PCG iteration on 2D Poisson 2048x2048 (n=4194304, nnz=20963328), FP32 CSR",
+                   "n": n, "nnz": nnz, "cg_bytes_per_iteration": cg_bytes,
+                   "cg_effective_GBps": round(cg_bytes / (ms_per_step * 1e-3) / 1e9, 1)},
+        "roofline": {"bound": "hbm", "kernel": "csr_spmv_kernel<float,float> (2048^2 Poisson)",
+                     "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": spmv_bytes, "avg_launch_us": round(spmv_ms * 1000, 2)},
+        "cpu_baseline": cpu,
+        "extras": extras,
+        "device": info["name"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_extras(cm, capi, ctx, stream, timer, rng):
+    out = {}
+    # ---- linear combination 8 x 2^26 (fused: one kernel)
+    N = 1 << 26
+    vecs = [cm.DeviceArray.from_numpy(rng.random(N, dtype=np.float32) * 2 - 1, ctx=ctx) for _ in range(8)]
+    coefs = rng.uniform(-1, 1, 8).astype(np.float32)
+    outv = cm.DeviceArray(N, dtype=np.float32, ctx=ctx)
+    e = vecs[0].expr() * float(coefs[0])
+    for i in range(1, 8):
+        e = e + vecs[i].expr() * float(coefs[i])
+    ptrs = (C.c_void_p * 8)(*[v.ptr for v in vecs])
+    cf = (C.c_float * 8)(*coefs)
+    lc_bytes = 9 * 4 * N
+    for label, fn in (("lincomb8_expr", lambda: cm.assign(outv, e, ctx=ctx)),
+                      ("lincomb8_kernel", lambda: capi.call("cm_lincomb_f32", ctx.handle, 8, ptrs, cf, outv.ptr, N))):
+        fn()
+        ctx.sync()
+        reps = 20
+        timer.start(stream)
+        for _ in range(reps):
+            fn()
+        ms = timer.stop_ms(stream) / reps
+        out[label] = {"us": round(ms * 1000, 1), "GBps": round(lc_bytes / (ms * 1e-3) / 1e9, 1),
+                      "frac": round(lc_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(label, out[label])
+    # ---- dot 2^26
+    d_out = cm.DeviceArray(1, dtype=np.float32, ctx=ctx)
+    cm.dot(vecs[0], vecs[1], out=d_out, ctx=ctx)
+    ctx.sync()
+    reps = 50
+    timer.start(stream)
+    for k in range(reps):
+        cm.dot(vecs[k % 8], vecs[(k + 1) % 8], out=d_out, ctx=ctx)
+    ms = timer.stop_ms(stream) / reps
+    out["dot_2^26"] = {"us": round(ms * 1000, 1), "GBps": round(8 * N / (ms * 1e-3) / 1e9, 1),
+                       "frac": round(8 * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log("dot", out["dot_2^26"])
+    del vecs, outv
+    # ---- axis sums of 1024 x 1024 x 64 (two rotating tensors of 256 MiB)
+    R, Cc, B = 1024, 1024, 64
+    ts = [cm.DeviceArray.from_numpy(rng.standard_normal(R * Cc * B, dtype=np.float32), rows=R, cols=Cc, batches=B,
+                                    ctx=ctx) for _ in range(2)]
+    for name, axis, ob in (("row_sum", 1, Cc * B), ("col_sum", 2, R * B), ("batch_sum", 4, R * Cc)):
+        res = [cm.reduce(t, "sum", axis, ctx=ctx) for t in ts]
+        ctx.sync()
+        reps = 20
+        timer.start(stream)
+        for k in range(reps):
+            cm.reduce(ts[k % 2], "sum", axis, out=res[k % 2], ctx=ctx)
+        ms = timer.stop_ms(stream) / reps
+        by = 4 * R * Cc * B + 4 * ob
+        out[name] = {"us": round(ms * 1000, 1), "GBps": round(by / (ms * 1e-3) / 1e9, 1),
+                     "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(name, out[name])
+    del ts
+    # ---- plain CG on 128^2 (the reference's CPU-runnable configuration)
+    from cumat_amd.problems import poisson2d
+    JA, IA, V = poisson2d(128)
+    n = 128 * 128
+    b = np.random.default_rng(20261203).uniform(-1, 1, n).astype(np.float32)
+    A = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
+    bd = cm.DeviceArray.from_numpy(b, ctx=ctx)
+    xd = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
+    for path, label in ((capi.CG_RESIDENT, "cg128_resident"), (capi.CG_MULTIKERNEL, "cg128_multikernel")):
+        cg = cm.ConjugateGradient(A, path=path).setTolerance(1e-6).setMaxIterations(1000)
+        cg.solve(bd, xd, ctx=ctx)
+        ts_ = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            cg.solve(bd, xd, ctx=ctx)
+            ts_.append(time.perf_counter() - t0)
+        t = min(ts_)
+        out[label] = {"iterations": cg.iterations(), "error": cg.error(), "ms_to_solution": round(t * 1e3, 3),
+                      "iters_per_s": round(cg.iterations() / t, 1)}
+        log(label, out[label])
+    return out
+
+
+def cpu_baseline(JA, IA, V, b, n, nnz):
+    """The reference's vendored Eigen ConjugateGradient (oracle/_ref) on the
+    host cores: a bounded sample of 2048^2 CG iterations (tol 0)."""
+    try:
+        import oracle
+        if not oracle.ref_available():
+            return None
+        L = oracle.ref()
+        threads = len(os.sched_getaffinity(0))
+        JA = np.ascontiguousarray(JA)
+        IA = np.ascontiguousarray(IA)
+        h = L.eref_csr_create_f32(n, n, nnz, JA.ctypes.data, IA.ctypes.data, V.ctypes.data)
+        x = np.zeros(n, np.float32)
+        err = C.c_double()
+        L.eref_pcg_handle_f32(h, b.ctypes.data, x.ctypes.data, 0.0, 5, threads, C.byref(err))  # warm
+        sample = 100
+        t0 = time.perf_counter()
+        it = L.eref_pcg_handle_f32(h, b.ctypes.data, x.ctypes.data, 0.0, sample, threads, C.byref(err))
+        t = time.perf_counter() - t0
+        L.eref_csr_destroy_f32(h)
+        v = it / t
+        log(f"CPU Eigen CG 2048^2: {v:.2f} it/s on {threads} threads")
+        return {"value": round(v, 3), "unit": "CG iterations/s", "cores": threads, "kind": "reference",
+                "sample": f"{it} Jacobi-PCG iterations of Eigen 3.3.90 (reference third-party) on the same "
+                          f"2048^2 FP32 system, Eigen::setNbThreads({threads})"}
+    except Exception as ex:  # the baseline is reported, never required
+        log("cpu baseline failed:", ex)
+        return None
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1084 @@
+// cg.hip -- Jacobi-preconditioned Conjugate Gradient on CSR matrices.
+//
+// Replaces ConjugateGradient<SparseMatrix<S,1,CSR>, DiagonalPreconditioner>
+// ::_solve_with_guess_impl (cuMat/src/ConjugateGradient.h:123-206). The
+// reference issues ~11 launches and one device->host copy + stream sync per
+// iteration. Here every scalar (alpha, beta, |r|^2, r.z, the convergence
+// test, the iteration count) stays on the device:
+//
+//  MULTIKERNEL path (any n, up to 16 right-hand sides):
+//    setup   r = b - A x, p = M^-1 r, |b|^2, |r|^2, r.p, early outs
+//    per iteration, three launches:
+//      K1  tmp = A p                 + partial p.tmp  -> last block: alpha
+//      K2  x += alpha p, r -= alpha tmp, z = M^-1 r (not stored)
+//                                     + partial |r|^2, r.z -> last block:
+//                                       convergence test, beta, counter
+//      K3  p = M^-1 r + beta p
+//    Kernels launched after convergence return at once; the host checks the
+//    status word every few iterations through pinned memory and stops
+//    launching.
+//  RESIDENT path (n <= 32768, one right-hand side): one workgroup runs the
+//    whole solve; p lives in LDS, x / r / M^-1 in registers, the reductions
+//    are DPP wave reductions + LDS, and no kernel boundary is crossed.
+//
+// Both follow the reference's semantics exactly: threshold tol^2 |b|^2,
+// early outs, the converging iteration is not counted, error =
+// sqrt(max_b |r|^2 / |b|^2), all batches iterate until all have converged.
+#include "spmv_core.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <tuple>
+
+namespace cm {
+
+constexpr int CG_MAXB = 16;
+constexpr int CG_BS = 256;
+constexpr int CG_RPB = 512;    // rows per block of the SpMV kernel
+constexpr int CG_CAP = 4096;
+constexpr int CG_VPT = 4;      // vector elements per thread in K2/K3
+
+enum CgStatus : int { CG_RUNNING = 0, CG_CONVERGED = 1, CG_EARLY_ZERO = 2, CG_EARLY_GUESS = 3, CG_MAXIT = 4 };
+
+template <typename T>
+struct CgState {
+    T rhs2[CG_MAXB];
+    T thr[CG_MAXB];
+    T absNew[CG_MAXB];
+    T alpha[CG_MAXB];
+    T beta[CG_MAXB];
+    T rr[CG_MAXB];
+    int status;
+    int iters;
+    int max_iter;
+    int batches;
+    double err;
+    unsigned counter[4];
+};
+
+// ---------------------------------------------------------------- setup
+// r = b - A x (x = 0 when !use_guess), partial sums of b.b, r.r, r.(M^-1 r).
+// p = M^-1 r is written too (the first search direction).
+template <typename VT, typename T>
+__global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, int nnz, const int* __restrict__ JA,
+                                                         const int* __restrict__ IA, const VT* __restrict__ vals,
+                                                         const T* __restrict__ invd, const T* __restrict__ b,
+                                                         T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
+                                                         int use_guess, T tol, T* __restrict__ part,
+                                                         CgState<T>* st, const T* __restrict__ diag)
+{
+    __shared__ SpmvSmem<T, CG_RPB, CG_CAP> sm;
+    __shared__ T red[3 * CG_BS / 64];
+    __shared__ int flag;
+    const int blk = blockIdx.x;
+    const int bb = blockIdx.y;
+    const int r0 = blk * CG_RPB;
+    const int nr = min(CG_RPB, n - r0);
+    const long long off = (long long)bb * n;
+    T ax[CG_RPB / CG_BS];
+    if (use_guess) {
+        block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vals, x + off, r0, nr, sm, ax, diag);
+    } else {
+#pragma unroll
+        for (int j = 0; j < CG_RPB / CG_BS; ++j) ax[j] = T(0);
+    }
+    T sbb = 0, srr = 0, srp = 0;
+#pragma unroll
+    for (int j = 0; j < CG_RPB / CG_BS; ++j) {
+        int i = threadIdx.x + j * CG_BS;
+        if (i < nr) {
+            long long row = r0 + i;
+            T bv = b[off + row];
+            T rv = bv - ax[j];
+            T pv = invd[row] * rv;
+            if (!use_guess) x[off + row] = T(0);
+            r[off + row] = rv;
+            p[off + row] = pv;
+            sbb += bv * bv;
+            srr += rv * rv;
+            srp += rv * pv;
+        }
+    }
+    // block partials (3 sums)
+    sbb = wave_reduce<OpSum>(sbb);
+    srr = wave_reduce<OpSum>(srr);
+    srp = wave_reduce<OpSum>(srp);
+    constexpr int NW = CG_BS / 64;
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[w] = sbb; red[NW + w] = srr; red[2 * NW + w] = srp; }
+    __syncthreads();
+    const int nblk = gridDim.x;
+    if (threadIdx.x == 0) {
+        T a = 0, c = 0, d = 0;
+        for (int i = 0; i < NW; ++i) { a += red[i]; c += red[NW + i]; d += red[2 * NW + i]; }
+        long long base = (long long)bb * nblk + blk;
+        part[3 * base + 0] = a;
+        part[3 * base + 1] = c;
+        part[3 * base + 2] = d;
+    }
+    if (!last_block_arrive(&st->counter[0], gridDim.x * gridDim.y, &flag)) return;
+    // last block: totals per batch in block order, then the early outs
+    const int B = gridDim.y;
+    for (int k = 0; k < B; ++k) {
+        T a = 0, c = 0, d = 0;
+        for (int i = threadIdx.x; i < nblk; i += CG_BS) {
+            long long base = (long long)k * nblk + i;
+            a += part[3 * base + 0];
+            c += part[3 * base + 1];
+            d += part[3 * base + 2];
+        }
+        T a2 = a, c2 = c;
+        block_sum2<CG_BS>(a2, c2, red);
+        d = block_reduce<OpSum, CG_BS>(d, red);
+        if (threadIdx.x == 0) {
+            st->rhs2[k] = a2;
+            st->rr[k] = c2;
+            st->absNew[k] = d;
+            st->thr[k] = tol * tol * a2;
+            st->beta[k] = T(0);
+        }
+    }
+    if (threadIdx.x == 0) {
+        bool all_zero = true, all_conv = true;
+        T m = 0;
+        for (int k = 0; k < B; ++k) {
+            all_zero = all_zero && (st->rhs2[k] < tol * tol);
+            all_conv = all_conv && (st->rr[k] < st->thr[k]);
+            T q = st->rr[k] / st->rhs2[k];
+            m = (k == 0 || q > m) ? q : m;
+        }
+        st->iters = 0;
+        if (all_zero) { st->status = CG_EARLY_ZERO; st->err = 0.0; }
+        else if (all_conv) { st->status = CG_EARLY_GUESS; st->err = sqrt((double)m); }
+        else if (st->max_iter <= 0) { st->status = CG_MAXIT; st->err = sqrt((double)m); }
+        else st->status = CG_RUNNING;
+        st->counter[0] = 0;
+    }
+}
+
+// ---------------------------------------------------------------- K1
+template <typename VT, typename T>
+__global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
+                                                        const VT* __restrict__ vals, const T* __restrict__ p,
+                                                        T* __restrict__ tmp, T* __restrict__ part, CgState<T>* st,
+                                                        const T* __restrict__ diag)
+{
+    if (st->status != CG_RUNNING) return;
+    __shared__ SpmvSmem<T, CG_RPB, CG_CAP> sm;
+    __shared__ T red[CG_BS / 64];
+    __shared__ int flag;
+    const int nblk = gridDim.x;
+    const int blk = xcd_remap(blockIdx.x, nblk);
+    const int bb = blockIdx.y;
+    const int r0 = blk * CG_RPB;
+    const int nr = min(CG_RPB, n - r0);
+    const long long off = (long long)bb * n;
+    T ap[CG_RPB / CG_BS];
+    block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vals, p + off, r0, nr, sm, ap, diag);
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < CG_RPB / CG_BS; ++j) {
+        int i = threadIdx.x + j * CG_BS;
+        if (i < nr) {
+            long long row = off + r0 + i;
+            tmp[row] = ap[j];
+            s += p[row] * ap[j];
+        }
+    }
+    s = block_reduce<OpSum, CG_BS>(s, red);
+    if (threadIdx.x == 0) part[(long long)bb * nblk + blk] = s;
+    if (!last_block_arrive(&st->counter[1], gridDim.x * gridDim.y, &flag)) return;
+    const int B = gridDim.y;
+    for (int k = 0; k < B; ++k) {
+        T a = 0;
+        for (int i = threadIdx.x; i < nblk; i += CG_BS) a += part[(long long)k * nblk + i];
+        a = block_reduce<OpSum, CG_BS>(a, red);
+        if (threadIdx.x == 0) st->alpha[k] = st->absNew[k] / a;
+    }
+    if (threadIdx.x == 0) st->counter[1] = 0;
+}
+
+// ---------------------------------------------------------------- K2
+template <typename T>
+__global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, const T* __restrict__ invd, const T* __restrict__ p,
+                                                          const T* __restrict__ tmp, T* __restrict__ x,
+                                                          T* __restrict__ r, T* __restrict__ part, CgState<T>* st)
+{
+    if (st->status != CG_RUNNING) return;
+    __shared__ T red[2 * CG_BS / 64];
+    __shared__ int flag;
+    const int bb = blockIdx.y;
+    const long long off = (long long)bb * n;
+    const T alpha = st->alpha[bb];
+    T srr = 0, srz = 0;
+    const long long stride = (long long)gridDim.x * CG_BS;
+#pragma unroll
+    for (int j = 0; j < CG_VPT; ++j) {
+        long long i = (long long)blockIdx.x * CG_BS + threadIdx.x + j * stride;
+        if (i < n) {
+            T xv = x[off + i] + alpha * p[off + i];
+            T rv = r[off + i] - alpha * tmp[off + i];
+            x[off + i] = xv;
+            r[off + i] = rv;
+            srr += rv * rv;
+            srz += rv * (invd[i] * rv);
+        }
+    }
+    block_sum2<CG_BS>(srr, srz, red);
+    const int nblk = gridDim.x;
+    if (threadIdx.x == 0) {
+        long long base = (long long)bb * nblk + blockIdx.x;
+        part[2 * base] = srr;
+        part[2 * base + 1] = srz;
+    }
+    if (!last_block_arrive(&st->counter[2], gridDim.x * gridDim.y, &flag)) return;
+    const int B = gridDim.y;
+    __shared__ T rz_tot[CG_MAXB];
+    for (int k = 0; k < B; ++k) {
+        T a = 0, c = 0;
+        for (int i = threadIdx.x; i < nblk; i += CG_BS) {
+            long long base = (long long)k * nblk + i;
+            a += part[2 * base];
+            c += part[2 * base + 1];
+        }
+        block_sum2<CG_BS>(a, c, red);
+        if (threadIdx.x == 0) { st->rr[k] = a; rz_tot[k] = c; }
+    }
+    if (threadIdx.x == 0) {
+        bool all = true;
+        T m = 0;
+        for (int k = 0; k < B; ++k) {
+            all = all && (st->rr[k] < st->thr[k]);
+            T q = st->rr[k] / st->rhs2[k];
+            m = (k == 0 || q > m) ? q : m;
+        }
+        st->err = sqrt((double)m);
+        if (all) {
+            st->status = CG_CONVERGED;  // the converging step is not counted
+        } else {
+            for (int k = 0; k < B; ++k) {
+                T absOld = st->absNew[k];
+                st->absNew[k] = rz_tot[k];
+                st->beta[k] = rz_tot[k] / absOld;
+            }
+            st->iters += 1;
+            if (st->iters >= st->max_iter) st->status = CG_MAXIT;
+        }
+        st->counter[2] = 0;
+    }
+}
+
+// ---------------------------------------------------------------- K3
+template <typename T>
+__global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, const T* __restrict__ invd, const T* __restrict__ r,
+                                                           T* __restrict__ p, const CgState<T>* st)
+{
+    if (st->status != CG_RUNNING) return;
+    const int bb = blockIdx.y;
+    const long long off = (long long)bb * n;
+    const T beta = st->beta[bb];
+    const long long stride = (long long)gridDim.x * CG_BS;
+#pragma unroll
+    for (int j = 0; j < CG_VPT; ++j) {
+        long long i = (long long)blockIdx.x * CG_BS + threadIdx.x + j * stride;
+        if (i < n) p[off + i] = invd[i] * r[off + i] + beta * p[off + i];
+    }
+}
+
+template <typename T>
+__global__ void cg_finalize_kernel(long long total, T* x, const CgState<T>* st)
+{
+    if (st->status != CG_EARLY_ZERO) return;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) x[i] = T(0);
+}
+
+template <typename T>
+__global__ void cg_state_init_kernel(CgState<T>* st, int max_iter, int batches)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->status = CG_RUNNING;
+        st->iters = 0;
+        st->max_iter = max_iter;
+        st->batches = batches;
+        st->err = 0.0;
+        for (int i = 0; i < 4; ++i) st->counter[i] = 0;
+    }
+}
+
+template <typename T>
+__global__ void invdiag_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA, const T* __restrict__ vals,
+                               T* __restrict__ inv)
+{
+    int rr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rr >= n) return;
+    T d = T(0);
+    for (int k = JA[rr]; k < JA[rr + 1]; ++k)
+        if (IA[k] == rr) { d = vals[k]; break; }
+    inv[rr] = (d == T(0)) ? T(1) : T(1) / d;
+}
+
+// ================================================================= resident
+// One workgroup of BS threads solves one system with n <= BS*RPT rows; p in
+// LDS, everything else in registers. Used for single small systems and, over
+// many systems, by the persistent batched solver (sys_rows/row_off != NULL).
+template <typename T>
+struct ResidentArgs {
+    int nsys;
+    const int* row_off;   // nsys+1 (NULL: a single system of n rows)
+    const int* order;     // processing order of systems (NULL: identity)
+    int n_single;
+    const int* JA;
+    const int* IA;
+    const T* vals;
+    const T* invd;        // may be NULL: computed from the diagonal
+    const T* b;
+    T* x;
+    int use_guess;
+    T tol;
+    int max_iter;
+    int* iters_out;       // per system
+    T* err_out;           // per system
+    unsigned* queue;      // work counter (zeroed before launch)
+};
+
+template <typename T, int BS, int RPT>
+__global__ __launch_bounds__(BS) void cg_resident_kernel(const ResidentArgs<T> A)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* pl = reinterpret_cast<T*>(smem_raw);
+    __shared__ T red[2 * BS / 64];
+    __shared__ int s_sys;
+    const int tid = threadIdx.x;
+    for (;;) {
+        // ---- fetch the next system (dynamic queue balances uneven sizes)
+        if (tid == 0) s_sys = (int)atomicAdd(A.queue, 1u);
+        __syncthreads();
+        const int q = s_sys;
+        __syncthreads();
+        if (q >= A.nsys) break;
+        const int sys = A.order ? A.order[q] : q;
+        const int row0 = A.row_off ? A.row_off[sys] : 0;
+        const int n = A.row_off ? (A.row_off[sys + 1] - row0) : A.n_single;
+        const int k0 = A.JA[row0];
+        T xr[RPT], rr[RPT], id[RPT];
+        // ---- setup: invdiag, x, r = b - A x
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            int i = tid + j * BS;
+            xr[j] = T(0);
+            id[j] = T(1);
+            if (i < n) {
+                if (A.invd) id[j] = A.invd[row0 + i];
+                else {
+                    T d = T(0);
+                    for (int k = A.JA[row0 + i]; k < A.JA[row0 + i + 1]; ++k)
+                        if (A.IA[k] - row0 == i) { d = A.vals[k]; break; }
+                    id[j] = (d == T(0)) ? T(1) : T(1) / d;
+                }
+                if (A.use_guess) xr[j] = A.x[row0 + i];
+                pl[i] = xr[j];
+            }
+        }
+        __syncthreads();
+        T bb = 0, r2 = 0;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            int i = tid + j * BS;
+            rr[j] = T(0);
+            if (i < n) {
+                T ax = T(0);
+                if (A.use_guess) {
+                    int s = A.JA[row0 + i], e = A.JA[row0 + i + 1];
+                    if (s < e) {
+                        ax = A.vals[s] * pl[A.IA[s] - row0];
+                        for (int k = s + 1; k < e; ++k) ax += A.vals[k] * pl[A.IA[k] - row0];
+                    }
+                }
+                T bv = A.b[row0 + i];
+                rr[j] = bv - ax;
+                bb += bv * bv;
+                r2 += rr[j] * rr[j];
+            }
+        }
+        block_sum2<BS>(bb, r2, red);
+        const T thr = A.tol * A.tol * bb;
+        int iters = 0;
+        T err = T(0);
+        bool done = false;
+        if (bb < A.tol * A.tol) {  // zero right-hand side
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) xr[j] = T(0);
+            done = true;
+        } else if (r2 < thr) {  // the guess is already converged
+            err = sqrt(r2 / bb);
+            done = true;
+        }
+        if (!done) {
+            // p = M^-1 r -> LDS, absNew = r.p
+            T absNew = 0, dummy = 0;
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                int i = tid + j * BS;
+                if (i < n) {
+                    T pv = id[j] * rr[j];
+                    pl[i] = pv;
+                    absNew += rr[j] * pv;
+                }
+            }
+            block_sum2<BS>(absNew, dummy, red);  // also orders the LDS writes
+            T rsq = r2;
+            int it = 0;
+            while (it < A.max_iter) {
+                // tmp = A p ; p.tmp
+                T tmp[RPT];
+                T pap = 0;
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) {
+                    int i = tid + j * BS;
+                    tmp[j] = T(0);
+                    if (i < n) {
+                        int s = A.JA[row0 + i], e = A.JA[row0 + i + 1];
+                        T acc = T(0);
+                        if (s < e) {
+                            acc = A.vals[s] * pl[A.IA[s] - row0];
+                            for (int k = s + 1; k < e; ++k) acc += A.vals[k] * pl[A.IA[k] - row0];
+                        }
+                        tmp[j] = acc;
+                        pap += pl[i] * acc;
+                    }
+                }
+                T z0 = 0;
+                block_sum2<BS>(pap, z0, red);
+                const T alpha = absNew / pap;
+                T sr = 0, sz = 0;
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) {
+                    int i = tid + j * BS;
+                    if (i < n) {
+                        xr[j] += alpha * pl[i];
+                        rr[j] -= alpha * tmp[j];
+                        sr += rr[j] * rr[j];
+                        sz += rr[j] * (id[j] * rr[j]);
+                    }
+                }
+                block_sum2<BS>(sr, sz, red);
+                rsq = sr;
+                if (sr < thr) break;
+                const T beta = sz / absNew;
+                absNew = sz;
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) {
+                    int i = tid + j * BS;
+                    if (i < n) pl[i] = id[j] * rr[j] + beta * pl[i];
+                }
+                __syncthreads();
+                ++it;
+            }
+            iters = it;
+            err = sqrt(rsq / bb);
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            int i = tid + j * BS;
+            if (i < n) A.x[row0 + i] = xr[j];
+        }
+        if (tid == 0) {
+            if (A.iters_out) A.iters_out[sys] = iters;
+            if (A.err_out) A.err_out[sys] = err;
+        }
+        (void)k0;
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- host side
+template <typename T, int BS, int RPT>
+static int launch_resident_one(cm_ctx_t ctx, const ResidentArgs<T>& A, int grid, size_t lds)
+{
+    auto k = cg_resident_kernel<T, BS, RPT>;
+    CM_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (grid <= 0) {
+        // persistent grid: as many workgroups as can be resident at once
+        int per_cu = 0;
+        CM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, BS, lds));
+        if (per_cu < 1) return fail(CM_E_UNSUPPORTED, "resident CG: tile does not fit on a CU");
+        grid = std::min(A.nsys, per_cu * (ctx->cu_count > 0 ? ctx->cu_count : 256));
+    }
+    k<<<grid, BS, lds, ctx->stream>>>(A);
+    CM_LAUNCH_CHECK();
+    return CM_OK;
+}
+
+// Picks the register tile RPT for a block size and the largest system.
+template <typename T, int BS>
+static int launch_resident_bs(cm_ctx_t ctx, const ResidentArgs<T>& A, int nmax, int grid)
+{
+    const size_t lds = sizeof(T) * (size_t)nmax;
+    const int rpt = (nmax + BS - 1) / BS;
+    if (rpt <= 1) return launch_resident_one<T, BS, 1>(ctx, A, grid, lds);
+    if (rpt <= 2) return launch_resident_one<T, BS, 2>(ctx, A, grid, lds);
+    if (rpt <= 4) return launch_resident_one<T, BS, 4>(ctx, A, grid, lds);
+    if (rpt <= 8) return launch_resident_one<T, BS, 8>(ctx, A, grid, lds);
+    if (rpt <= 12) return launch_resident_one<T, BS, 12>(ctx, A, grid, lds);
+    if (rpt <= 16) return launch_resident_one<T, BS, 16>(ctx, A, grid, lds);
+    if constexpr (BS <= 512) {
+        if (rpt <= 24) return launch_resident_one<T, BS, 24>(ctx, A, grid, lds);
+        if (rpt <= 32) return launch_resident_one<T, BS, 32>(ctx, A, grid, lds);
+    }
+    return fail(CM_E_UNSUPPORTED, "resident CG: %d rows exceed %d threads x tile", nmax, BS);
+}
+
+template <typename T>
+static int launch_resident(cm_ctx_t ctx, const ResidentArgs<T>& A, int nmax, int bs, int grid)
+{
+    switch (bs) {
+    case 256: return launch_resident_bs<T, 256>(ctx, A, nmax, grid);
+    case 512: return launch_resident_bs<T, 512>(ctx, A, nmax, grid);
+    case 1024: return launch_resident_bs<T, 1024>(ctx, A, nmax, grid);
+    default: return fail(CM_E_ARG, "resident CG: block size %d (256, 512 or 1024)", bs);
+    }
+}
+
+template <typename VT, typename T>
+static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, const int* IA, const VT* vals,
+                           const T* invd, const T* b, T* x, int use_guess, double tol, int max_iter, int* iters_out,
+                           double* err_out, cm_cg_stats* stats, const T* diag = nullptr)
+{
+    constexpr bool same_type = std::is_same<VT, T>::value;
+    CM_REQUIRE(invd || same_type, CM_E_ARG, "CG: mixed storage needs an explicit inverse diagonal");
+    CM_REQUIRE(B >= 1 && B <= CG_MAXB, CM_E_UNSUPPORTED, "CG: %d right-hand sides (max %d)", B, CG_MAXB);
+    Temps tmp(ctx);
+    const long long N = (long long)n * B;
+    void *r_, *p_, *t_, *st_, *part_;
+    CM_TRY(tmp.get(sizeof(T) * N, &r_));
+    CM_TRY(tmp.get(sizeof(T) * N, &p_));
+    CM_TRY(tmp.get(sizeof(T) * N, &t_));
+    CM_TRY(tmp.get(sizeof(CgState<T>), &st_));
+    const int nblk_spmv = div_up(n, CG_RPB);
+    const int nblk_vec = div_up(n, (long long)CG_BS * CG_VPT);
+    const long long nparts = 3ll * B * std::max(nblk_spmv, nblk_vec);
+    CM_TRY(tmp.get(sizeof(T) * nparts, &part_));
+    T* r = (T*)r_;
+    T* p = (T*)p_;
+    T* t = (T*)t_;
+    T* part = (T*)part_;
+    CgState<T>* st = (CgState<T>*)st_;
+    if (!invd) {
+        void* id_;
+        CM_TRY(tmp.get(sizeof(T) * n, &id_));
+        // the matrix value type equals the vector type on this path
+        invdiag_kernel<T><<<div_up(n, 256), 256, 0, ctx->stream>>>(n, JA, IA, (const T*)vals, (T*)id_);
+        CM_LAUNCH_CHECK();
+        invd = (const T*)id_;
+    }
+    int launches = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    cg_state_init_kernel<T><<<1, 64, 0, ctx->stream>>>(st, max_iter, B);
+    CM_LAUNCH_CHECK();
+    cg_setup_kernel<VT, T><<<dim3(nblk_spmv, B), CG_BS, 0, ctx->stream>>>(n, nnz, JA, IA, vals, invd, b, x, r, p,
+                                                                          use_guess, (T)tol, part, st, diag);
+    CM_LAUNCH_CHECK();
+    launches += 2;
+    int* hstat = nullptr;
+    CM_TRY(pinned(ctx, (void**)&hstat));
+    hstat[0] = 0;
+    hipEvent_t ev;
+    CM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    const int CHUNK = 8;
+    bool pending = false;
+    int it = 0;
+    while (it < max_iter) {
+        int todo = std::min(CHUNK, max_iter - it);
+        for (int k = 0; k < todo; ++k) {
+            cg_spmv_kernel<VT, T><<<dim3(nblk_spmv, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, p, t, part, st,
+                                                                                 diag);
+            cg_update_kernel<T><<<dim3(nblk_vec, B), CG_BS, 0, ctx->stream>>>(n, invd, p, t, x, r, part, st);
+            cg_pupdate_kernel<T><<<dim3(nblk_vec, B), CG_BS, 0, ctx->stream>>>(n, invd, r, p, st);
+        }
+        CM_LAUNCH_CHECK();
+        launches += 3 * todo;
+        it += todo;
+        // lagged status check: look at the previous chunk's word without
+        // draining the queue, then request the current one
+        if (pending) {
+            CM_HIP(hipEventSynchronize(ev));
+            if (hstat[0] != CG_RUNNING) break;
+        }
+        CM_HIP(hipMemcpyAsync(hstat, &st->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        CM_HIP(hipEventRecord(ev, ctx->stream));
+        pending = true;
+    }
+    cg_finalize_kernel<T><<<div_up(N, 256), 256, 0, ctx->stream>>>(N, x, st);
+    CM_LAUNCH_CHECK();
+    launches += 1;
+    CgState<T> hs;
+    CM_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_HIP(hipEventDestroy(ev));
+    auto t1 = std::chrono::steady_clock::now();
+    if (iters_out) *iters_out = hs.iters;
+    if (err_out) *err_out = hs.err;
+    if (stats) {
+        stats->path_used = CM_CG_MULTIKERNEL;
+        stats->launches = launches;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return CM_OK;
+}
+
+template <typename T>
+static int pcg_resident(cm_ctx_t ctx, int n, const int* JA, const int* IA, const T* vals, const T* invd, const T* b,
+                        T* x, int use_guess, double tol, int max_iter, int* iters_out, double* err_out,
+                        cm_cg_stats* stats)
+{
+    Temps tmp(ctx);
+    void* q;
+    void* io;
+    void* eo;
+    CM_TRY(tmp.get(256, &q));
+    CM_TRY(tmp.get(256, &io));
+    CM_TRY(tmp.get(256, &eo));
+    CM_TRY(fill_zero(ctx, q, 256));
+    ResidentArgs<T> A{};
+    A.nsys = 1;
+    A.n_single = n;
+    A.JA = JA;
+    A.IA = IA;
+    A.vals = vals;
+    A.invd = invd;
+    A.b = b;
+    A.x = x;
+    A.use_guess = use_guess;
+    A.tol = (T)tol;
+    A.max_iter = max_iter;
+    A.iters_out = (int*)io;
+    A.err_out = (T*)eo;
+    A.queue = (unsigned*)q;
+    auto t0 = std::chrono::steady_clock::now();
+    CM_TRY(launch_resident<T>(ctx, A, n, 1024, 1));
+    int hi = 0;
+    T he = 0;
+    CM_HIP(hipMemcpyAsync(&hi, io, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipMemcpyAsync(&he, eo, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipStreamSynchronize(ctx->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    if (iters_out) *iters_out = hi;
+    if (err_out) *err_out = (double)he;
+    if (stats) {
+        stats->path_used = CM_CG_RESIDENT;
+        stats->launches = 2;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return CM_OK;
+}
+
+// ------------------------------------------------------ batched systems
+struct BatchedPlan {
+    int nmax = 0;
+    std::vector<int> order;  // systems by decreasing nnz (longest first)
+};
+
+static int make_plan(cm_ctx_t ctx, int nsys, const int* row_off_dev, const int* JA_dev, BatchedPlan& plan)
+{
+    std::vector<int> ro(nsys + 1);
+    CM_HIP(hipMemcpyAsync(ro.data(), row_off_dev, sizeof(int) * (nsys + 1), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<int> ja(nsys + 1);
+    CM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s <= nsys; ++s)
+        CM_HIP(hipMemcpyAsync(&ja[s], JA_dev + ro[s], sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipStreamSynchronize(ctx->stream));
+    plan.nmax = 0;
+    std::vector<std::pair<long long, int>> cost(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        int n = ro[s + 1] - ro[s];
+        CM_REQUIRE(n >= 0, CM_E_ARG, "batched CG: row offsets decrease at system %d", s);
+        plan.nmax = std::max(plan.nmax, n);
+        // CG iterations grow with the grid side ~ sqrt(n) for these systems;
+        // work per iteration ~ nnz: schedule the heaviest first
+        cost[s] = {-(long long)(ja[s + 1] - ja[s]) * (long long)(1 + (int)std::sqrt((double)n)), s};
+    }
+    std::sort(cost.begin(), cost.end());
+    plan.order.resize(nsys);
+    for (int s = 0; s < nsys; ++s) plan.order[s] = cost[s].second;
+    return CM_OK;
+}
+
+struct GraphKey {
+    cm_ctx_t ctx;
+    const void* ptrs[9];
+    int nsys, bs, max_iter;
+    float tol;
+    bool operator<(const GraphKey& o) const { return memcmp(this, &o, sizeof(GraphKey)) < 0; }
+};
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    void* order = nullptr;
+    void* queue = nullptr;
+};
+static std::mutex g_graph_mutex;
+static std::map<GraphKey, GraphEntry> g_graphs;
+static std::map<std::tuple<int, int, int, int>, int> g_tuned_bs;
+
+static int batched_launch(cm_ctx_t ctx, int nsys, const int* row_off, const int* JA, const int* IA, const float* vals,
+                          const float* b, float* x, float tol, int max_iter, int* iters, float* errors, int bs,
+                          const BatchedPlan& plan, void* order_dev, void* queue_dev)
+{
+    CM_TRY(fill_zero(ctx, queue_dev, 256));
+    ResidentArgs<float> A{};
+    A.nsys = nsys;
+    A.row_off = row_off;
+    A.order = (const int*)order_dev;
+    A.JA = JA;
+    A.IA = IA;
+    A.vals = vals;
+    A.invd = nullptr;
+    A.b = b;
+    A.x = x;
+    A.use_guess = 0;
+    A.tol = tol;
+    A.max_iter = max_iter;
+    A.iters_out = iters;
+    A.err_out = errors;
+    A.queue = (unsigned*)queue_dev;
+    return launch_resident<float>(ctx, A, plan.nmax, bs, 0);
+}
+
+static int default_bs(int nmax)
+{
+    if (nmax <= 2048) return 256;
+    if (nmax <= 8192) return 512;
+    return 1024;
+}
+
+// ============================================ mixed precision + FP64 refinement
+// FP64 CSR -> inner storage. F32: every value rounded to float. BF16: the
+// off-diagonals rounded to bf16 (round to nearest even), the diagonal kept
+// in FP32 and shifted by the row's rounding errors so that each row sum of
+// the stored matrix equals the row sum of A (for an M-matrix / weighted
+// Laplacian this keeps the stored matrix spectrally within 1 +- 2^-9 of A).
+__device__ __forceinline__ unsigned short f64_to_bf16(double v)
+{
+    float f = (float)v;
+    unsigned u = __float_as_uint(f);
+    unsigned lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;  // round to nearest, ties to even (finite inputs)
+    return (unsigned short)(u >> 16);
+}
+
+template <bool BF16>
+__global__ void ir_convert_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
+                                  const double* __restrict__ v64, void* __restrict__ vlow, float* __restrict__ diag,
+                                  float* __restrict__ invd)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int k = JA[i]; k < JA[i + 1]; ++k) {
+        double a = v64[k];
+        if (IA[k] == i) {
+            d += a;
+            if (BF16) ((unsigned short*)vlow)[k] = 0;
+            else ((float*)vlow)[k] = (float)a;
+        } else if (BF16) {
+            unsigned short h = f64_to_bf16(a);
+            ((unsigned short*)vlow)[k] = h;
+            d += a - (double)bf16_to_float(h);
+        } else {
+            ((float*)vlow)[k] = (float)a;
+        }
+    }
+    float df = (float)d;
+    if (BF16) diag[i] = df;
+    else {
+        // diagonal as stored in the FP32 matrix
+        df = 0.f;
+        for (int k = JA[i]; k < JA[i + 1]; ++k)
+            if (IA[k] == i) { df = ((float*)vlow)[k]; break; }
+    }
+    invd[i] = (df == 0.f) ? 1.f : 1.f / df;
+}
+
+// r = b - A x in FP64 (FP64 values, FP64 vectors)
+__global__ __launch_bounds__(CG_BS) void ir_residual_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
+                                                            const double* __restrict__ v64, const double* __restrict__ b,
+                                                            const double* __restrict__ x, double* __restrict__ r)
+{
+    __shared__ SpmvSmem<double, CG_RPB, CG_CAP> sm;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int r0 = blk * CG_RPB;
+    if (r0 >= n) return;
+    const int nr = min(CG_RPB, n - r0);
+    double ax[CG_RPB / CG_BS];
+    block_csr_spmv<double, double, CG_BS, CG_RPB, CG_CAP>(JA, IA, v64, x, r0, nr, sm, ax);
+#pragma unroll
+    for (int j = 0; j < CG_RPB / CG_BS; ++j) {
+        int i = threadIdx.x + j * CG_BS;
+        if (i < nr) r[r0 + i] = b[r0 + i] - ax[j];
+    }
+}
+
+__global__ void ir_scale_kernel(int n, const double* __restrict__ r, const double* __restrict__ rn2, float* __restrict__ r32)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 1.0 / sqrt(*rn2);
+    r32[i] = (float)(r[i] * s);
+}
+
+__global__ void ir_update_kernel(int n, double* __restrict__ x, const float* __restrict__ e, const double* __restrict__ rn2)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] += sqrt(*rn2) * (double)e[i];
+}
+
+static int pcg_ir(cm_ctx_t ctx, int n, int nnz, const int* JA, const int* IA, const double* v64, int vdt,
+                  const double* b, double* x, double tol, double inner_tol, int max_outer, int max_inner,
+                  cm_ir_stats* stats)
+{
+    Temps tmp(ctx);
+    void *vlow, *diag, *invd, *r, *r32, *e, *dd;
+    CM_TRY(tmp.get((vdt == CM_BF16 ? 2 : 4) * (size_t)std::max(nnz, 1), &vlow));
+    CM_TRY(tmp.get(sizeof(float) * n, &diag));
+    CM_TRY(tmp.get(sizeof(float) * n, &invd));
+    CM_TRY(tmp.get(sizeof(double) * n, &r));
+    CM_TRY(tmp.get(sizeof(float) * n, &r32));
+    CM_TRY(tmp.get(sizeof(float) * n, &e));
+    CM_TRY(tmp.get(sizeof(double) * 2, &dd));
+    double* bn2 = (double*)dd;
+    double* rn2 = bn2 + 1;
+    auto t0 = std::chrono::steady_clock::now();
+    const unsigned g = div_up(n, 256);
+    if (vdt == CM_BF16)
+        ir_convert_kernel<true><<<g, 256, 0, ctx->stream>>>(n, JA, IA, v64, vlow, (float*)diag, (float*)invd);
+    else
+        ir_convert_kernel<false><<<g, 256, 0, ctx->stream>>>(n, JA, IA, v64, vlow, (float*)diag, (float*)invd);
+    CM_LAUNCH_CHECK();
+    CM_TRY(fill_zero(ctx, x, sizeof(double) * n));
+    CM_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    CM_TRY(cm_dot(ctx, CM_F64, n, 1, b, 1, b, 1, bn2));
+    double hbn2 = 0, hrn2 = 0;
+    CM_HIP(hipMemcpyAsync(&hbn2, bn2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipStreamSynchronize(ctx->stream));
+    int outer = 0, inner_total = 0;
+    double rel = (hbn2 > 0) ? 1.0 : 0.0;
+    while (hbn2 > 0) {
+        CM_TRY(cm_dot(ctx, CM_F64, n, 1, r, 1, r, 1, rn2));
+        CM_HIP(hipMemcpyAsync(&hrn2, rn2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        CM_HIP(hipStreamSynchronize(ctx->stream));
+        rel = std::sqrt(hrn2 / hbn2);
+        if (rel <= tol || outer >= max_outer || hrn2 == 0.0) break;
+        ir_scale_kernel<<<g, 256, 0, ctx->stream>>>(n, (const double*)r, rn2, (float*)r32);
+        CM_LAUNCH_CHECK();
+        int it = 0;
+        double err = 0;
+        if (vdt == CM_BF16)
+            CM_TRY((pcg_multikernel<unsigned short, float>(ctx, n, nnz, 1, JA, IA, (const unsigned short*)vlow,
+                                                           (const float*)invd, (const float*)r32, (float*)e, 0,
+                                                           inner_tol, max_inner, &it, &err, nullptr,
+                                                           (const float*)diag)));
+        else
+            CM_TRY((pcg_multikernel<float, float>(ctx, n, nnz, 1, JA, IA, (const float*)vlow, (const float*)invd,
+                                                  (const float*)r32, (float*)e, 0, inner_tol, max_inner, &it, &err,
+                                                  nullptr)));
+        inner_total += it;
+        ir_update_kernel<<<g, 256, 0, ctx->stream>>>(n, x, (const float*)e, rn2);
+        CM_LAUNCH_CHECK();
+        ir_residual_kernel<<<div_up(n, CG_RPB), CG_BS, 0, ctx->stream>>>(n, JA, IA, v64, b, x, (double*)r);
+        CM_LAUNCH_CHECK();
+        ++outer;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->outer_iterations = outer;
+        stats->inner_iterations = inner_total;
+        stats->final_rel_residual = rel;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return CM_OK;
+}
+
+}  // namespace cm
+
+extern "C" {
+
+int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off, int total_rows, int total_nnz, const int* JA,
+                         const int* IA, const float* vals, const float* b, float* x, float tol, int max_iter,
+                         int* iterations, float* errors, int block_size, int use_graph)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(nsys >= 0 && total_rows >= 0 && total_nnz >= 0, CM_E_DIM, "cm_pcg_batched_solve: bad sizes");
+    if (nsys == 0) return CM_OK;
+    CM_REQUIRE(row_off && JA && IA && vals && b && x, CM_E_ARG, "cm_pcg_batched_solve: NULL pointer");
+    cm::BatchedPlan plan;
+    CM_TRY(cm::make_plan(ctx, nsys, row_off, JA, plan));
+    CM_REQUIRE(plan.nmax <= 16384, CM_E_UNSUPPORTED, "batched CG: system of %d rows (max 16384)", plan.nmax);
+    if (block_size == 0) {
+        std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
+        auto it = cm::g_tuned_bs.find({nsys, total_rows, total_nnz, plan.nmax});
+        block_size = (it != cm::g_tuned_bs.end()) ? it->second : cm::default_bs(plan.nmax);
+    }
+    if (!use_graph) {
+        cm::Temps tmp(ctx);
+        void *ord, *q;
+        CM_TRY(tmp.get(sizeof(int) * nsys, &ord));
+        CM_TRY(tmp.get(256, &q));
+        CM_HIP(hipMemcpyAsync(ord, plan.order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
+        CM_TRY(cm::batched_launch(ctx, nsys, row_off, JA, IA, vals, b, x, tol, max_iter, iterations, errors,
+                                  block_size, plan, ord, q));
+        CM_HIP(hipStreamSynchronize(ctx->stream));
+        return CM_OK;
+    }
+    // graph path: the (queue reset + persistent solve) pair is captured once
+    // per argument set and replayed
+    cm::GraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.ctx = ctx;
+    const void* ps[9] = {row_off, JA, IA, vals, b, x, iterations, errors, nullptr};
+    memcpy(key.ptrs, ps, sizeof(ps));
+    key.nsys = nsys;
+    key.bs = block_size;
+    key.max_iter = max_iter;
+    key.tol = tol;
+    std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
+    auto it = cm::g_graphs.find(key);
+    if (it == cm::g_graphs.end()) {
+        cm::GraphEntry ge;
+        CM_TRY(cm::dalloc(ctx, sizeof(int) * nsys, &ge.order));
+        CM_TRY(cm::dalloc(ctx, 256, &ge.queue));
+        CM_HIP(hipMemcpyAsync(ge.order, plan.order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
+        // the kernel attribute must be set outside capture
+        CM_TRY(cm::batched_launch(ctx, 0, row_off, JA, IA, vals, b, x, tol, max_iter, iterations, errors, block_size,
+                                  plan, ge.order, ge.queue));
+        CM_HIP(hipStreamSynchronize(ctx->stream));
+        hipGraph_t g;
+        CM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int s = cm::batched_launch(ctx, nsys, row_off, JA, IA, vals, b, x, tol, max_iter, iterations, errors,
+                                   block_size, plan, ge.order, ge.queue);
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        CM_TRY(s);
+        CM_HIP(e);
+        CM_HIP(hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0));
+        CM_HIP(hipGraphDestroy(g));
+        it = cm::g_graphs.emplace(key, ge).first;
+    }
+    CM_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
+    cm::count(CM_CNT_KERNEL_LAUNCH, 2);
+    CM_HIP(hipStreamSynchronize(ctx->stream));
+    return CM_OK;
+    CM_API_END
+}
+
+int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off, int total_rows, int total_nnz, const int* JA,
+                            const int* IA, const float* vals, const float* b, float tol, int max_iter,
+                            int* best_block_size, float* best_ms)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(nsys > 0, CM_E_DIM, "autotune: no systems");
+    cm::BatchedPlan plan;
+    CM_TRY(cm::make_plan(ctx, nsys, row_off, JA, plan));
+    cm::Temps tmp(ctx);
+    void *xs, *it, *er, *ord, *q;
+    CM_TRY(tmp.get(sizeof(float) * (size_t)total_rows, &xs));
+    CM_TRY(tmp.get(sizeof(int) * nsys, &it));
+    CM_TRY(tmp.get(sizeof(float) * nsys, &er));
+    CM_TRY(tmp.get(sizeof(int) * nsys, &ord));
+    CM_TRY(tmp.get(256, &q));
+    CM_HIP(hipMemcpyAsync(ord, plan.order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
+    hipEvent_t e0, e1;
+    CM_HIP(hipEventCreate(&e0));
+    CM_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float bestms = 1e30f;
+    const int cands[3] = {256, 512, 1024};
+    for (int bs : cands) {
+        if ((plan.nmax + bs - 1) / bs > (bs <= 512 ? 32 : 16)) continue;  // register tile too large
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            CM_HIP(hipEventRecord(e0, ctx->stream));
+            int s = cm::batched_launch(ctx, nsys, row_off, JA, IA, vals, b, (float*)xs, tol, max_iter, (int*)it,
+                                       (float*)er, bs, plan, ord, q);
+            if (s != CM_OK) break;
+            CM_HIP(hipEventRecord(e1, ctx->stream));
+            CM_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms_min = std::min(ms_min, ms);
+        }
+        if (ms_min < bestms) { bestms = ms_min; best = bs; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    CM_REQUIRE(best > 0, CM_E_UNSUPPORTED, "autotune: no launch shape fits systems of %d rows", plan.nmax);
+    {
+        std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
+        cm::g_tuned_bs[{nsys, total_rows, total_nnz, plan.nmax}] = best;
+    }
+    if (best_block_size) *best_block_size = best;
+    if (best_ms) *best_ms = bestms;
+    return CM_OK;
+    CM_API_END
+}
+
+int cm_pcg_solve(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int* JA, const int* IA,
+                 const void* vals, const void* invdiag, const void* b, void* x, int use_guess, double tol,
+                 int max_iter, int path, int* iterations_out, double* error_out, cm_cg_stats* stats)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(n >= 0 && nnz >= 0 && batches >= 1, CM_E_DIM, "cm_pcg_solve: bad sizes");
+    CM_REQUIRE(tol >= 0, CM_E_ARG, "cm_pcg_solve: negative tolerance");
+    if (max_iter < 0) max_iter = 2 * n;  // IterativeSolverBase::maxIterations (cuMat/src/IterativeSolverBase.h:85)
+    if (n == 0) {
+        if (iterations_out) *iterations_out = 0;
+        if (error_out) *error_out = 0;
+        return CM_OK;
+    }
+    CM_REQUIRE(JA && IA && vals && b && x, CM_E_ARG, "cm_pcg_solve: NULL pointer");
+    const bool resident_ok = batches == 1 && n <= 16384;
+    if (path == CM_CG_AUTO) path = (resident_ok && n <= 16384) ? CM_CG_RESIDENT : CM_CG_MULTIKERNEL;
+    if (path == CM_CG_RESIDENT) {
+        CM_REQUIRE(resident_ok, CM_E_UNSUPPORTED, "resident CG needs one rhs and n <= 16384");
+        if (dtype == CM_F32)
+            return cm::pcg_resident<float>(ctx, n, JA, IA, (const float*)vals, (const float*)invdiag, (const float*)b,
+                                           (float*)x, use_guess, tol, max_iter, iterations_out, error_out, stats);
+        if (dtype == CM_F64)
+            return cm::pcg_resident<double>(ctx, n, JA, IA, (const double*)vals, (const double*)invdiag,
+                                            (const double*)b, (double*)x, use_guess, tol, max_iter, iterations_out,
+                                            error_out, stats);
+        return cm::fail(CM_E_UNSUPPORTED, "cm_pcg_solve: dtype %d", dtype);
+    }
+    if (dtype == CM_F32)
+        return cm::pcg_multikernel<float, float>(ctx, n, nnz, batches, JA, IA, (const float*)vals,
+                                                 (const float*)invdiag, (const float*)b, (float*)x, use_guess, tol,
+                                                 max_iter, iterations_out, error_out, stats);
+    if (dtype == CM_F64)
+        return cm::pcg_multikernel<double, double>(ctx, n, nnz, batches, JA, IA, (const double*)vals,
+                                                   (const double*)invdiag, (const double*)b, (double*)x, use_guess,
+                                                   tol, max_iter, iterations_out, error_out, stats);
+    return cm::fail(CM_E_UNSUPPORTED, "cm_pcg_solve: dtype %d", dtype);
+    CM_API_END
+}
+
+int cm_pcg_ir_solve(cm_ctx_t ctx, int n, int nnz, const int* JA, const int* IA, const double* vals64, int val_dtype,
+                    const double* b, double* x, double tol, double inner_tol, int max_outer, int max_inner,
+                    cm_ir_stats* stats)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(n >= 1 && nnz >= 0, CM_E_DIM, "cm_pcg_ir_solve: bad sizes");
+    CM_REQUIRE(val_dtype == CM_F32 || val_dtype == CM_BF16, CM_E_ARG, "cm_pcg_ir_solve: inner storage %d",
+               val_dtype);
+    CM_REQUIRE(JA && IA && vals64 && b && x, CM_E_ARG, "cm_pcg_ir_solve: NULL pointer");
+    CM_REQUIRE(tol > 0 && inner_tol > 0 && max_outer >= 0, CM_E_ARG, "cm_pcg_ir_solve: bad tolerances");
+    if (max_inner < 0) max_inner = 2 * n;
+    return cm::pcg_ir(ctx, n, nnz, JA, IA, vals64, val_dtype, b, x, tol, inner_tol, max_outer, max_inner, stats);
+    CM_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,334 @@
+// cm_common.hpp -- shared pieces of the gfx950 library: error plumbing of the
+// C ABI, the context object, scratch memory, launch helpers, and wave64
+// DPP reductions used by every reduction-shaped kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cumat_amd.h"
+
+namespace cm {
+
+// ------------------------------------------------------------------ errors
+int fail(int code, const char* fmt, ...);
+void clear_error();
+
+#define CM_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::cm::fail(CM_E_HIP, "%s failed: %s (%s:%d)", #call,              \
+                              hipGetErrorString(e_), __FILE__, __LINE__);             \
+    } while (0)
+
+#define CM_LAUNCH_CHECK()                                                             \
+    do {                                                                              \
+        hipError_t e_ = hipGetLastError();                                            \
+        if (e_ != hipSuccess)                                                         \
+            return ::cm::fail(CM_E_HIP, "kernel launch failed: %s (%s:%d)",          \
+                              hipGetErrorString(e_), __FILE__, __LINE__);             \
+        ::cm::count(CM_CNT_KERNEL_LAUNCH);                                            \
+    } while (0)
+
+#define CM_REQUIRE(cond, code, ...)                                                   \
+    do {                                                                              \
+        if (!(cond)) return ::cm::fail(code, __VA_ARGS__);                            \
+    } while (0)
+
+#define CM_TRY(expr)                                                                  \
+    do {                                                                              \
+        int s_ = (expr);                                                              \
+        if (s_ != CM_OK) return s_;                                                   \
+    } while (0)
+
+// Entry-point guard: the C ABI never lets an exception escape.
+#define CM_API_BEGIN try {
+#define CM_API_END                                                                    \
+    }                                                                                 \
+    catch (const std::exception& ex) { return ::cm::fail(CM_E_STATE, "%s", ex.what()); } \
+    catch (...) { return ::cm::fail(CM_E_STATE, "unknown exception"); }
+
+// ---------------------------------------------------------------- counters
+void count(int counter, long long v = 1);
+
+// ----------------------------------------------------------------- context
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace cm
+
+struct cm_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+    // Device scratch (grow-only) for reduction partials and solver state.
+    cm::Scratch scratch[4];
+    // Pinned host word(s) for flag read-back.
+    void* pinned = nullptr;
+    // Cached device properties.
+    int cu_count = 0;
+};
+
+namespace cm {
+
+// Resolve a possibly-NULL context to the thread's default one and make its
+// device current.
+int resolve(cm_ctx_t in, cm_ctx_t* out);
+// Zero device memory with a fill kernel (capture-safe).
+int fill_zero(cm_ctx_t ctx, void* p, size_t bytes);
+// Grow-only scratch slot 'slot' of at least 'bytes' bytes (256-aligned).
+int scratch(cm_ctx_t ctx, int slot, size_t bytes, void** out);
+// Pinned host memory of the context (at least 256 bytes).
+int pinned(cm_ctx_t ctx, void** out);
+// Caching device allocator (the same pool as cm_malloc / cm_free).
+int dalloc(cm_ctx_t ctx, size_t bytes, void** out);
+int dfree(cm_ctx_t ctx, void* p);
+
+// RAII holder for temporaries of one entry point; the blocks return to the
+// cache when the holder dies (after a stream sync, as the work may be queued).
+struct Temps {
+    cm_ctx_t ctx;
+    std::vector<void*> ptrs;
+    explicit Temps(cm_ctx_t c) : ctx(c) {}
+    int get(size_t bytes, void** out)
+    {
+        int s = dalloc(ctx, bytes, out);
+        if (s == CM_OK) ptrs.push_back(*out);
+        return s;
+    }
+    ~Temps()
+    {
+        if (ptrs.empty()) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        for (void* p : ptrs) dfree(ctx, p);
+    }
+};
+
+inline size_t dtype_size(int dt)
+{
+    switch (dt) {
+    case CM_I32: return 4;
+    case CM_F32: return 4;
+    case CM_F64: return 8;
+    case CM_BOOL: return 1;
+    case CM_BF16: return 2;
+    default: return 0;
+    }
+}
+
+inline unsigned div_up(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+// -------------------------------------------------------------- device side
+// DPP control words (GCN/CDNA ISA).
+enum : int {
+    DPP_QUAD_XOR1 = 0xB1,        // quad_perm [1,0,3,2]
+    DPP_QUAD_XOR2 = 0x4E,        // quad_perm [2,3,0,1]
+    DPP_ROW_HALF_MIRROR = 0x141, // lane i <- lane 7-i within 8
+    DPP_ROW_MIRROR = 0x140,      // lane i <- lane 15-i within 16
+};
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v)
+{
+    return __int_as_float(dpp_i32<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v)
+{
+    return dpp_i32<CTRL>(v);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v)
+{
+    long long b = __double_as_longlong(v);
+    int lo = dpp_i32<CTRL>((int)(b & 0xffffffffll));
+    int hi = dpp_i32<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ float readlane(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double readlane(double v, int l)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Reduction operators (cuMat functor::Sum..BitwiseOr semantics).
+struct OpSum {
+    template <typename T> __device__ __forceinline__ static T apply(T a, T b) { return a + b; }
+};
+struct OpProd {
+    template <typename T> __device__ __forceinline__ static T apply(T a, T b) { return a * b; }
+};
+struct OpMin {
+    template <typename T> __device__ __forceinline__ static T apply(T a, T b) { return (a < b) ? a : b; }
+};
+struct OpMax {
+    template <typename T> __device__ __forceinline__ static T apply(T a, T b) { return (a > b) ? a : b; }
+};
+struct OpAll {
+    template <typename T> __device__ __forceinline__ static T apply(T a, T b) { return (T)((a != T(0)) && (b != T(0))); }
+};
+struct OpAny {
+    template <typename T> __device__ __forceinline__ static T apply(T a, T b) { return (T)((a != T(0)) || (b != T(0))); }
+};
+struct OpBand {
+    template <typename T> __device__ __forceinline__ static T apply(T a, T b) { return a & b; }
+};
+struct OpBor {
+    template <typename T> __device__ __forceinline__ static T apply(T a, T b) { return a | b; }
+};
+
+// Wrapping integer arithmetic: int32 sums/products are done on uint32.
+template <> __device__ __forceinline__ int OpSum::apply<int>(int a, int b)
+{
+    return (int)((unsigned)a + (unsigned)b);
+}
+template <> __device__ __forceinline__ int OpProd::apply<int>(int a, int b)
+{
+    return (int)((unsigned)a * (unsigned)b);
+}
+
+// Reduce within each row of 16 lanes; every lane of the row ends with the
+// row's result. The order of combination is fixed, so the result is
+// bit-reproducible.
+template <typename Op, typename T>
+__device__ __forceinline__ T row16_reduce(T v)
+{
+    v = Op::apply(v, dpp<DPP_QUAD_XOR1>(v));
+    v = Op::apply(v, dpp<DPP_QUAD_XOR2>(v));
+    v = Op::apply(v, dpp<DPP_ROW_HALF_MIRROR>(v));
+    v = Op::apply(v, dpp<DPP_ROW_MIRROR>(v));
+    return v;
+}
+
+// Full wave64 reduction; the result is returned (uniform) in every lane.
+// All 64 lanes must be active.
+template <typename Op, typename T>
+__device__ __forceinline__ T wave_reduce(T v)
+{
+    v = row16_reduce<Op>(v);
+    T a = readlane(v, 0), b = readlane(v, 16), c = readlane(v, 32), d = readlane(v, 48);
+    return Op::apply(Op::apply(a, b), Op::apply(c, d));
+}
+
+// Block reduction for blockDim.x == BS (multiple of 64). 'lds' needs BS/64
+// entries. Result valid in every thread. Contains two barriers.
+template <typename Op, int BS, typename T>
+__device__ __forceinline__ T block_reduce(T v, T* lds)
+{
+    constexpr int NW = BS / 64;
+    v = wave_reduce<Op>(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    T r = lds[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) r = Op::apply(r, lds[i]);
+    __syncthreads();
+    return r;
+}
+
+// Two simultaneous block sums (shares the barriers). lds needs 2*BS/64.
+template <int BS, typename T>
+__device__ __forceinline__ void block_sum2(T& a, T& b, T* lds)
+{
+    constexpr int NW = BS / 64;
+    a = wave_reduce<OpSum>(a);
+    b = wave_reduce<OpSum>(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { lds[w] = a; lds[NW + w] = b; }
+    __syncthreads();
+    T ra = lds[0], rb = lds[NW];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) { ra += lds[i]; rb += lds[NW + i]; }
+    __syncthreads();
+    a = ra;
+    b = rb;
+}
+
+// bf16 storage helpers (round-to-nearest-even on conversion from float).
+__device__ __forceinline__ float bf16_to_float(unsigned short h)
+{
+    return __uint_as_float(((unsigned)h) << 16);
+}
+
+// Value loader: converts stored type S to compute type T.
+template <typename S, typename T> struct Load {
+    __device__ __forceinline__ static T get(const S* p, long long i) { return (T)p[i]; }
+};
+template <typename T> struct Load<unsigned short, T> {
+    __device__ __forceinline__ static T get(const unsigned short* p, long long i)
+    {
+        return (T)bf16_to_float(p[i]);
+    }
+};
+
+// Apply an assignment mode (CwiseAssignmentHandler, cuMat/src/CwiseOp.h:15-62).
+template <typename T>
+__device__ __forceinline__ T assign_mode(int mode, T old, T v)
+{
+    switch (mode) {
+    case CM_ADD: return old + v;
+    case CM_SUB: return old - v;
+    case CM_MUL: return old * v;
+    case CM_DIV: return old / v;
+    default: return v;
+    }
+}
+template <>
+__device__ __forceinline__ int assign_mode<int>(int mode, int old, int v)
+{
+    switch (mode) {
+    case CM_ADD: return (int)((unsigned)old + (unsigned)v);
+    case CM_SUB: return (int)((unsigned)old - (unsigned)v);
+    case CM_MUL: return (int)((unsigned)old * (unsigned)v);
+    case CM_DIV: return old / v;
+    case CM_MOD: return old % v;
+    case CM_AND: return old & v;
+    case CM_OR: return old | v;
+    default: return v;
+    }
+}
+
+// "Last block done" protocol: every block publishes its partial, then takes a
+// ticket; the block that takes ticket nblocks-1 combines all partials in
+// block order (deterministic) and re-arms the counter for the next launch.
+__device__ __forceinline__ bool last_block_arrive(unsigned* counter, unsigned nblocks, int* lds_flag)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = atomicAdd(counter, 1u);
+        *lds_flag = (t == nblocks - 1);
+    }
+    __syncthreads();
+    bool last = *lds_flag != 0;
+    if (last) __threadfence();
+    return last;
+}
+
+}  // namespace cm

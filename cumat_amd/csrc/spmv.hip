@@ -1,0 +1,182 @@
+// spmv.hip -- CSR and ELLPACK sparse matrix-vector products and the Jacobi
+// diagonal extraction.
+//   CSR:     replaces CSRMVKernel_StaticBatches + ProductAssignment
+//            (cuMat/src/SparseProductEvaluation.h:43-139)
+//   ELLPACK: replaces ELLPACKMVKernel_StaticBatches
+//            (cuMat/src/SparseProductEvaluation.h:194-324)
+//   Jacobi:  replaces ExtractDiagonalOp over SparseMatrix::coeff +
+//            cwiseInverseCheck (cuMat/src/ConjugateGradient.h:228-230)
+#include "spmv_core.hpp"
+
+namespace cm {
+
+constexpr int SPMV_BS = 256;
+constexpr int SPMV_RPB = 512;
+constexpr int SPMV_CAP = 4096;
+
+template <typename VT, typename T>
+__global__ __launch_bounds__(SPMV_BS) void csr_spmv_kernel(int rows, int cols, int nnz,
+                                                           const int* __restrict__ JA,
+                                                           const int* __restrict__ IA,
+                                                           const VT* __restrict__ vals, int vals_batched,
+                                                           const T* __restrict__ x, int x_batched,
+                                                           T* __restrict__ y, int mode)
+{
+    __shared__ SpmvSmem<T, SPMV_RPB, SPMV_CAP> sm;
+    const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int r0 = (int)blk * SPMV_RPB;
+    if (r0 >= rows) return;
+    const int nr = min(SPMV_RPB, rows - r0);
+    const int b = blockIdx.y;
+    const VT* v = vals + (vals_batched ? (long long)b * nnz : 0);
+    const T* xb = x + (x_batched ? (long long)b * cols : 0);
+    T out[SPMV_RPB / SPMV_BS];
+    block_csr_spmv<VT, T, SPMV_BS, SPMV_RPB, SPMV_CAP>(JA, IA, v, xb, r0, nr, sm, out);
+    T* yb = y + (long long)b * rows;
+#pragma unroll
+    for (int j = 0; j < SPMV_RPB / SPMV_BS; ++j) {
+        int i = threadIdx.x + j * SPMV_BS;
+        if (i < nr) {
+            long long r = r0 + i;
+            yb[r] = (mode == CM_ASSIGN) ? out[j] : assign_mode<T>(mode, yb[r], out[j]);
+        }
+    }
+}
+
+template <typename T>
+__global__ void ell_spmv_kernel(int rows, int cols, int npr, const int* __restrict__ idx,
+                                const T* __restrict__ vals, int vals_batched,
+                                const T* __restrict__ x, int x_batched, T* __restrict__ y, int mode)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int b = blockIdx.y;
+    const T* v = vals + (vals_batched ? (long long)b * rows * npr : 0);
+    const T* xb = x + (x_batched ? (long long)b * cols : 0);
+    T acc = T(0);
+    for (int k = 0; k < npr; ++k) {
+        int c = idx[r + (long long)rows * k];   // column-major: coalesced across rows
+        if (c < 0) continue;
+        acc += v[r + (long long)rows * k] * xb[c];
+    }
+    T* yy = y + r + (long long)rows * b;
+    *yy = (mode == CM_ASSIGN) ? acc : assign_mode<T>(mode, *yy, acc);
+}
+
+template <typename T>
+__global__ void csr_inverse_diagonal_kernel(int rows, int nnz, const int* __restrict__ JA,
+                                            const int* __restrict__ IA, const T* __restrict__ vals,
+                                            int vals_batched, T* __restrict__ inv)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int b = blockIdx.y;
+    const T* v = vals + (vals_batched ? (long long)b * nnz : 0);
+    T d = T(0);
+    for (int k = JA[r]; k < JA[r + 1]; ++k)
+        if (IA[k] == r) { d = v[k]; break; }
+    inv[r + (long long)rows * b] = (d == T(0)) ? T(1) : T(1) / d;
+}
+
+template <typename VT, typename T>
+static int launch_csr(cm_ctx_t ctx, int rows, int cols, int nnz, int batches, const int* JA,
+                      const int* IA, const void* vals, int vb, const void* x, int xb, void* y, int mode)
+{
+    dim3 grid(div_up(rows, SPMV_RPB), batches);
+    csr_spmv_kernel<VT, T><<<grid, SPMV_BS, 0, ctx->stream>>>(rows, cols, nnz, JA, IA, (const VT*)vals, vb,
+                                                              (const T*)x, xb, (T*)y, mode);
+    CM_LAUNCH_CHECK();
+    return CM_OK;
+}
+
+}  // namespace cm
+
+extern "C" {
+
+int cm_csr_spmv(cm_ctx_t ctx, int val_dtype, int vec_dtype, int rows, int cols, int nnz, int batches,
+                const int* JA, const int* IA, const void* vals, int vals_batched, const void* x,
+                int x_batched, void* y, int mode)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && batches >= 1, CM_E_DIM,
+               "cm_csr_spmv: bad sizes rows=%d cols=%d nnz=%d batches=%d", rows, cols, nnz, batches);
+    CM_REQUIRE(mode >= CM_ASSIGN && mode <= CM_OR, CM_E_ARG, "cm_csr_spmv: bad mode %d", mode);
+    cm::count(CM_CNT_EVAL_MATMUL_SPARSE);
+    cm::count(CM_CNT_EVAL_ANY);
+    if (rows == 0) return CM_OK;
+    CM_REQUIRE(JA && y && (nnz == 0 || (IA && vals && x)), CM_E_ARG, "cm_csr_spmv: NULL pointer");
+    if (val_dtype == CM_F32 && vec_dtype == CM_F32)
+        return cm::launch_csr<float, float>(ctx, rows, cols, nnz, batches, JA, IA, vals, vals_batched, x,
+                                            x_batched, y, mode);
+    if (val_dtype == CM_F64 && vec_dtype == CM_F64)
+        return cm::launch_csr<double, double>(ctx, rows, cols, nnz, batches, JA, IA, vals, vals_batched, x,
+                                              x_batched, y, mode);
+    if (val_dtype == CM_I32 && vec_dtype == CM_I32)
+        return cm::launch_csr<int, int>(ctx, rows, cols, nnz, batches, JA, IA, vals, vals_batched, x,
+                                        x_batched, y, mode);
+    if (val_dtype == CM_BF16 && vec_dtype == CM_F32)
+        return cm::launch_csr<unsigned short, float>(ctx, rows, cols, nnz, batches, JA, IA, vals,
+                                                     vals_batched, x, x_batched, y, mode);
+    return cm::fail(CM_E_UNSUPPORTED, "cm_csr_spmv: unsupported dtype pair (%d,%d)", val_dtype, vec_dtype);
+    CM_API_END
+}
+
+int cm_ell_spmv(cm_ctx_t ctx, int dtype, int rows, int cols, int nnz_per_row, int batches,
+                const int* indices, const void* vals, int vals_batched, const void* x, int x_batched,
+                void* y, int mode)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(rows >= 0 && cols >= 0 && nnz_per_row >= 0 && batches >= 1, CM_E_DIM,
+               "cm_ell_spmv: bad sizes");
+    CM_REQUIRE(mode >= CM_ASSIGN && mode <= CM_OR, CM_E_ARG, "cm_ell_spmv: bad mode %d", mode);
+    cm::count(CM_CNT_EVAL_MATMUL_SPARSE);
+    cm::count(CM_CNT_EVAL_ANY);
+    if (rows == 0) return CM_OK;
+    dim3 grid(cm::div_up(rows, 256), batches);
+    if (dtype == CM_F32) {
+        cm::ell_spmv_kernel<float><<<grid, 256, 0, ctx->stream>>>(rows, cols, nnz_per_row, indices,
+                                                                   (const float*)vals, vals_batched,
+                                                                   (const float*)x, x_batched, (float*)y, mode);
+    } else if (dtype == CM_F64) {
+        cm::ell_spmv_kernel<double><<<grid, 256, 0, ctx->stream>>>(rows, cols, nnz_per_row, indices,
+                                                                    (const double*)vals, vals_batched,
+                                                                    (const double*)x, x_batched, (double*)y,
+                                                                    mode);
+    } else if (dtype == CM_I32) {
+        cm::ell_spmv_kernel<int><<<grid, 256, 0, ctx->stream>>>(rows, cols, nnz_per_row, indices,
+                                                                 (const int*)vals, vals_batched, (const int*)x,
+                                                                 x_batched, (int*)y, mode);
+    } else {
+        return cm::fail(CM_E_UNSUPPORTED, "cm_ell_spmv: unsupported dtype %d", dtype);
+    }
+    CM_LAUNCH_CHECK();
+    return CM_OK;
+    CM_API_END
+}
+
+int cm_csr_inverse_diagonal(cm_ctx_t ctx, int dtype, int rows, int nnz, int batches, const int* JA,
+                            const int* IA, const void* vals, int vals_batched, void* invdiag)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(rows >= 0 && batches >= 1, CM_E_DIM, "cm_csr_inverse_diagonal: bad sizes");
+    cm::count(CM_CNT_EVAL_CWISE);
+    cm::count(CM_CNT_EVAL_ANY);
+    if (rows == 0) return CM_OK;
+    dim3 grid(cm::div_up(rows, 256), vals_batched ? batches : 1);
+    if (dtype == CM_F32)
+        cm::csr_inverse_diagonal_kernel<float><<<grid, 256, 0, ctx->stream>>>(
+            rows, nnz, JA, IA, (const float*)vals, vals_batched, (float*)invdiag);
+    else if (dtype == CM_F64)
+        cm::csr_inverse_diagonal_kernel<double><<<grid, 256, 0, ctx->stream>>>(
+            rows, nnz, JA, IA, (const double*)vals, vals_batched, (double*)invdiag);
+    else
+        return cm::fail(CM_E_UNSUPPORTED, "cm_csr_inverse_diagonal: unsupported dtype %d", dtype);
+    CM_LAUNCH_CHECK();
+    return CM_OK;
+    CM_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// spmv_core.hpp -- block-level CSR SpMV used by the standalone SpMV entry
+// point and by the fused CG kernels.
+//
+// Layout of the work: a workgroup owns RPB consecutive rows. When the rows'
+// non-zeros fit in the LDS product buffer (always for stencil matrices),
+// the block streams IA/vals/x-gathers in CSR order with fully coalesced
+// loads, parks the products in LDS and then each thread sums its own row
+// sequentially in CSR order (the reference's order,
+// cuMat/src/SparseProductEvaluation.h:55-87). Blocks whose rows hold more
+// non-zeros than fit fall back to one wavefront per row with a fixed-order
+// DPP reduction.
+#pragma once
+#include "cm_common.hpp"
+
+namespace cm {
+
+// Map the hardware block index to a logical row block so that consecutive
+// row blocks run on the same XCD (hardware dispatches block i to XCD i%8):
+// stencil neighbours of a row block then hit the same L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned hw, unsigned nblocks)
+{
+    constexpr unsigned NXCD = 8;
+    if (nblocks < NXCD * 2) return hw;
+    unsigned per = nblocks / NXCD;          // blocks per XCD in the regular part
+    unsigned regular = per * NXCD;
+    if (hw >= regular) return hw;           // tail keeps its index
+    return (hw % NXCD) * per + hw / NXCD;
+}
+
+template <typename T, int RPB, int CAP>
+struct SpmvSmem {
+    T prod[CAP];
+    int rp[RPB + 1];
+};
+
+template <typename VT, typename T, int BS, int RPB, int CAP>
+__device__ __forceinline__ void block_csr_spmv_nodiag(const int* __restrict__ JA, const int* __restrict__ IA,
+                                                      const VT* __restrict__ vals,
+                                                      const T* __restrict__ x, int r0, int nr,
+                                                      SpmvSmem<T, RPB, CAP>& sm, T (&out)[RPB / BS]);
+
+// Computes y_local[i] = (A x)[r0 + i] for i < nr into 'out' (registers of the
+// calling thread: row r0 + tid + j*BS -> out[j]). VT: stored value type.
+// With 'diag' != NULL the matrix is diag[i] (kept in T) plus the stored
+// entries (whose diagonal slots hold 0): the mixed-precision matrices keep
+// their diagonal in FP32. Assumes blockDim.x == BS.
+template <typename VT, typename T, int BS, int RPB, int CAP>
+__device__ __forceinline__ void block_csr_spmv(const int* __restrict__ JA, const int* __restrict__ IA,
+                                               const VT* __restrict__ vals,
+                                               const T* __restrict__ x, int r0, int nr,
+                                               SpmvSmem<T, RPB, CAP>& sm, T (&out)[RPB / BS],
+                                               const T* __restrict__ diag = nullptr)
+{
+    block_csr_spmv_nodiag<VT, T, BS, RPB, CAP>(JA, IA, vals, x, r0, nr, sm, out);
+    if (diag) {
+#pragma unroll
+        for (int j = 0; j < RPB / BS; ++j) {
+            int i = threadIdx.x + j * BS;
+            if (i < nr) out[j] += diag[r0 + i] * x[r0 + i];
+        }
+    }
+}
+
+template <typename VT, typename T, int BS, int RPB, int CAP>
+__device__ __forceinline__ void block_csr_spmv_nodiag(const int* __restrict__ JA, const int* __restrict__ IA,
+                                                      const VT* __restrict__ vals,
+                                                      const T* __restrict__ x, int r0, int nr,
+                                                      SpmvSmem<T, RPB, CAP>& sm, T (&out)[RPB / BS])
+{
+    constexpr int RPT = RPB / BS;
+    const int tid = threadIdx.x;
+    for (int i = tid; i <= nr; i += BS) sm.rp[i] = JA[r0 + i];
+    __syncthreads();
+    const int s = sm.rp[0];
+    const int e = sm.rp[nr];
+    if (e - s <= CAP) {
+        constexpr int U = 4;
+        for (int base = s; base < e; base += BS * U) {
+            int col[U];
+            VT v[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                int k = base + tid + j * BS;
+                bool ok = k < e;
+                col[j] = ok ? IA[k] : 0;
+                v[j] = ok ? vals[k] : VT(0);
+            }
+            T xv[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                int k = base + tid + j * BS;
+                xv[j] = (k < e) ? x[col[j]] : T(0);
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                int k = base + tid + j * BS;
+                if (k < e) sm.prod[k - s] = Load<VT, T>::get(v, j) * xv[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            int i = tid + j * BS;
+            T acc = T(0);
+            if (i < nr) {
+                int a = sm.rp[i] - s, b = sm.rp[i + 1] - s;
+                if (a < b) {
+                    acc = sm.prod[a];
+                    for (int k = a + 1; k < b; ++k) acc += sm.prod[k];
+                }
+            }
+            out[j] = acc;
+        }
+    } else {
+        // long rows: one wavefront per row, lanes stride the row, fixed-order
+        // wave reduction; results parked in prod[] (RPB <= CAP).
+        const int lane = tid & 63, wave = tid >> 6;
+        for (int i = wave; i < nr; i += BS / 64) {
+            int a = sm.rp[i], b = sm.rp[i + 1];
+            T acc = T(0);
+            for (int k = a + lane; k < b; k += 64) acc += Load<VT, T>::get(vals, k) * x[IA[k]];
+            acc = wave_reduce<OpSum>(acc);
+            if (lane == 0) sm.prod[i] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            int i = tid + j * BS;
+            out[j] = (i < nr) ? sm.prod[i] : T(0);
+        }
+    }
+}
+
+}  // namespace cm

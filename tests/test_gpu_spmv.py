@@ -1,0 +1,176 @@
+"""GPU parity of the CSR / ELLPACK SpMV against the reference's fixtures and the
+CPU oracle (restating cuMat/src/SparseProductEvaluation.h:43-88,194-226)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_tests.json")))
+
+
+def _csr4x5(cm, batches=1, vscales=(1,)):
+    g = GOLD["csr_4x5"]
+    vals = np.concatenate([np.array(g["vals"], np.float32) * s for s in vscales])
+    pat = cm.SparsityPattern(4, 5, g["JA"], g["IA"])
+    return cm.SparseMatrix(pat, vals, batches=len(vscales)), g
+
+
+def test_csr_golden(gpu):
+    cm = gpu
+    A, g = _csr4x5(cm)
+    b = cm.DeviceArray.from_numpy(np.array(g["b"], np.float32))
+    cm.reset_counters()
+    y = A.matvec(b)
+    c = cm.counters()
+    assert c["EvalAny"] == 1 and c["EvalMatmulSparse"] == 1  # one kernel per product
+    assert np.array_equal(y.numpy(), np.array(g["y"], np.float32))
+
+
+@pytest.mark.parametrize("case", ["matrix", "vector", "both"])
+def test_csr_golden_batched(gpu, case):
+    cm = gpu
+    g = GOLD["csr_4x5"]
+    if case == "matrix":
+        ms, vs, es = g["batched_matrix_scales"], [1], g["batched_matrix_expected_scales"]
+    elif case == "vector":
+        ms, vs, es = [1], g["batched_vector_scales"], g["batched_vector_expected_scales"]
+    else:
+        ms, vs, es = g["batched_both_matrix_scales"], g["batched_both_vector_scales"], g["batched_both_expected_scales"]
+    A, _ = _csr4x5(cm, vscales=ms)
+    b = np.concatenate([np.array(g["b"], np.float32) * s for s in vs])
+    bd = cm.DeviceArray.from_numpy(b, rows=5, cols=1, batches=len(vs))
+    y = A.matvec(bd).numpy()
+    exp = np.concatenate([np.array(g["y"], np.float32) * s for s in es])
+    assert np.array_equal(y, exp)
+
+
+def test_ell_golden(gpu):
+    cm = gpu
+    g = GOLD["ell_4x5"]
+    E = cm.ELLMatrix(4, 5, g["indices_rows"], np.array(g["values_rows"], np.float32))
+    y = E.matvec(cm.DeviceArray.from_numpy(np.array(g["b"], np.float32))).numpy()
+    assert np.array_equal(y, np.array(g["y"], np.float32))
+
+
+@pytest.mark.parametrize("g", [1, 7, 64, 333])
+def test_poisson_integer_x_bitexact(gpu, orc, g):
+    """Integer-valued x makes every FP32 product and partial sum exact: the
+    GPU result must equal the oracle bit for bit."""
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    JA, IA, V = poisson2d(g)
+    n = g * g
+    x = np.random.default_rng(g).integers(-50, 51, n).astype(np.float32)
+    A = cm.SparseMatrix.from_csr(n, n, JA, IA, V)
+    y = A.matvec(cm.DeviceArray.from_numpy(x)).numpy()
+    assert np.array_equal(y, orc.csr_spmv(JA, IA, V, x, n, n))
+    # closed form: A * 1 = number of missing neighbours (Dirichlet rows)
+    y1 = A.matvec(cm.DeviceArray.from_numpy(np.ones(n, np.float32))).numpy()
+    JA2, IA2, V2 = orc.poisson2d(g)
+    assert np.array_equal(JA, JA2) and np.array_equal(IA, IA2)
+    assert np.array_equal(y1, 5.0 - np.diff(JA).astype(np.float32))
+
+
+def test_poisson_2048_full_size(gpu, orc):
+    """The benchmark matrix (n = 4,194,304, nnz = 20,963,328) at full size:
+    bit-exact against the oracle for integer x, and the oracle order is
+    reproduced exactly for random x (products are summed in CSR order)."""
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    g = 2048
+    JA, IA, V = poisson2d(g)
+    n = g * g
+    assert IA.size == 20963328
+    A = cm.SparseMatrix.from_csr(n, n, JA, IA, V)
+    rng = np.random.default_rng(5)
+    xi = rng.integers(-1000, 1001, n).astype(np.float32)
+    assert np.array_equal(A.matvec(cm.DeviceArray.from_numpy(xi)).numpy(), orc.csr_spmv(JA, IA, V, xi, n, n))
+    xr = rng.uniform(-1, 1, n).astype(np.float32)
+    y = A.matvec(cm.DeviceArray.from_numpy(xr)).numpy()
+    assert np.array_equal(y, orc.csr_spmv(JA, IA, V, xr, n, n))
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_spmv_modes(gpu, orc, mode):
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    JA, IA, V = poisson2d(20)
+    n = 400
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, n).astype(np.float32)
+    y0 = rng.uniform(-1, 1, n).astype(np.float32)
+    A = cm.SparseMatrix.from_csr(n, n, JA, IA, V)
+    yd = cm.DeviceArray.from_numpy(y0)
+    A.matvec(cm.DeviceArray.from_numpy(x), out=yd, mode=mode)
+    exp = orc.csr_spmv(JA, IA, V, x, n, n, y=y0.copy(), mode=mode)
+    assert np.array_equal(yd.numpy(), exp)
+
+
+def test_spmv_ragged_and_empty_rows(gpu, orc):
+    """Empty rows, one very long row (the wavefront-per-row fallback) and a
+    rectangular matrix."""
+    cm = gpu
+    rng = np.random.default_rng(11)
+    rows, cols = 3000, 9000
+    lens = rng.integers(0, 6, rows)
+    lens[::7] = 0
+    lens[1500] = 8000  # forces the long-row path for its block
+    JA = np.zeros(rows + 1, np.int32)
+    JA[1:] = np.cumsum(lens)
+    IA = np.concatenate([np.sort(rng.choice(cols, l, replace=False)) for l in lens]).astype(np.int32)
+    V = rng.uniform(-1, 1, IA.size).astype(np.float32)
+    x = rng.uniform(-1, 1, cols).astype(np.float32)
+    A = cm.SparseMatrix.from_csr(rows, cols, JA, IA, V)
+    yd = cm.DeviceArray.from_numpy(np.full(rows, 7.0, np.float32))  # garbage to be overwritten
+    y = A.matvec(cm.DeviceArray.from_numpy(x), out=yd).numpy()
+    exp = orc.csr_spmv(JA, IA, V, x, rows, cols)
+    assert np.all(y[lens == 0] == 0.0)
+    np.testing.assert_allclose(y, exp, rtol=1e-5, atol=1e-5 * np.abs(exp).max())
+    short = np.ones(rows, bool)
+    short[1024:1536] = False  # rows of the fallback block
+    assert np.array_equal(y[short], exp[short])
+
+
+def test_spmv_int32_and_f64(gpu, orc):
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    JA, IA, V = poisson2d(50)
+    n = 2500
+    rng = np.random.default_rng(2)
+    Vi = rng.integers(-1000, 1000, IA.size).astype(np.int32)
+    xi = rng.integers(-1000, 1000, n).astype(np.int32)
+    Ai = cm.SparseMatrix.from_csr(n, n, JA, IA, Vi)
+    assert np.array_equal(Ai.matvec(cm.DeviceArray.from_numpy(xi)).numpy(), orc.csr_spmv(JA, IA, Vi, xi, n, n))
+    Vd = rng.uniform(-1, 1, IA.size)
+    xd = rng.uniform(-1, 1, n)
+    Ad = cm.SparseMatrix.from_csr(n, n, JA, IA, Vd)
+    assert np.array_equal(Ad.matvec(cm.DeviceArray.from_numpy(xd)).numpy(), orc.csr_spmv(JA, IA, Vd, xd, n, n))
+
+
+def test_spmv_bf16_values(gpu, orc):
+    """BF16 matrix storage with FP32 vectors: exact against the oracle on the
+    bf16-rounded values."""
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    JA, IA, V = poisson2d(40)
+    n = 1600
+    rng = np.random.default_rng(9)
+    V = (V * rng.uniform(0.5, 2, V.size)).astype(np.float32)
+    bits = V.view(np.uint32)
+    bf = ((bits + 0x7FFF + ((bits >> 16) & 1)) >> 16).astype(np.uint16)
+    Vr = (bf.astype(np.uint32) << 16).view(np.float32)
+    x = rng.uniform(-1, 1, n).astype(np.float32)
+    pat = cm.SparsityPattern(n, n, JA, IA)
+    A = cm.SparseMatrix(pat, cm.DeviceArray.from_numpy(bf))
+    y = A.matvec(cm.DeviceArray.from_numpy(x)).numpy()
+    assert np.array_equal(y, orc.csr_spmv(JA, IA, Vr, x, n, n))
+
+
+def test_spmv_dimension_errors(gpu):
+    cm = gpu
+    A, g = _csr4x5(cm)
+    with pytest.raises(cm.CumatArgError):
+        A.matvec(cm.DeviceArray.from_numpy(np.ones(4, np.float32)))
