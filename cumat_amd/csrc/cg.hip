@@ -6,17 +6,14 @@
 // iteration. Here every scalar (alpha, beta, |r|^2, r.z, the convergence
 // test, the iteration count) stays on the device:
 //
-//  MULTIKERNEL path (any n, up to 16 right-hand sides):
+//  MULTIKERNEL path (any n, up to 16 right-hand sides; cg_multi.hpp):
 //    setup   r = b - A x, p = M^-1 r, |b|^2, |r|^2, r.p, early outs
-//    per iteration, three launches:
-//      K1  tmp = A p                 + partial p.tmp  -> last block: alpha
-//      K2  x += alpha p, r -= alpha tmp, z = M^-1 r (not stored)
-//                                     + partial |r|^2, r.z -> last block:
-//                                       convergence test, beta, counter
-//      K3  p = M^-1 r + beta p
-//    Kernels launched after convergence return at once; the host checks the
-//    status word every few iterations through pinned memory and stops
-//    launching.
+//    per iteration, three launches (K1 SpMV + p.Ap partials, K2 x/r update
+//    + |r|^2, r.z partials, K3 convergence test + p update); consumers
+//    re-reduce the producers' partials in a fixed order, so no device-scope
+//    fence is needed. Kernels launched after convergence return at once;
+//    the host checks the status word every few iterations through pinned
+//    memory and stops launching.
 //  RESIDENT path (n <= 32768, one right-hand side): one workgroup runs the
 //    whole solve; p lives in LDS, x / r / M^-1 in registers, the reductions
 //    are DPP wave reductions + LDS, and no kernel boundary is crossed.
@@ -24,7 +21,7 @@
 // Both follow the reference's semantics exactly: threshold tol^2 |b|^2,
 // early outs, the converging iteration is not counted, error =
 // sqrt(max_b |r|^2 / |b|^2), all batches iterate until all have converged.
-#include "spmv_core.hpp"
+#include "cg_multi.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -33,292 +30,6 @@
 #include <tuple>
 
 namespace cm {
-
-constexpr int CG_MAXB = 16;
-constexpr int CG_BS = 256;
-constexpr int CG_RPB = 512;    // rows per block of the SpMV kernel
-constexpr int CG_CAP = 4096;
-constexpr int CG_VPT = 4;      // vector elements per thread in K2/K3
-
-enum CgStatus : int { CG_RUNNING = 0, CG_CONVERGED = 1, CG_EARLY_ZERO = 2, CG_EARLY_GUESS = 3, CG_MAXIT = 4 };
-
-template <typename T>
-struct CgState {
-    T rhs2[CG_MAXB];
-    T thr[CG_MAXB];
-    T absNew[CG_MAXB];
-    T alpha[CG_MAXB];
-    T beta[CG_MAXB];
-    T rr[CG_MAXB];
-    int status;
-    int iters;
-    int max_iter;
-    int batches;
-    double err;
-    unsigned counter[4];
-};
-
-// ---------------------------------------------------------------- setup
-// r = b - A x (x = 0 when !use_guess), partial sums of b.b, r.r, r.(M^-1 r).
-// p = M^-1 r is written too (the first search direction).
-template <typename VT, typename T>
-__global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, int nnz, const int* __restrict__ JA,
-                                                         const int* __restrict__ IA, const VT* __restrict__ vals,
-                                                         const T* __restrict__ invd, const T* __restrict__ b,
-                                                         T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
-                                                         int use_guess, T tol, T* __restrict__ part,
-                                                         CgState<T>* st, const T* __restrict__ diag)
-{
-    __shared__ SpmvSmem<T, CG_RPB, CG_CAP> sm;
-    __shared__ T red[3 * CG_BS / 64];
-    __shared__ int flag;
-    const int blk = blockIdx.x;
-    const int bb = blockIdx.y;
-    const int r0 = blk * CG_RPB;
-    const int nr = min(CG_RPB, n - r0);
-    const long long off = (long long)bb * n;
-    T ax[CG_RPB / CG_BS];
-    if (use_guess) {
-        block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vals, x + off, r0, nr, sm, ax, diag);
-    } else {
-#pragma unroll
-        for (int j = 0; j < CG_RPB / CG_BS; ++j) ax[j] = T(0);
-    }
-    T sbb = 0, srr = 0, srp = 0;
-#pragma unroll
-    for (int j = 0; j < CG_RPB / CG_BS; ++j) {
-        int i = threadIdx.x + j * CG_BS;
-        if (i < nr) {
-            long long row = r0 + i;
-            T bv = b[off + row];
-            T rv = bv - ax[j];
-            T pv = invd[row] * rv;
-            if (!use_guess) x[off + row] = T(0);
-            r[off + row] = rv;
-            p[off + row] = pv;
-            sbb += bv * bv;
-            srr += rv * rv;
-            srp += rv * pv;
-        }
-    }
-    // block partials (3 sums)
-    sbb = wave_reduce<OpSum>(sbb);
-    srr = wave_reduce<OpSum>(srr);
-    srp = wave_reduce<OpSum>(srp);
-    constexpr int NW = CG_BS / 64;
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[w] = sbb; red[NW + w] = srr; red[2 * NW + w] = srp; }
-    __syncthreads();
-    const int nblk = gridDim.x;
-    if (threadIdx.x == 0) {
-        T a = 0, c = 0, d = 0;
-        for (int i = 0; i < NW; ++i) { a += red[i]; c += red[NW + i]; d += red[2 * NW + i]; }
-        long long base = (long long)bb * nblk + blk;
-        part[3 * base + 0] = a;
-        part[3 * base + 1] = c;
-        part[3 * base + 2] = d;
-    }
-    if (!last_block_arrive(&st->counter[0], gridDim.x * gridDim.y, &flag)) return;
-    // last block: totals per batch in block order, then the early outs
-    const int B = gridDim.y;
-    for (int k = 0; k < B; ++k) {
-        T a = 0, c = 0, d = 0;
-        for (int i = threadIdx.x; i < nblk; i += CG_BS) {
-            long long base = (long long)k * nblk + i;
-            a += part[3 * base + 0];
-            c += part[3 * base + 1];
-            d += part[3 * base + 2];
-        }
-        T a2 = a, c2 = c;
-        block_sum2<CG_BS>(a2, c2, red);
-        d = block_reduce<OpSum, CG_BS>(d, red);
-        if (threadIdx.x == 0) {
-            st->rhs2[k] = a2;
-            st->rr[k] = c2;
-            st->absNew[k] = d;
-            st->thr[k] = tol * tol * a2;
-            st->beta[k] = T(0);
-        }
-    }
-    if (threadIdx.x == 0) {
-        bool all_zero = true, all_conv = true;
-        T m = 0;
-        for (int k = 0; k < B; ++k) {
-            all_zero = all_zero && (st->rhs2[k] < tol * tol);
-            all_conv = all_conv && (st->rr[k] < st->thr[k]);
-            T q = st->rr[k] / st->rhs2[k];
-            m = (k == 0 || q > m) ? q : m;
-        }
-        st->iters = 0;
-        if (all_zero) { st->status = CG_EARLY_ZERO; st->err = 0.0; }
-        else if (all_conv) { st->status = CG_EARLY_GUESS; st->err = sqrt((double)m); }
-        else if (st->max_iter <= 0) { st->status = CG_MAXIT; st->err = sqrt((double)m); }
-        else st->status = CG_RUNNING;
-        st->counter[0] = 0;
-    }
-}
-
-// ---------------------------------------------------------------- K1
-template <typename VT, typename T>
-__global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
-                                                        const VT* __restrict__ vals, const T* __restrict__ p,
-                                                        T* __restrict__ tmp, T* __restrict__ part, CgState<T>* st,
-                                                        const T* __restrict__ diag)
-{
-    if (st->status != CG_RUNNING) return;
-    __shared__ SpmvSmem<T, CG_RPB, CG_CAP> sm;
-    __shared__ T red[CG_BS / 64];
-    __shared__ int flag;
-    const int nblk = gridDim.x;
-    const int blk = xcd_remap(blockIdx.x, nblk);
-    const int bb = blockIdx.y;
-    const int r0 = blk * CG_RPB;
-    const int nr = min(CG_RPB, n - r0);
-    const long long off = (long long)bb * n;
-    T ap[CG_RPB / CG_BS];
-    block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vals, p + off, r0, nr, sm, ap, diag);
-    T s = 0;
-#pragma unroll
-    for (int j = 0; j < CG_RPB / CG_BS; ++j) {
-        int i = threadIdx.x + j * CG_BS;
-        if (i < nr) {
-            long long row = off + r0 + i;
-            tmp[row] = ap[j];
-            s += p[row] * ap[j];
-        }
-    }
-    s = block_reduce<OpSum, CG_BS>(s, red);
-    if (threadIdx.x == 0) part[(long long)bb * nblk + blk] = s;
-    if (!last_block_arrive(&st->counter[1], gridDim.x * gridDim.y, &flag)) return;
-    const int B = gridDim.y;
-    for (int k = 0; k < B; ++k) {
-        T a = 0;
-        for (int i = threadIdx.x; i < nblk; i += CG_BS) a += part[(long long)k * nblk + i];
-        a = block_reduce<OpSum, CG_BS>(a, red);
-        if (threadIdx.x == 0) st->alpha[k] = st->absNew[k] / a;
-    }
-    if (threadIdx.x == 0) st->counter[1] = 0;
-}
-
-// ---------------------------------------------------------------- K2
-template <typename T>
-__global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, const T* __restrict__ invd, const T* __restrict__ p,
-                                                          const T* __restrict__ tmp, T* __restrict__ x,
-                                                          T* __restrict__ r, T* __restrict__ part, CgState<T>* st)
-{
-    if (st->status != CG_RUNNING) return;
-    __shared__ T red[2 * CG_BS / 64];
-    __shared__ int flag;
-    const int bb = blockIdx.y;
-    const long long off = (long long)bb * n;
-    const T alpha = st->alpha[bb];
-    T srr = 0, srz = 0;
-    const long long stride = (long long)gridDim.x * CG_BS;
-#pragma unroll
-    for (int j = 0; j < CG_VPT; ++j) {
-        long long i = (long long)blockIdx.x * CG_BS + threadIdx.x + j * stride;
-        if (i < n) {
-            T xv = x[off + i] + alpha * p[off + i];
-            T rv = r[off + i] - alpha * tmp[off + i];
-            x[off + i] = xv;
-            r[off + i] = rv;
-            srr += rv * rv;
-            srz += rv * (invd[i] * rv);
-        }
-    }
-    block_sum2<CG_BS>(srr, srz, red);
-    const int nblk = gridDim.x;
-    if (threadIdx.x == 0) {
-        long long base = (long long)bb * nblk + blockIdx.x;
-        part[2 * base] = srr;
-        part[2 * base + 1] = srz;
-    }
-    if (!last_block_arrive(&st->counter[2], gridDim.x * gridDim.y, &flag)) return;
-    const int B = gridDim.y;
-    __shared__ T rz_tot[CG_MAXB];
-    for (int k = 0; k < B; ++k) {
-        T a = 0, c = 0;
-        for (int i = threadIdx.x; i < nblk; i += CG_BS) {
-            long long base = (long long)k * nblk + i;
-            a += part[2 * base];
-            c += part[2 * base + 1];
-        }
-        block_sum2<CG_BS>(a, c, red);
-        if (threadIdx.x == 0) { st->rr[k] = a; rz_tot[k] = c; }
-    }
-    if (threadIdx.x == 0) {
-        bool all = true;
-        T m = 0;
-        for (int k = 0; k < B; ++k) {
-            all = all && (st->rr[k] < st->thr[k]);
-            T q = st->rr[k] / st->rhs2[k];
-            m = (k == 0 || q > m) ? q : m;
-        }
-        st->err = sqrt((double)m);
-        if (all) {
-            st->status = CG_CONVERGED;  // the converging step is not counted
-        } else {
-            for (int k = 0; k < B; ++k) {
-                T absOld = st->absNew[k];
-                st->absNew[k] = rz_tot[k];
-                st->beta[k] = rz_tot[k] / absOld;
-            }
-            st->iters += 1;
-            if (st->iters >= st->max_iter) st->status = CG_MAXIT;
-        }
-        st->counter[2] = 0;
-    }
-}
-
-// ---------------------------------------------------------------- K3
-template <typename T>
-__global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, const T* __restrict__ invd, const T* __restrict__ r,
-                                                           T* __restrict__ p, const CgState<T>* st)
-{
-    if (st->status != CG_RUNNING) return;
-    const int bb = blockIdx.y;
-    const long long off = (long long)bb * n;
-    const T beta = st->beta[bb];
-    const long long stride = (long long)gridDim.x * CG_BS;
-#pragma unroll
-    for (int j = 0; j < CG_VPT; ++j) {
-        long long i = (long long)blockIdx.x * CG_BS + threadIdx.x + j * stride;
-        if (i < n) p[off + i] = invd[i] * r[off + i] + beta * p[off + i];
-    }
-}
-
-template <typename T>
-__global__ void cg_finalize_kernel(long long total, T* x, const CgState<T>* st)
-{
-    if (st->status != CG_EARLY_ZERO) return;
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < total) x[i] = T(0);
-}
-
-template <typename T>
-__global__ void cg_state_init_kernel(CgState<T>* st, int max_iter, int batches)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->status = CG_RUNNING;
-        st->iters = 0;
-        st->max_iter = max_iter;
-        st->batches = batches;
-        st->err = 0.0;
-        for (int i = 0; i < 4; ++i) st->counter[i] = 0;
-    }
-}
-
-template <typename T>
-__global__ void invdiag_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA, const T* __restrict__ vals,
-                               T* __restrict__ inv)
-{
-    int rr = blockIdx.x * blockDim.x + threadIdx.x;
-    if (rr >= n) return;
-    T d = T(0);
-    for (int k = JA[rr]; k < JA[rr + 1]; ++k)
-        if (IA[k] == rr) { d = vals[k]; break; }
-    inv[rr] = (d == T(0)) ? T(1) : T(1) / d;
-}
 
 // ================================================================= resident
 // One workgroup of BS threads solves one system with n <= BS*RPT rows; p in
@@ -540,93 +251,6 @@ static int launch_resident(cm_ctx_t ctx, const ResidentArgs<T>& A, int nmax, int
     case 1024: return launch_resident_bs<T, 1024>(ctx, A, nmax, grid);
     default: return fail(CM_E_ARG, "resident CG: block size %d (256, 512 or 1024)", bs);
     }
-}
-
-template <typename VT, typename T>
-static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, const int* IA, const VT* vals,
-                           const T* invd, const T* b, T* x, int use_guess, double tol, int max_iter, int* iters_out,
-                           double* err_out, cm_cg_stats* stats, const T* diag = nullptr)
-{
-    constexpr bool same_type = std::is_same<VT, T>::value;
-    CM_REQUIRE(invd || same_type, CM_E_ARG, "CG: mixed storage needs an explicit inverse diagonal");
-    CM_REQUIRE(B >= 1 && B <= CG_MAXB, CM_E_UNSUPPORTED, "CG: %d right-hand sides (max %d)", B, CG_MAXB);
-    Temps tmp(ctx);
-    const long long N = (long long)n * B;
-    void *r_, *p_, *t_, *st_, *part_;
-    CM_TRY(tmp.get(sizeof(T) * N, &r_));
-    CM_TRY(tmp.get(sizeof(T) * N, &p_));
-    CM_TRY(tmp.get(sizeof(T) * N, &t_));
-    CM_TRY(tmp.get(sizeof(CgState<T>), &st_));
-    const int nblk_spmv = div_up(n, CG_RPB);
-    const int nblk_vec = div_up(n, (long long)CG_BS * CG_VPT);
-    const long long nparts = 3ll * B * std::max(nblk_spmv, nblk_vec);
-    CM_TRY(tmp.get(sizeof(T) * nparts, &part_));
-    T* r = (T*)r_;
-    T* p = (T*)p_;
-    T* t = (T*)t_;
-    T* part = (T*)part_;
-    CgState<T>* st = (CgState<T>*)st_;
-    if (!invd) {
-        void* id_;
-        CM_TRY(tmp.get(sizeof(T) * n, &id_));
-        // the matrix value type equals the vector type on this path
-        invdiag_kernel<T><<<div_up(n, 256), 256, 0, ctx->stream>>>(n, JA, IA, (const T*)vals, (T*)id_);
-        CM_LAUNCH_CHECK();
-        invd = (const T*)id_;
-    }
-    int launches = 0;
-    auto t0 = std::chrono::steady_clock::now();
-    cg_state_init_kernel<T><<<1, 64, 0, ctx->stream>>>(st, max_iter, B);
-    CM_LAUNCH_CHECK();
-    cg_setup_kernel<VT, T><<<dim3(nblk_spmv, B), CG_BS, 0, ctx->stream>>>(n, nnz, JA, IA, vals, invd, b, x, r, p,
-                                                                          use_guess, (T)tol, part, st, diag);
-    CM_LAUNCH_CHECK();
-    launches += 2;
-    int* hstat = nullptr;
-    CM_TRY(pinned(ctx, (void**)&hstat));
-    hstat[0] = 0;
-    hipEvent_t ev;
-    CM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    const int CHUNK = 8;
-    bool pending = false;
-    int it = 0;
-    while (it < max_iter) {
-        int todo = std::min(CHUNK, max_iter - it);
-        for (int k = 0; k < todo; ++k) {
-            cg_spmv_kernel<VT, T><<<dim3(nblk_spmv, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, p, t, part, st,
-                                                                                 diag);
-            cg_update_kernel<T><<<dim3(nblk_vec, B), CG_BS, 0, ctx->stream>>>(n, invd, p, t, x, r, part, st);
-            cg_pupdate_kernel<T><<<dim3(nblk_vec, B), CG_BS, 0, ctx->stream>>>(n, invd, r, p, st);
-        }
-        CM_LAUNCH_CHECK();
-        launches += 3 * todo;
-        it += todo;
-        // lagged status check: look at the previous chunk's word without
-        // draining the queue, then request the current one
-        if (pending) {
-            CM_HIP(hipEventSynchronize(ev));
-            if (hstat[0] != CG_RUNNING) break;
-        }
-        CM_HIP(hipMemcpyAsync(hstat, &st->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        CM_HIP(hipEventRecord(ev, ctx->stream));
-        pending = true;
-    }
-    cg_finalize_kernel<T><<<div_up(N, 256), 256, 0, ctx->stream>>>(N, x, st);
-    CM_LAUNCH_CHECK();
-    launches += 1;
-    CgState<T> hs;
-    CM_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipStreamSynchronize(ctx->stream));
-    CM_HIP(hipEventDestroy(ev));
-    auto t1 = std::chrono::steady_clock::now();
-    if (iters_out) *iters_out = hs.iters;
-    if (err_out) *err_out = hs.err;
-    if (stats) {
-        stats->path_used = CM_CG_MULTIKERNEL;
-        stats->launches = launches;
-        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
-    }
-    return CM_OK;
 }
 
 template <typename T>

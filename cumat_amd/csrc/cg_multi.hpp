@@ -1,0 +1,397 @@
+// cg_multi.hpp -- the multi-kernel Jacobi-PCG (any n, up to CG_MAXB
+// right-hand sides), fence-free.
+//
+// Per iteration i, three launches:
+//   K1  tmp = A p, per-block partial p.tmp                    (G1 partials)
+//   K2  every block reduces K1's partials in the same fixed order -> the same
+//       alpha = absNew[i&1] / p.tmp; x += alpha p, r -= alpha tmp;
+//       per-block partials of |r|^2 and r.(M^-1 r)          (2*G2 partials)
+//   K3  every block reduces K2's partials -> convergence test and
+//       beta = rz / absNew[i&1]; p = M^-1 r + beta p; block 0 records
+//       absNew[(i+1)&1] = rz, the status, the iteration count and the error.
+// Cross-block data only flows through kernel boundaries (no device-scope
+// fences, which on a multi-XCD part write back the L2 each time), and every
+// reduction is done in one fixed order, so runs are bit-reproducible.
+#pragma once
+#include "spmv_core.hpp"
+
+namespace cm {
+
+constexpr int CG_MAXB = 16;
+constexpr int CG_BS = 256;
+constexpr int CG_RPB = 512;    // rows per block of the SpMV phase
+constexpr int CG_CAP = 4096;   // staged non-zeros per row block
+constexpr int CG_VPT = 4;      // vector elements per thread per sweep
+constexpr int CG_GMAX = 2048;  // max blocks (= max partials) per kernel
+
+enum CgStatus : int { CG_RUNNING = 0, CG_CONVERGED = 1, CG_EARLY_ZERO = 2, CG_EARLY_GUESS = 3, CG_MAXIT = 4 };
+
+template <typename T>
+struct CgState {
+    T rhs2[CG_MAXB];
+    T thr[CG_MAXB];
+    T absNew[2][CG_MAXB];
+    int status;
+    int iters;
+    int max_iter;
+    int batches;
+    double err;
+};
+
+// Row blocks [0, NB) are split into 8 contiguous ranges, one per XCD;
+// hardware block h (dispatched to XCD h % 8) walks its XCD's range with
+// stride G/8, so stencil neighbours of a row block share an L2.
+struct RowBlockWalk {
+    int lo, hi, step, cur;
+    __device__ RowBlockWalk(int h, int G, int NB)
+    {
+        const int x = h & 7, l = h >> 3;
+        lo = (int)(((long long)NB * x) >> 3);
+        hi = (int)(((long long)NB * (x + 1)) >> 3);
+        step = G >> 3;
+        cur = lo + l;
+    }
+    __device__ bool valid() const { return cur < hi; }
+    __device__ void next() { cur += step; }
+};
+
+// Sum of G partials (stride 'stride' between consecutive ones) in a fixed
+// order; identical in every block. lds: BS/64 entries.
+template <typename T>
+__device__ __forceinline__ T reduce_partials(const T* __restrict__ part, int G, int stride, T* lds)
+{
+    T a = T(0);
+    for (int i = threadIdx.x; i < G; i += CG_BS) a += part[(long long)i * stride];
+    return block_reduce<OpSum, CG_BS>(a, lds);
+}
+
+// ---------------------------------------------------------------- setup
+template <typename VT, typename T>
+__global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
+                                                         const VT* __restrict__ vals, const T* __restrict__ invd,
+                                                         const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r,
+                                                         T* __restrict__ p, int use_guess, T* __restrict__ part,
+                                                         const T* __restrict__ diag)
+{
+    __shared__ SpmvSmem<T, CG_RPB, CG_CAP> sm;
+    __shared__ T red[2 * CG_BS / 64];
+    const int G = gridDim.x;
+    const int NB = div_up_dev(n, CG_RPB);
+    const int bb = blockIdx.y;
+    const long long off = (long long)bb * n;
+    T sbb = 0, srr = 0, srp = 0;
+    for (RowBlockWalk w(blockIdx.x, G, NB); w.valid(); w.next()) {
+        const int r0 = w.cur * CG_RPB;
+        const int nr = min(CG_RPB, n - r0);
+        T ax[CG_RPB / CG_BS];
+        if (use_guess) {
+            block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vals, x + off, r0, nr, sm, ax, diag);
+            __syncthreads();  // sm is reused by the next row block
+        } else {
+#pragma unroll
+            for (int j = 0; j < CG_RPB / CG_BS; ++j) ax[j] = T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < CG_RPB / CG_BS; ++j) {
+            int i = threadIdx.x + j * CG_BS;
+            if (i < nr) {
+                long long row = r0 + i;
+                T bv = b[off + row];
+                T rv = bv - ax[j];
+                T pv = invd[row] * rv;
+                if (!use_guess) x[off + row] = T(0);
+                r[off + row] = rv;
+                p[off + row] = pv;
+                sbb += bv * bv;
+                srr += rv * rv;
+                srp += rv * pv;
+            }
+        }
+    }
+    block_sum2<CG_BS>(sbb, srr, red);
+    srp = block_reduce<OpSum, CG_BS>(srp, red);
+    if (threadIdx.x == 0) {
+        long long base = 3ll * ((long long)bb * G + blockIdx.x);
+        part[base + 0] = sbb;
+        part[base + 1] = srr;
+        part[base + 2] = srp;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(CG_BS) void cg_setup_finalize_kernel(int G, int B, T tol, const T* __restrict__ part,
+                                                                  CgState<T>* st)
+{
+    __shared__ T red[CG_BS / 64];
+    __shared__ T tot[3][CG_MAXB];
+    for (int k = 0; k < B; ++k)
+        for (int c = 0; c < 3; ++c) {
+            T v = reduce_partials(part + 3ll * k * G + c, G, 3, red);
+            if (threadIdx.x == 0) tot[c][k] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bool all_zero = true, all_conv = true;
+        T m = 0;
+        for (int k = 0; k < B; ++k) {
+            st->rhs2[k] = tot[0][k];
+            st->thr[k] = tol * tol * tot[0][k];
+            st->absNew[0][k] = tot[2][k];
+            all_zero = all_zero && (tot[0][k] < tol * tol);
+            all_conv = all_conv && (tot[1][k] < st->thr[k]);
+            T q = tot[1][k] / tot[0][k];
+            m = (k == 0 || q > m) ? q : m;
+        }
+        st->iters = 0;
+        st->batches = B;
+        if (all_zero) { st->status = CG_EARLY_ZERO; st->err = 0.0; }
+        else if (all_conv) { st->status = CG_EARLY_GUESS; st->err = sqrt((double)m); }
+        else if (st->max_iter <= 0) { st->status = CG_MAXIT; st->err = sqrt((double)m); }
+        else { st->status = CG_RUNNING; st->err = sqrt((double)m); }
+    }
+}
+
+// ---------------------------------------------------------------- K1
+template <typename VT, typename T>
+__global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
+                                                        const VT* __restrict__ vals, const T* __restrict__ p,
+                                                        T* __restrict__ tmp, T* __restrict__ part,
+                                                        const CgState<T>* st, const T* __restrict__ diag)
+{
+    if (st->status != CG_RUNNING) return;
+    __shared__ SpmvSmem<T, CG_RPB, CG_CAP> sm;
+    __shared__ T red[CG_BS / 64];
+    const int G = gridDim.x;
+    const int NB = div_up_dev(n, CG_RPB);
+    const int bb = blockIdx.y;
+    const long long off = (long long)bb * n;
+    T s = 0;
+    for (RowBlockWalk w(blockIdx.x, G, NB); w.valid(); w.next()) {
+        const int r0 = w.cur * CG_RPB;
+        const int nr = min(CG_RPB, n - r0);
+        T ap[CG_RPB / CG_BS];
+        block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vals, p + off, r0, nr, sm, ap, diag);
+#pragma unroll
+        for (int j = 0; j < CG_RPB / CG_BS; ++j) {
+            int i = threadIdx.x + j * CG_BS;
+            if (i < nr) {
+                long long row = off + r0 + i;
+                tmp[row] = ap[j];
+                s += p[row] * ap[j];
+            }
+        }
+        __syncthreads();  // sm is reused by the next row block
+    }
+    s = block_reduce<OpSum, CG_BS>(s, red);
+    if (threadIdx.x == 0) part[(long long)bb * G + blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------- K2
+template <typename T>
+__global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1, const T* __restrict__ part1,
+                                                          const T* __restrict__ invd, const T* __restrict__ p,
+                                                          const T* __restrict__ tmp, T* __restrict__ x,
+                                                          T* __restrict__ r, T* __restrict__ part2,
+                                                          const CgState<T>* st)
+{
+    if (st->status != CG_RUNNING) return;
+    __shared__ T red[2 * CG_BS / 64];
+    const int bb = blockIdx.y;
+    const long long off = (long long)bb * n;
+    const T pap = reduce_partials(part1 + (long long)bb * G1, G1, 1, red);
+    const T alpha = st->absNew[it & 1][bb] / pap;
+    T srr = 0, srz = 0;
+    const long long stride = (long long)gridDim.x * CG_BS;
+    for (long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x; i0 < n; i0 += stride * CG_VPT) {
+#pragma unroll
+        for (int j = 0; j < CG_VPT; ++j) {
+            long long i = i0 + j * stride;
+            if (i < n) {
+                T xv = x[off + i] + alpha * p[off + i];
+                T rv = r[off + i] - alpha * tmp[off + i];
+                x[off + i] = xv;
+                r[off + i] = rv;
+                srr += rv * rv;
+                srz += rv * (invd[i] * rv);
+            }
+        }
+    }
+    block_sum2<CG_BS>(srr, srz, red);
+    if (threadIdx.x == 0) {
+        long long base = 2ll * ((long long)bb * gridDim.x + blockIdx.x);
+        part2[base] = srr;
+        part2[base + 1] = srz;
+    }
+}
+
+// ---------------------------------------------------------------- K3
+template <typename T>
+__global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B, int G2, const T* __restrict__ part2,
+                                                           const T* __restrict__ invd, const T* __restrict__ r,
+                                                           T* __restrict__ p, CgState<T>* st)
+{
+    if (st->status != CG_RUNNING) return;
+    __shared__ T red[CG_BS / 64];
+    __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB];
+    // every block computes the same totals (fixed order)
+    for (int k = 0; k < B; ++k) {
+        T a = reduce_partials(part2 + 2ll * k * G2, G2, 2, red);
+        T c = reduce_partials(part2 + 2ll * k * G2 + 1, G2, 2, red);
+        if (threadIdx.x == 0) { rr_s[k] = a; rz_s[k] = c; }
+    }
+    __syncthreads();
+    bool all = true;
+    T m = 0;
+    for (int k = 0; k < B; ++k) {
+        all = all && (rr_s[k] < st->thr[k]);
+        T q = rr_s[k] / st->rhs2[k];
+        m = (k == 0 || q > m) ? q : m;
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        st->err = sqrt((double)m);
+        if (all) {
+            st->status = CG_CONVERGED;  // the converging step is not counted
+            st->iters = it;
+        } else {
+            for (int k = 0; k < B; ++k) st->absNew[(it + 1) & 1][k] = rz_s[k];
+            st->iters = it + 1;
+            if (it + 1 >= st->max_iter) st->status = CG_MAXIT;
+        }
+    }
+    if (all) return;
+    const int bb = blockIdx.y;
+    const long long off = (long long)bb * n;
+    const T beta = rz_s[bb] / st->absNew[it & 1][bb];
+    const long long stride = (long long)gridDim.x * CG_BS;
+    for (long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x; i0 < n; i0 += stride * CG_VPT) {
+#pragma unroll
+        for (int j = 0; j < CG_VPT; ++j) {
+            long long i = i0 + j * stride;
+            if (i < n) p[off + i] = invd[i] * r[off + i] + beta * p[off + i];
+        }
+    }
+}
+
+template <typename T>
+__global__ void cg_finalize_kernel(long long total, T* x, const CgState<T>* st)
+{
+    if (st->status != CG_EARLY_ZERO) return;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) x[i] = T(0);
+}
+
+template <typename T>
+__global__ void cg_state_init_kernel(CgState<T>* st, int max_iter, int batches)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->status = CG_RUNNING;
+        st->iters = 0;
+        st->max_iter = max_iter;
+        st->batches = batches;
+        st->err = 0.0;
+    }
+}
+
+template <typename T>
+__global__ void invdiag_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA, const T* __restrict__ vals,
+                               T* __restrict__ inv)
+{
+    int rr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rr >= n) return;
+    T d = T(0);
+    for (int k = JA[rr]; k < JA[rr + 1]; ++k)
+        if (IA[k] == rr) { d = vals[k]; break; }
+    inv[rr] = (d == T(0)) ? T(1) : T(1) / d;
+}
+
+// ------------------------------------------------------------- host driver
+template <typename VT, typename T>
+static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, const int* IA, const VT* vals,
+                           const T* invd, const T* b, T* x, int use_guess, double tol, int max_iter, int* iters_out,
+                           double* err_out, cm_cg_stats* stats, const T* diag = nullptr)
+{
+    constexpr bool same_type = std::is_same<VT, T>::value;
+    CM_REQUIRE(invd || same_type, CM_E_ARG, "CG: mixed storage needs an explicit inverse diagonal");
+    CM_REQUIRE(B >= 1 && B <= CG_MAXB, CM_E_UNSUPPORTED, "CG: %d right-hand sides (max %d)", B, CG_MAXB);
+    Temps tmp(ctx);
+    const long long N = (long long)n * B;
+    void *r_, *p_, *t_, *st_, *p1_, *p2_;
+    CM_TRY(tmp.get(sizeof(T) * N, &r_));
+    CM_TRY(tmp.get(sizeof(T) * N, &p_));
+    CM_TRY(tmp.get(sizeof(T) * N, &t_));
+    CM_TRY(tmp.get(sizeof(CgState<T>), &st_));
+    const int NB = (int)div_up(n, CG_RPB);
+    // grids: multiples of 8 (one share per XCD), at most CG_GMAX blocks
+    const int G1 = std::max(8, std::min(CG_GMAX, ((NB + 7) / 8) * 8));
+    const int G2 = std::max(8, std::min(CG_GMAX, (int)(((div_up(n, (long long)CG_BS * CG_VPT)) + 7) / 8) * 8));
+    CM_TRY(tmp.get(sizeof(T) * 3 * (size_t)B * std::max(G1, G2), &p1_));
+    CM_TRY(tmp.get(sizeof(T) * 2 * (size_t)B * G2, &p2_));
+    T* r = (T*)r_;
+    T* p = (T*)p_;
+    T* t = (T*)t_;
+    T* part1 = (T*)p1_;
+    T* part2 = (T*)p2_;
+    CgState<T>* st = (CgState<T>*)st_;
+    if (!invd) {
+        void* id_;
+        CM_TRY(tmp.get(sizeof(T) * n, &id_));
+        invdiag_kernel<T><<<div_up(n, 256), 256, 0, ctx->stream>>>(n, JA, IA, (const T*)vals, (T*)id_);
+        CM_LAUNCH_CHECK();
+        invd = (const T*)id_;
+    }
+    int launches = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    cg_state_init_kernel<T><<<1, 64, 0, ctx->stream>>>(st, max_iter, B);
+    CM_LAUNCH_CHECK();
+    cg_setup_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, invd, b, x, r, p, use_guess,
+                                                                   part1, diag);
+    CM_LAUNCH_CHECK();
+    cg_setup_finalize_kernel<T><<<1, CG_BS, 0, ctx->stream>>>(G1, B, (T)tol, part1, st);
+    CM_LAUNCH_CHECK();
+    launches += 3;
+    int* hstat = nullptr;
+    CM_TRY(pinned(ctx, (void**)&hstat));
+    hstat[0] = 0;
+    hipEvent_t ev;
+    CM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    const int CHUNK = 8;
+    bool pending = false;
+    int it = 0;
+    while (it < max_iter) {
+        int todo = std::min(CHUNK, max_iter - it);
+        for (int k = 0; k < todo; ++k, ++it) {
+            cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, p, t, part1, st, diag);
+            cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, p, t, x, r, part2, st);
+            cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, B, G2, part2, invd, r, p, st);
+        }
+        CM_LAUNCH_CHECK();
+        launches += 3 * todo;
+        // lagged status check: read the previous chunk's word without draining
+        // the queue, then request the current one
+        if (pending) {
+            CM_HIP(hipEventSynchronize(ev));
+            if (hstat[0] != CG_RUNNING) break;
+        }
+        CM_HIP(hipMemcpyAsync(hstat, &st->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        CM_HIP(hipEventRecord(ev, ctx->stream));
+        pending = true;
+    }
+    cg_finalize_kernel<T><<<div_up(N, 256), 256, 0, ctx->stream>>>(N, x, st);
+    CM_LAUNCH_CHECK();
+    launches += 1;
+    CgState<T> hs;
+    CM_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_HIP(hipEventDestroy(ev));
+    auto t1 = std::chrono::steady_clock::now();
+    if (iters_out) *iters_out = hs.iters;
+    if (err_out) *err_out = hs.err;
+    if (stats) {
+        stats->path_used = CM_CG_MULTIKERNEL;
+        stats->launches = launches;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return CM_OK;
+}
+
+}  // namespace cm

@@ -129,6 +129,7 @@ inline size_t dtype_size(int dt)
 }
 
 inline unsigned div_up(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+__device__ __forceinline__ int div_up_dev(int a, int b) { return (a + b - 1) / b; }
 
 // -------------------------------------------------------------- device side
 // DPP control words (GCN/CDNA ISA).
@@ -312,23 +313,6 @@ __device__ __forceinline__ int assign_mode<int>(int mode, int old, int v)
     case CM_OR: return old | v;
     default: return v;
     }
-}
-
-// "Last block done" protocol: every block publishes its partial, then takes a
-// ticket; the block that takes ticket nblocks-1 combines all partials in
-// block order (deterministic) and re-arms the counter for the next launch.
-__device__ __forceinline__ bool last_block_arrive(unsigned* counter, unsigned nblocks, int* lds_flag)
-{
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(counter, 1u);
-        *lds_flag = (t == nblocks - 1);
-    }
-    __syncthreads();
-    bool last = *lds_flag != 0;
-    if (last) __threadfence();
-    return last;
 }
 
 }  // namespace cm

@@ -106,15 +106,15 @@ __global__ __launch_bounds__(RBS) void seg_wave_vec4_kernel(const T* __restrict_
 }
 
 // Block per chunk (long segments / full reductions). nchunk chunks of
-// 'chunk' elements per segment; partials combined by the last block of each
-// segment in chunk order.
+// 'chunk' elements per segment; with nchunk > 1 each block writes one
+// partial and seg_combine_kernel folds them in chunk order (a kernel
+// boundary instead of device-scope fences, which would write back the L2
+// of the XCD once per block).
 template <typename Op, typename T, typename S, typename Src>
 __global__ __launch_bounds__(RBS) void seg_block_kernel(Src src, long long L, long long seg_stride,
-                                                        int nchunk, long long chunk, T* partials,
-                                                        unsigned* counters, S* out)
+                                                        int nchunk, long long chunk, T* partials, S* out)
 {
     __shared__ T lds[RBS / 64];
-    __shared__ int flag;
     const long long s = blockIdx.x / nchunk;
     const int c = blockIdx.x % nchunk;
     const long long beg = c * chunk;
@@ -133,16 +133,18 @@ __global__ __launch_bounds__(RBS) void seg_block_kernel(Src src, long long L, lo
         return;
     }
     if (threadIdx.x == 0) partials[blockIdx.x] = acc;
-    if (!last_block_arrive(&counters[s], (unsigned)nchunk, &flag)) return;
-    // combine this segment's partials in chunk order
+}
+
+// One block per segment folds that segment's nchunk partials (fixed order).
+template <typename Op, typename T, typename S>
+__global__ __launch_bounds__(RBS) void seg_combine_kernel(const T* __restrict__ partials, int nchunk, S* out)
+{
+    __shared__ T lds[RBS / 64];
+    const long long s = blockIdx.x;
     T r = Identity<Op, T>::v();
     for (int k = threadIdx.x; k < nchunk; k += RBS) r = Op::apply(r, partials[s * nchunk + k]);
-    // each thread holds a strided subset; combine them in thread order
     r = block_reduce<Op, RBS>(r, lds);
-    if (threadIdx.x == 0) {
-        out[s] = (S)r;
-        counters[s] = 0;  // re-arm for the next launch
-    }
+    if (threadIdx.x == 0) out[s] = (S)r;
 }
 
 // ------------------------------------------------- strided columns
@@ -152,10 +154,9 @@ __global__ __launch_bounds__(RBS) void seg_block_kernel(Src src, long long L, lo
 template <typename Op, typename T, typename S>
 __global__ __launch_bounds__(RBS) void col_kernel(const S* __restrict__ in, long long P, long long Q,
                                                   long long K, long long st, long long PS, int ksplit,
-                                                  T* partials, unsigned* counters, S* out)
+                                                  T* partials, S* out)
 {
     __shared__ T lds[RBS];
-    __shared__ int flag;
     constexpr int NW = RBS / 64;
     const long long qtiles = (Q + 63) / 64;
     const long long tile = blockIdx.x / ksplit;  // (p, qtile)
@@ -189,13 +190,20 @@ __global__ __launch_bounds__(RBS) void col_kernel(const S* __restrict__ in, long
         return;
     }
     if (w == 0) partials[(long long)blockIdx.x * 64 + threadIdx.x] = acc;
-    if (!last_block_arrive(&counters[tile], (unsigned)ksplit, &flag)) return;
-    if (w == 0) {
-        T r = Identity<Op, T>::v();
-        for (int k2 = 0; k2 < ksplit; ++k2) r = Op::apply(r, partials[(tile * ksplit + k2) * 64 + threadIdx.x]);
-        if (q < Q) out[p * Q + q] = (S)r;
-        if (threadIdx.x == 0) counters[tile] = 0;
-    }
+}
+
+// Folds the ksplit partial rows of every (p, q-tile) in split order.
+template <typename Op, typename T, typename S>
+__global__ __launch_bounds__(64) void col_combine_kernel(const T* __restrict__ partials, long long Q, int ksplit,
+                                                         S* out)
+{
+    const long long qtiles = (Q + 63) / 64;
+    const long long tile = blockIdx.x;
+    const long long p = tile / qtiles;
+    const long long q = (tile % qtiles) * 64 + threadIdx.x;
+    T r = Identity<Op, T>::v();
+    for (int k2 = 0; k2 < ksplit; ++k2) r = Op::apply(r, partials[(tile * ksplit + k2) * 64 + threadIdx.x]);
+    if (q < Q) out[p * Q + q] = (S)r;
 }
 
 // ---------------------------------------------------------- host side
@@ -244,16 +252,16 @@ static int run_segments(cm_ctx_t ctx, Src src, long long L, long long nseg, S* o
     long long chunk = (L + nchunk - 1) / nchunk;
     nchunk = (L + chunk - 1) / chunk;
     void* part = nullptr;
-    void* cnt = nullptr;
-    if (nchunk > 1) {
-        CM_TRY(scratch(ctx, 1, sizeof(T) * nseg * nchunk, &part));
-        CM_TRY(scratch(ctx, 0, sizeof(unsigned) * nseg, &cnt));
-    }
+    if (nchunk > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * nseg * nchunk, &part));
     long long blocks = nseg * nchunk;
     CM_REQUIRE(blocks < (1ll << 31), CM_E_DIM, "reduction too large");
-    seg_block_kernel<Op, T, S, Src><<<(unsigned)blocks, RBS, 0, ctx->stream>>>(
-        src, L, L, (int)nchunk, chunk, (T*)part, (unsigned*)cnt, out);
+    seg_block_kernel<Op, T, S, Src><<<(unsigned)blocks, RBS, 0, ctx->stream>>>(src, L, L, (int)nchunk, chunk,
+                                                                               (T*)part, out);
     CM_LAUNCH_CHECK();
+    if (nchunk > 1) {
+        seg_combine_kernel<Op, T, S><<<(unsigned)nseg, RBS, 0, ctx->stream>>>((const T*)part, (int)nchunk, out);
+        CM_LAUNCH_CHECK();
+    }
     return CM_OK;
 }
 
@@ -274,16 +282,16 @@ static int run_columns(cm_ctx_t ctx, const S* in, long long P, long long Q, long
         if (ksplit > 1024) ksplit = 1024;
     }
     void* part = nullptr;
-    void* cnt = nullptr;
-    if (ksplit > 1) {
-        CM_TRY(scratch(ctx, 1, sizeof(T) * tiles * ksplit * 64, &part));
-        CM_TRY(scratch(ctx, 0, sizeof(unsigned) * tiles, &cnt));
-    }
+    if (ksplit > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * tiles * ksplit * 64, &part));
     long long blocks = tiles * ksplit;
     CM_REQUIRE(blocks < (1ll << 31), CM_E_DIM, "reduction too large");
-    col_kernel<Op, T, S><<<(unsigned)blocks, RBS, 0, ctx->stream>>>(in, P, Q, K, st, PS, (int)ksplit,
-                                                                      (T*)part, (unsigned*)cnt, out);
+    col_kernel<Op, T, S><<<(unsigned)blocks, RBS, 0, ctx->stream>>>(in, P, Q, K, st, PS, (int)ksplit, (T*)part,
+                                                                      out);
     CM_LAUNCH_CHECK();
+    if (ksplit > 1) {
+        col_combine_kernel<Op, T, S><<<(unsigned)tiles, 64, 0, ctx->stream>>>((const T*)part, Q, (int)ksplit, out);
+        CM_LAUNCH_CHECK();
+    }
     return CM_OK;
 }
 

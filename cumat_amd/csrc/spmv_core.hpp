@@ -30,7 +30,6 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned hw, unsigned nblocks)
 template <typename T, int RPB, int CAP>
 struct SpmvSmem {
     T prod[CAP];
-    int rp[RPB + 1];
 };
 
 template <typename VT, typename T, int BS, int RPB, int CAP>
@@ -68,46 +67,49 @@ __device__ __forceinline__ void block_csr_spmv_nodiag(const int* __restrict__ JA
                                                       SpmvSmem<T, RPB, CAP>& sm, T (&out)[RPB / BS])
 {
     constexpr int RPT = RPB / BS;
+    constexpr int EPT = CAP / BS;  // staged non-zeros per thread
     const int tid = threadIdx.x;
-    for (int i = tid; i <= nr; i += BS) sm.rp[i] = JA[r0 + i];
-    __syncthreads();
-    const int s = sm.rp[0];
-    const int e = sm.rp[nr];
+    // block range (uniform) and this thread's row bounds, all issued at once
+    const int s = JA[r0];
+    const int e = JA[r0 + nr];
+    int rb[RPT], re[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        int i = tid + j * BS;
+        rb[j] = (i < nr) ? JA[r0 + i] : 0;
+        re[j] = (i < nr) ? JA[r0 + i + 1] : 0;
+    }
     if (e - s <= CAP) {
-        constexpr int U = 4;
-        for (int base = s; base < e; base += BS * U) {
-            int col[U];
-            VT v[U];
+        // every index/value load of the block is in flight before the first
+        // x gather; then every gather before the first product
+        int col[EPT];
+        VT v[EPT];
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                int k = base + tid + j * BS;
-                bool ok = k < e;
-                col[j] = ok ? IA[k] : 0;
-                v[j] = ok ? vals[k] : VT(0);
-            }
-            T xv[U];
+        for (int j = 0; j < EPT; ++j) {
+            int k = s + tid + j * BS;
+            bool ok = k < e;
+            col[j] = ok ? __builtin_nontemporal_load(IA + k) : 0;
+            v[j] = ok ? __builtin_nontemporal_load(vals + k) : VT(0);
+        }
+        T xv[EPT];
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                int k = base + tid + j * BS;
-                xv[j] = (k < e) ? x[col[j]] : T(0);
-            }
+        for (int j = 0; j < EPT; ++j) {
+            int k = s + tid + j * BS;
+            xv[j] = (k < e) ? x[col[j]] : T(0);
+        }
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                int k = base + tid + j * BS;
-                if (k < e) sm.prod[k - s] = Load<VT, T>::get(v, j) * xv[j];
-            }
+        for (int j = 0; j < EPT; ++j) {
+            int k = s + tid + j * BS;
+            if (k < e) sm.prod[k - s] = Load<VT, T>::get(v, j) * xv[j];
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
-            int i = tid + j * BS;
             T acc = T(0);
-            if (i < nr) {
-                int a = sm.rp[i] - s, b = sm.rp[i + 1] - s;
-                if (a < b) {
-                    acc = sm.prod[a];
-                    for (int k = a + 1; k < b; ++k) acc += sm.prod[k];
-                }
+            int a = rb[j] - s, b = re[j] - s;
+            if (a < b) {
+                acc = sm.prod[a];
+                for (int k = a + 1; k < b; ++k) acc += sm.prod[k];
             }
             out[j] = acc;
         }
@@ -116,7 +118,7 @@ __device__ __forceinline__ void block_csr_spmv_nodiag(const int* __restrict__ JA
         // wave reduction; results parked in prod[] (RPB <= CAP).
         const int lane = tid & 63, wave = tid >> 6;
         for (int i = wave; i < nr; i += BS / 64) {
-            int a = sm.rp[i], b = sm.rp[i + 1];
+            int a = JA[r0 + i], b = JA[r0 + i + 1];
             T acc = T(0);
             for (int k = a + lane; k < b; k += 64) acc += Load<VT, T>::get(vals, k) * x[IA[k]];
             acc = wave_reduce<OpSum>(acc);
