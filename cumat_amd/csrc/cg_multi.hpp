@@ -14,13 +14,15 @@
 // reduction is done in one fixed order, so runs are bit-reproducible.
 #pragma once
 #include "spmv_core.hpp"
+#include "spmv_wave.hpp"
 
 namespace cm {
 
 constexpr int CG_MAXB = 16;
 constexpr int CG_BS = 256;
-constexpr int CG_RPB = 512;    // rows per block of the SpMV phase
-constexpr int CG_CAP = 4096;   // staged non-zeros per row block
+constexpr int CG_RPB = 512;    // rows per block of the setup SpMV
+constexpr int CG_CAP = 4096;   // staged non-zeros per setup row block
+constexpr int CG_EPT = 8;      // staged non-zeros per lane in K1
 constexpr int CG_VPT = 4;      // vector elements per thread per sweep
 constexpr int CG_GMAX = 2048;  // max blocks (= max partials) per kernel
 
@@ -36,23 +38,6 @@ struct CgState {
     int max_iter;
     int batches;
     double err;
-};
-
-// Row blocks [0, NB) are split into 8 contiguous ranges, one per XCD;
-// hardware block h (dispatched to XCD h % 8) walks its XCD's range with
-// stride G/8, so stencil neighbours of a row block share an L2.
-struct RowBlockWalk {
-    int lo, hi, step, cur;
-    __device__ RowBlockWalk(int h, int G, int NB)
-    {
-        const int x = h & 7, l = h >> 3;
-        lo = (int)(((long long)NB * x) >> 3);
-        hi = (int)(((long long)NB * (x + 1)) >> 3);
-        step = G >> 3;
-        cur = lo + l;
-    }
-    __device__ bool valid() const { return cur < hi; }
-    __device__ void next() { cur += step; }
 };
 
 // Sum of G partials (stride 'stride' between consecutive ones) in a fixed
@@ -159,34 +144,43 @@ __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __rest
                                                         const CgState<T>* st, const T* __restrict__ diag)
 {
     if (st->status != CG_RUNNING) return;
-    __shared__ SpmvSmem<T, CG_RPB, CG_CAP> sm;
+    constexpr int WPB = CG_BS / 64;
+    __shared__ T lds[WPB][CG_EPT * 64];
     __shared__ T red[CG_BS / 64];
-    const int G = gridDim.x;
-    const int NB = div_up_dev(n, CG_RPB);
+    T* slice = lds[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
     const int bb = blockIdx.y;
     const long long off = (long long)bb * n;
+    const T* pb = p + off;
     T s = 0;
-    for (RowBlockWalk w(blockIdx.x, G, NB); w.valid(); w.next()) {
-        const int r0 = w.cur * CG_RPB;
-        const int nr = min(CG_RPB, n - r0);
-        T ap[CG_RPB / CG_BS];
-        block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vals, p + off, r0, nr, sm, ap, diag);
-#pragma unroll
-        for (int j = 0; j < CG_RPB / CG_BS; ++j) {
-            int i = threadIdx.x + j * CG_BS;
-            if (i < nr) {
-                long long row = off + r0 + i;
-                tmp[row] = ap[j];
-                s += p[row] * ap[j];
+    ChunkWalk w(div_up_dev(n, 64), WPB);
+    if (w.valid()) {
+        WaveChunk<VT, CG_EPT> cur, nxt;
+        chunk_load(cur, JA, IA, vals, w.cur * 64, n);
+        for (;;) {
+            const int nc = w.peek_next();
+            const bool more = nc < w.hi;
+            if (more) chunk_load(nxt, JA, IA, vals, nc * 64, n);
+            T ap = chunk_compute<VT, T, CG_EPT>(cur, JA, IA, vals, [pb](int c) { return pb[c]; }, slice);
+            if (lane < cur.nr) {
+                const long long row = cur.r0 + lane;
+                const T pv = pb[row];
+                if (diag) ap += diag[row] * pv;
+                tmp[off + row] = ap;
+                s += pv * ap;
             }
+            if (!more) break;
+            cur = nxt;
+            w.next();
         }
-        __syncthreads();  // sm is reused by the next row block
     }
     s = block_reduce<OpSum, CG_BS>(s, red);
-    if (threadIdx.x == 0) part[(long long)bb * G + blockIdx.x] = s;
+    if (threadIdx.x == 0) part[(long long)bb * gridDim.x + blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------- K2
+// The first sweep's operands are loaded before the partials of K1 are
+// reduced, so the streaming loads overlap that reduction's latency.
 template <typename T>
 __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1, const T* __restrict__ part1,
                                                           const T* __restrict__ invd, const T* __restrict__ p,
@@ -198,23 +192,40 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1,
     __shared__ T red[2 * CG_BS / 64];
     const int bb = blockIdx.y;
     const long long off = (long long)bb * n;
+    const long long stride = (long long)gridDim.x * CG_BS;
+    long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
+    T xv[CG_VPT], pv[CG_VPT], rv[CG_VPT], tv[CG_VPT], dv[CG_VPT];
+    auto load = [&](long long base) {
+#pragma unroll
+        for (int j = 0; j < CG_VPT; ++j) {
+            const long long i = base + j * stride;
+            const bool ok = i < n;
+            xv[j] = ok ? x[off + i] : T(0);
+            pv[j] = ok ? p[off + i] : T(0);
+            rv[j] = ok ? r[off + i] : T(0);
+            tv[j] = ok ? tmp[off + i] : T(0);
+            dv[j] = ok ? invd[i] : T(0);
+        }
+    };
+    load(i0);
     const T pap = reduce_partials(part1 + (long long)bb * G1, G1, 1, red);
     const T alpha = st->absNew[it & 1][bb] / pap;
     T srr = 0, srz = 0;
-    const long long stride = (long long)gridDim.x * CG_BS;
-    for (long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x; i0 < n; i0 += stride * CG_VPT) {
+    while (i0 < n) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
-            long long i = i0 + j * stride;
+            const long long i = i0 + j * stride;
             if (i < n) {
-                T xv = x[off + i] + alpha * p[off + i];
-                T rv = r[off + i] - alpha * tmp[off + i];
-                x[off + i] = xv;
-                r[off + i] = rv;
-                srr += rv * rv;
-                srz += rv * (invd[i] * rv);
+                const T xn = xv[j] + alpha * pv[j];
+                const T rn = rv[j] - alpha * tv[j];
+                x[off + i] = xn;
+                r[off + i] = rn;
+                srr += rn * rn;
+                srz += rn * (dv[j] * rn);
             }
         }
+        i0 += stride * CG_VPT;
+        if (i0 < n) load(i0);
     }
     block_sum2<CG_BS>(srr, srz, red);
     if (threadIdx.x == 0) {
@@ -233,6 +244,22 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
     if (st->status != CG_RUNNING) return;
     __shared__ T red[CG_BS / 64];
     __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB];
+    const int bb = blockIdx.y;
+    const long long off = (long long)bb * n;
+    const long long stride = (long long)gridDim.x * CG_BS;
+    long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
+    T pv[CG_VPT], rv[CG_VPT], dv[CG_VPT];
+    auto load = [&](long long base) {
+#pragma unroll
+        for (int j = 0; j < CG_VPT; ++j) {
+            const long long i = base + j * stride;
+            const bool ok = i < n;
+            pv[j] = ok ? p[off + i] : T(0);
+            rv[j] = ok ? r[off + i] : T(0);
+            dv[j] = ok ? invd[i] : T(0);
+        }
+    };
+    load(i0);
     // every block computes the same totals (fixed order)
     for (int k = 0; k < B; ++k) {
         T a = reduce_partials(part2 + 2ll * k * G2, G2, 2, red);
@@ -247,6 +274,7 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
         T q = rr_s[k] / st->rhs2[k];
         m = (k == 0 || q > m) ? q : m;
     }
+    const T absOld = st->absNew[it & 1][bb];
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
         st->err = sqrt((double)m);
         if (all) {
@@ -259,16 +287,15 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
         }
     }
     if (all) return;
-    const int bb = blockIdx.y;
-    const long long off = (long long)bb * n;
-    const T beta = rz_s[bb] / st->absNew[it & 1][bb];
-    const long long stride = (long long)gridDim.x * CG_BS;
-    for (long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x; i0 < n; i0 += stride * CG_VPT) {
+    const T beta = rz_s[bb] / absOld;
+    while (i0 < n) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
-            long long i = i0 + j * stride;
-            if (i < n) p[off + i] = invd[i] * r[off + i] + beta * p[off + i];
+            const long long i = i0 + j * stride;
+            if (i < n) p[off + i] = dv[j] * rv[j] + beta * pv[j];
         }
+        i0 += stride * CG_VPT;
+        if (i0 < n) load(i0);
     }
 }
 
@@ -320,10 +347,11 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     CM_TRY(tmp.get(sizeof(T) * N, &p_));
     CM_TRY(tmp.get(sizeof(T) * N, &t_));
     CM_TRY(tmp.get(sizeof(CgState<T>), &st_));
-    const int NB = (int)div_up(n, CG_RPB);
-    // grids: multiples of 8 (one share per XCD), at most CG_GMAX blocks
-    const int G1 = std::max(8, std::min(CG_GMAX, ((NB + 7) / 8) * 8));
-    const int G2 = std::max(8, std::min(CG_GMAX, (int)(((div_up(n, (long long)CG_BS * CG_VPT)) + 7) / 8) * 8));
+    // grids: multiples of 8 (one share per XCD), at most CG_GMAX blocks;
+    // K1 walks 64-row chunks with 4 wavefronts per block, the setup walks
+    // 512-row blocks with the same grid
+    const int G1 = walk_grid(div_up(div_up(n, 64), CG_BS / 64), CG_GMAX);
+    const int G2 = walk_grid(div_up(n, (long long)CG_BS * CG_VPT), CG_GMAX);
     CM_TRY(tmp.get(sizeof(T) * 3 * (size_t)B * std::max(G1, G2), &p1_));
     CM_TRY(tmp.get(sizeof(T) * 2 * (size_t)B * G2, &p2_));
     T* r = (T*)r_;

@@ -7,12 +7,12 @@
 //   Jacobi:  replaces ExtractDiagonalOp over SparseMatrix::coeff +
 //            cwiseInverseCheck (cuMat/src/ConjugateGradient.h:228-230)
 #include "spmv_core.hpp"
+#include "spmv_wave.hpp"
 
 namespace cm {
 
 constexpr int SPMV_BS = 256;
-constexpr int SPMV_RPB = 512;
-constexpr int SPMV_CAP = 4096;
+constexpr int SPMV_EPT = 8;   // staged non-zeros per lane (8 per row on average)
 
 template <typename VT, typename T>
 __global__ __launch_bounds__(SPMV_BS) void csr_spmv_kernel(int rows, int cols, int nnz,
@@ -22,24 +22,31 @@ __global__ __launch_bounds__(SPMV_BS) void csr_spmv_kernel(int rows, int cols, i
                                                            const T* __restrict__ x, int x_batched,
                                                            T* __restrict__ y, int mode)
 {
-    __shared__ SpmvSmem<T, SPMV_RPB, SPMV_CAP> sm;
-    const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int r0 = (int)blk * SPMV_RPB;
-    if (r0 >= rows) return;
-    const int nr = min(SPMV_RPB, rows - r0);
+    constexpr int WPB = SPMV_BS / 64;
+    __shared__ T lds[WPB][SPMV_EPT * 64];
+    T* slice = lds[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
     const int b = blockIdx.y;
     const VT* v = vals + (vals_batched ? (long long)b * nnz : 0);
     const T* xb = x + (x_batched ? (long long)b * cols : 0);
-    T out[SPMV_RPB / SPMV_BS];
-    block_csr_spmv<VT, T, SPMV_BS, SPMV_RPB, SPMV_CAP>(JA, IA, v, xb, r0, nr, sm, out);
     T* yb = y + (long long)b * rows;
-#pragma unroll
-    for (int j = 0; j < SPMV_RPB / SPMV_BS; ++j) {
-        int i = threadIdx.x + j * SPMV_BS;
-        if (i < nr) {
-            long long r = r0 + i;
-            yb[r] = (mode == CM_ASSIGN) ? out[j] : assign_mode<T>(mode, yb[r], out[j]);
+    ChunkWalk w(div_up_dev(rows, 64), WPB);
+    if (!w.valid()) return;  // wave-uniform; no block barrier below
+    WaveChunk<VT, SPMV_EPT> cur, nxt;
+    chunk_load(cur, JA, IA, v, w.cur * 64, rows);
+    for (;;) {
+        const int nc = w.peek_next();
+        const bool more = nc < w.hi;
+        if (more) chunk_load(nxt, JA, IA, v, nc * 64, rows);  // in flight during the gathers below
+        const T acc = chunk_compute<VT, T, SPMV_EPT>(cur, JA, IA, v, [xb](int c) { return xb[c]; }, slice);
+        if (lane < cur.nr) {
+            T* yy = yb + cur.r0 + lane;
+            if (mode == CM_ASSIGN) __builtin_nontemporal_store(acc, yy);
+            else *yy = assign_mode<T>(mode, *yy, acc);
         }
+        if (!more) break;
+        cur = nxt;
+        w.next();
     }
 }
 
@@ -82,7 +89,13 @@ template <typename VT, typename T>
 static int launch_csr(cm_ctx_t ctx, int rows, int cols, int nnz, int batches, const int* JA,
                       const int* IA, const void* vals, int vb, const void* x, int xb, void* y, int mode)
 {
-    dim3 grid(div_up(rows, SPMV_RPB), batches);
+    // grid cap: 8 resident blocks per CU (the wave limit at 256 threads);
+    // CUMAT_SPMV_GRID overrides it for tuning sweeps
+    static const int gmax = [] {
+        const char* s = getenv("CUMAT_SPMV_GRID");
+        return (s && atoi(s) >= 8) ? atoi(s) : 2048;
+    }();
+    dim3 grid(walk_grid(div_up(div_up(rows, 64), SPMV_BS / 64), gmax), batches);
     csr_spmv_kernel<VT, T><<<grid, SPMV_BS, 0, ctx->stream>>>(rows, cols, nnz, JA, IA, (const VT*)vals, vb,
                                                               (const T*)x, xb, (T*)y, mode);
     CM_LAUNCH_CHECK();

@@ -27,6 +27,33 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned hw, unsigned nblocks)
     return (hw % NXCD) * per + hw / NXCD;
 }
 
+// Row blocks [0, NB) are split into 8 contiguous ranges, one per XCD;
+// hardware block h (dispatched to XCD h % 8) walks its XCD's range with
+// stride G/8 (G: grid size, a multiple of 8), so stencil neighbours of a row
+// block share an L2.
+struct RowBlockWalk {
+    int lo, hi, step, cur;
+    __device__ RowBlockWalk(int h, int G, int NB)
+    {
+        const int x = h & 7, l = h >> 3;
+        lo = (int)(((long long)NB * x) >> 3);
+        hi = (int)(((long long)NB * (x + 1)) >> 3);
+        step = G >> 3;
+        cur = lo + l;
+    }
+    __device__ bool valid() const { return cur < hi; }
+    __device__ void next() { cur += step; }
+};
+
+// Grid for a walk over NB row blocks: a multiple of 8, at most gmax.
+inline int walk_grid(long long NB, int gmax)
+{
+    long long g = ((NB + 7) / 8) * 8;
+    if (g > gmax) g = gmax;
+    if (g < 8) g = 8;
+    return (int)g;
+}
+
 template <typename T, int RPB, int CAP>
 struct SpmvSmem {
     T prod[CAP];
