@@ -100,6 +100,8 @@ _PROTOS = {
     "cm_csr_inverse_diagonal": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP], _I32),
     "cm_pcg_solve": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _VP, _VP, _VP, _I32, _F64, _I32,
                       _I32, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(CgStats)], _I32),
+    "cm_pcg_solve_ex": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP, _VP, _VP, _I32, _F64,
+                         _I32, _I32, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(CgStats)], _I32),
     "cm_pcg_batched_solve": ([_VP, _I32, _VP, _I32, _I32, _VP, _VP, _VP, _VP, _VP, C.c_float, _I32,
                               _VP, _VP, _I32, _I32], _I32),
     "cm_pcg_batched_autotune": ([_VP, _I32, _VP, _I32, _I32, _VP, _VP, _VP, _VP, C.c_float, _I32,

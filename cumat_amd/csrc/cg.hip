@@ -218,6 +218,7 @@ static int launch_resident_one(cm_ctx_t ctx, const ResidentArgs<T>& A, int grid,
         if (per_cu < 1) return fail(CM_E_UNSUPPORTED, "resident CG: tile does not fit on a CU");
         grid = std::min(A.nsys, per_cu * (ctx->cu_count > 0 ? ctx->cu_count : 256));
     }
+    if (grid <= 0) return CM_OK;  // nothing to solve (attribute set up only)
     k<<<grid, BS, lds, ctx->stream>>>(A);
     CM_LAUNCH_CHECK();
     return CM_OK;
@@ -307,12 +308,9 @@ struct BatchedPlan {
 
 static int make_plan(cm_ctx_t ctx, int nsys, const int* row_off_dev, const int* JA_dev, BatchedPlan& plan)
 {
+    (void)JA_dev;
     std::vector<int> ro(nsys + 1);
     CM_HIP(hipMemcpyAsync(ro.data(), row_off_dev, sizeof(int) * (nsys + 1), hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<int> ja(nsys + 1);
-    CM_HIP(hipStreamSynchronize(ctx->stream));
-    for (int s = 0; s <= nsys; ++s)
-        CM_HIP(hipMemcpyAsync(&ja[s], JA_dev + ro[s], sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     CM_HIP(hipStreamSynchronize(ctx->stream));
     plan.nmax = 0;
     std::vector<std::pair<long long, int>> cost(nsys);
@@ -320,9 +318,9 @@ static int make_plan(cm_ctx_t ctx, int nsys, const int* row_off_dev, const int* 
         int n = ro[s + 1] - ro[s];
         CM_REQUIRE(n >= 0, CM_E_ARG, "batched CG: row offsets decrease at system %d", s);
         plan.nmax = std::max(plan.nmax, n);
-        // CG iterations grow with the grid side ~ sqrt(n) for these systems;
-        // work per iteration ~ nnz: schedule the heaviest first
-        cost[s] = {-(long long)(ja[s + 1] - ja[s]) * (long long)(1 + (int)std::sqrt((double)n)), s};
+        // work per iteration ~ n and CG iterations ~ sqrt(n) for grid
+        // Laplacians: schedule the heaviest systems first (LPT order)
+        cost[s] = {-(long long)((double)n * std::sqrt((double)n)), s};
     }
     std::sort(cost.begin(), cost.end());
     plan.order.resize(nsys);
@@ -686,6 +684,54 @@ int cm_pcg_solve(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int
                                                    (const double*)invdiag, (const double*)b, (double*)x, use_guess,
                                                    tol, max_iter, iterations_out, error_out, stats);
     return cm::fail(CM_E_UNSUPPORTED, "cm_pcg_solve: dtype %d", dtype);
+    CM_API_END
+}
+
+int cm_pcg_solve_ex(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int* JA, const int* IA,
+                    const void* vals, int vals_batched, const void* invdiag, const void* b, void* x, int use_guess,
+                    double tol, int max_iter, int per_batch, int* iterations_each, double* error_each,
+                    cm_cg_stats* stats)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(n >= 0 && nnz >= 0 && batches >= 1 && batches <= cm::CG_MAXB, CM_E_DIM,
+               "cm_pcg_solve_ex: bad sizes (batches 1..%d)", cm::CG_MAXB);
+    CM_REQUIRE(tol >= 0, CM_E_ARG, "cm_pcg_solve_ex: negative tolerance");
+    if (max_iter < 0) max_iter = 2 * n;
+    if (n == 0) {
+        for (int k = 0; k < batches; ++k) {
+            if (iterations_each) iterations_each[k] = 0;
+            if (error_each) error_each[k] = 0;
+        }
+        return CM_OK;
+    }
+    CM_REQUIRE(JA && IA && vals && b && x, CM_E_ARG, "cm_pcg_solve_ex: NULL pointer");
+    cm::CgBatchOpts opt;
+    opt.vals_batched = vals_batched;
+    opt.per_batch = per_batch;
+    std::vector<int> its(batches, 0);
+    std::vector<double> errs(batches, 0.0);
+    opt.biters = its.data();
+    opt.berr = errs.data();
+    int it = 0;
+    double err = 0;
+    int s;
+    if (dtype == CM_F32)
+        s = cm::pcg_multikernel<float, float>(ctx, n, nnz, batches, JA, IA, (const float*)vals, (const float*)invdiag,
+                                              (const float*)b, (float*)x, use_guess, tol, max_iter, &it, &err, stats,
+                                              nullptr, opt);
+    else if (dtype == CM_F64)
+        s = cm::pcg_multikernel<double, double>(ctx, n, nnz, batches, JA, IA, (const double*)vals,
+                                                (const double*)invdiag, (const double*)b, (double*)x, use_guess, tol,
+                                                max_iter, &it, &err, stats, nullptr, opt);
+    else
+        return cm::fail(CM_E_UNSUPPORTED, "cm_pcg_solve_ex: dtype %d", dtype);
+    CM_TRY(s);
+    for (int k = 0; k < batches; ++k) {
+        if (iterations_each) iterations_each[k] = per_batch ? its[k] : it;
+        if (error_each) error_each[k] = per_batch ? errs[k] : err;
+    }
+    return CM_OK;
     CM_API_END
 }
 

@@ -2,23 +2,32 @@
 // right-hand sides), fence-free.
 //
 // Per iteration i, three launches:
-//   K1  tmp = A p, per-block partial p.tmp                    (G1 partials)
-//   K2  every block reduces K1's partials in the same fixed order -> the same
-//       alpha = absNew[i&1] / p.tmp; x += alpha p, r -= alpha tmp;
-//       per-block partials of |r|^2 and r.(M^-1 r)          (2*G2 partials)
-//   K3  every block reduces K2's partials -> convergence test and
-//       beta = rz / absNew[i&1]; p = M^-1 r + beta p; block 0 records
+//   K1  tmp = A p, per-block partial p.tmp
+//   K2  every block reduces K1's partials of its batch in the same fixed
+//       order -> the same alpha = absNew[i&1] / p.tmp; x += alpha p,
+//       r -= alpha tmp; per-block partials of |r|^2 and r.(M^-1 r)
+//   K3  every block reduces K2's partials of all batches -> convergence test
+//       and beta = rz / absNew[i&1]; p = M^-1 r + beta p; block 0 records
 //       absNew[(i+1)&1] = rz, the status, the iteration count and the error.
 // Cross-block data only flows through kernel boundaries (no device-scope
 // fences, which on a multi-XCD part write back the L2 each time), and every
 // reduction is done in one fixed order, so runs are bit-reproducible.
+//
+// Batches: the matrix is either shared (the reference's case: a batch-1
+// matrix with batched right-hand sides, cuMat/src/ConjugateGradient.h:223)
+// or has its own values per batch (vstride = nnz). Convergence is either the
+// reference's -- every batch iterates until all have converged
+// (cuMat/src/ConjugateGradient.h:43-45,190-193) -- or per batch: a converged
+// batch stops updating and keeps its own iteration count and error.
+// The grid shrinks as the batch count grows so that each reduction folds at
+// most CG_GMAX partials.
 #pragma once
 #include "spmv_core.hpp"
 #include "spmv_wave.hpp"
 
 namespace cm {
 
-constexpr int CG_MAXB = 16;
+constexpr int CG_MAXB = 64;
 constexpr int CG_BS = 256;
 constexpr int CG_RPB = 512;    // rows per block of the setup SpMV
 constexpr int CG_CAP = 4096;   // staged non-zeros per setup row block
@@ -33,10 +42,14 @@ struct CgState {
     T rhs2[CG_MAXB];
     T thr[CG_MAXB];
     T absNew[2][CG_MAXB];
+    int bdone[CG_MAXB];   // per-batch mode: CgStatus of each batch
+    int biters[CG_MAXB];
+    double berr[CG_MAXB];
     int status;
     int iters;
     int max_iter;
     int batches;
+    int per_batch;
     double err;
 };
 
@@ -50,10 +63,17 @@ __device__ __forceinline__ T reduce_partials(const T* __restrict__ part, int G, 
     return block_reduce<OpSum, CG_BS>(a, lds);
 }
 
+template <typename T>
+__device__ __forceinline__ bool batch_inactive(const CgState<T>* st, int bb)
+{
+    return st->status != CG_RUNNING || (st->per_batch && st->bdone[bb] != CG_RUNNING);
+}
+
 // ---------------------------------------------------------------- setup
 template <typename VT, typename T>
 __global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
-                                                         const VT* __restrict__ vals, const T* __restrict__ invd,
+                                                         const VT* __restrict__ vals, long long vstride,
+                                                         const T* __restrict__ invd, int istride,
                                                          const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r,
                                                          T* __restrict__ p, int use_guess, T* __restrict__ part,
                                                          const T* __restrict__ diag)
@@ -64,13 +84,15 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, const int* __res
     const int NB = div_up_dev(n, CG_RPB);
     const int bb = blockIdx.y;
     const long long off = (long long)bb * n;
+    const VT* vb = vals + bb * vstride;
+    const T* ib = invd + (long long)bb * istride;
     T sbb = 0, srr = 0, srp = 0;
     for (RowBlockWalk w(blockIdx.x, G, NB); w.valid(); w.next()) {
         const int r0 = w.cur * CG_RPB;
         const int nr = min(CG_RPB, n - r0);
         T ax[CG_RPB / CG_BS];
         if (use_guess) {
-            block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vals, x + off, r0, nr, sm, ax, diag);
+            block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vb, x + off, r0, nr, sm, ax, diag);
             __syncthreads();  // sm is reused by the next row block
         } else {
 #pragma unroll
@@ -83,7 +105,7 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, const int* __res
                 long long row = r0 + i;
                 T bv = b[off + row];
                 T rv = bv - ax[j];
-                T pv = invd[row] * rv;
+                T pv = ib[row] * rv;
                 if (!use_guess) x[off + row] = T(0);
                 r[off + row] = rv;
                 p[off + row] = pv;
@@ -116,21 +138,28 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_finalize_kernel(int G, int B, 
         }
     __syncthreads();
     if (threadIdx.x == 0) {
-        bool all_zero = true, all_conv = true;
+        bool all_zero = true, all_conv = true, all_done = true;
         T m = 0;
         for (int k = 0; k < B; ++k) {
             st->rhs2[k] = tot[0][k];
             st->thr[k] = tol * tol * tot[0][k];
             st->absNew[0][k] = tot[2][k];
-            all_zero = all_zero && (tot[0][k] < tol * tol);
-            all_conv = all_conv && (tot[1][k] < st->thr[k]);
+            const bool zero = tot[0][k] < tol * tol;
+            const bool conv = tot[1][k] < st->thr[k];
+            all_zero = all_zero && zero;
+            all_conv = all_conv && conv;
             T q = tot[1][k] / tot[0][k];
             m = (k == 0 || q > m) ? q : m;
+            // per-batch early outs (the plain method's, applied per system)
+            st->bdone[k] = zero ? CG_EARLY_ZERO : (conv ? CG_EARLY_GUESS : CG_RUNNING);
+            st->biters[k] = 0;
+            st->berr[k] = zero ? 0.0 : sqrt((double)q);
+            all_done = all_done && st->bdone[k] != CG_RUNNING;
         }
         st->iters = 0;
         st->batches = B;
-        if (all_zero) { st->status = CG_EARLY_ZERO; st->err = 0.0; }
-        else if (all_conv) { st->status = CG_EARLY_GUESS; st->err = sqrt((double)m); }
+        if (st->per_batch ? (all_done && all_zero) : all_zero) { st->status = CG_EARLY_ZERO; st->err = 0.0; }
+        else if (st->per_batch ? all_done : all_conv) { st->status = CG_EARLY_GUESS; st->err = sqrt((double)m); }
         else if (st->max_iter <= 0) { st->status = CG_MAXIT; st->err = sqrt((double)m); }
         else { st->status = CG_RUNNING; st->err = sqrt((double)m); }
     }
@@ -139,29 +168,31 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_finalize_kernel(int G, int B, 
 // ---------------------------------------------------------------- K1
 template <typename VT, typename T>
 __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
-                                                        const VT* __restrict__ vals, const T* __restrict__ p,
-                                                        T* __restrict__ tmp, T* __restrict__ part,
-                                                        const CgState<T>* st, const T* __restrict__ diag)
+                                                        const VT* __restrict__ vals, long long vstride,
+                                                        const T* __restrict__ p, T* __restrict__ tmp,
+                                                        T* __restrict__ part, const CgState<T>* st,
+                                                        const T* __restrict__ diag)
 {
-    if (st->status != CG_RUNNING) return;
+    const int bb = blockIdx.y;
+    if (batch_inactive(st, bb)) return;
     constexpr int WPB = CG_BS / 64;
     __shared__ T lds[WPB][CG_EPT * 64];
     __shared__ T red[CG_BS / 64];
     T* slice = lds[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
-    const int bb = blockIdx.y;
     const long long off = (long long)bb * n;
     const T* pb = p + off;
+    const VT* vb = vals + bb * vstride;
     T s = 0;
     ChunkWalk w(div_up_dev(n, 64), WPB);
     if (w.valid()) {
         WaveChunk<VT, CG_EPT> cur, nxt;
-        chunk_load(cur, JA, IA, vals, w.cur * 64, n);
+        chunk_load(cur, JA, IA, vb, w.cur * 64, n);
         for (;;) {
             const int nc = w.peek_next();
             const bool more = nc < w.hi;
-            if (more) chunk_load(nxt, JA, IA, vals, nc * 64, n);
-            T ap = chunk_compute<VT, T, CG_EPT>(cur, JA, IA, vals, [pb](int c) { return pb[c]; }, slice);
+            if (more) chunk_load(nxt, JA, IA, vb, nc * 64, n);
+            T ap = chunk_compute<VT, T, CG_EPT>(cur, JA, IA, vb, [pb](int c) { return pb[c]; }, slice);
             if (lane < cur.nr) {
                 const long long row = cur.r0 + lane;
                 const T pv = pb[row];
@@ -183,15 +214,16 @@ __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __rest
 // reduced, so the streaming loads overlap that reduction's latency.
 template <typename T>
 __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1, const T* __restrict__ part1,
-                                                          const T* __restrict__ invd, const T* __restrict__ p,
-                                                          const T* __restrict__ tmp, T* __restrict__ x,
-                                                          T* __restrict__ r, T* __restrict__ part2,
-                                                          const CgState<T>* st)
+                                                          const T* __restrict__ invd, int istride,
+                                                          const T* __restrict__ p, const T* __restrict__ tmp,
+                                                          T* __restrict__ x, T* __restrict__ r,
+                                                          T* __restrict__ part2, const CgState<T>* st)
 {
-    if (st->status != CG_RUNNING) return;
-    __shared__ T red[2 * CG_BS / 64];
     const int bb = blockIdx.y;
+    if (batch_inactive(st, bb)) return;
+    __shared__ T red[2 * CG_BS / 64];
     const long long off = (long long)bb * n;
+    const T* ib = invd + (long long)bb * istride;
     const long long stride = (long long)gridDim.x * CG_BS;
     long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
     T xv[CG_VPT], pv[CG_VPT], rv[CG_VPT], tv[CG_VPT], dv[CG_VPT];
@@ -204,7 +236,7 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1,
             pv[j] = ok ? p[off + i] : T(0);
             rv[j] = ok ? r[off + i] : T(0);
             tv[j] = ok ? tmp[off + i] : T(0);
-            dv[j] = ok ? invd[i] : T(0);
+            dv[j] = ok ? ib[i] : T(0);
         }
     };
     load(i0);
@@ -238,14 +270,20 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1,
 // ---------------------------------------------------------------- K3
 template <typename T>
 __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B, int G2, const T* __restrict__ part2,
-                                                           const T* __restrict__ invd, const T* __restrict__ r,
-                                                           T* __restrict__ p, CgState<T>* st)
+                                                           const T* __restrict__ invd, int istride,
+                                                           const T* __restrict__ r, T* __restrict__ p,
+                                                           CgState<T>* st)
 {
+    const int bb = blockIdx.y;
     if (st->status != CG_RUNNING) return;
+    const bool per_batch = st->per_batch != 0;
+    // a batch that finished in an earlier iteration has nothing to update;
+    // block (0,0) still runs, it owns the global status
+    if (per_batch && st->bdone[bb] != CG_RUNNING && !(blockIdx.x == 0 && bb == 0)) return;
     __shared__ T red[CG_BS / 64];
     __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB];
-    const int bb = blockIdx.y;
     const long long off = (long long)bb * n;
+    const T* ib = invd + (long long)bb * istride;
     const long long stride = (long long)gridDim.x * CG_BS;
     long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
     T pv[CG_VPT], rv[CG_VPT], dv[CG_VPT];
@@ -256,11 +294,12 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
             const bool ok = i < n;
             pv[j] = ok ? p[off + i] : T(0);
             rv[j] = ok ? r[off + i] : T(0);
-            dv[j] = ok ? invd[i] : T(0);
+            dv[j] = ok ? ib[i] : T(0);
         }
     };
     load(i0);
-    // every block computes the same totals (fixed order)
+    // every block computes the same totals (fixed order); the partials of a
+    // batch that finished earlier are stale and are not used
     for (int k = 0; k < B; ++k) {
         T a = reduce_partials(part2 + 2ll * k * G2, G2, 2, red);
         T c = reduce_partials(part2 + 2ll * k * G2 + 1, G2, 2, red);
@@ -269,24 +308,44 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
     __syncthreads();
     bool all = true;
     T m = 0;
+    bool have_m = false;
     for (int k = 0; k < B; ++k) {
-        all = all && (rr_s[k] < st->thr[k]);
-        T q = rr_s[k] / st->rhs2[k];
-        m = (k == 0 || q > m) ? q : m;
-    }
-    const T absOld = st->absNew[it & 1][bb];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        st->err = sqrt((double)m);
-        if (all) {
-            st->status = CG_CONVERGED;  // the converging step is not counted
-            st->iters = it;
-        } else {
-            for (int k = 0; k < B; ++k) st->absNew[(it + 1) & 1][k] = rz_s[k];
-            st->iters = it + 1;
-            if (it + 1 >= st->max_iter) st->status = CG_MAXIT;
+        const bool prev = per_batch && st->bdone[k] != CG_RUNNING;
+        all = all && (prev || rr_s[k] < st->thr[k]);
+        if (!prev) {
+            T q = rr_s[k] / st->rhs2[k];
+            m = (!have_m || q > m) ? q : m;
+            have_m = true;
         }
     }
-    if (all) return;
+    const bool own_prev = per_batch && st->bdone[bb] != CG_RUNNING;
+    const bool own_conv = own_prev || rr_s[bb] < st->thr[bb];
+    const T absOld = st->absNew[it & 1][bb];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (per_batch && !own_prev) {
+            st->berr[bb] = sqrt((double)(rr_s[bb] / st->rhs2[bb]));
+            if (own_conv) {
+                st->bdone[bb] = CG_CONVERGED;
+                st->biters[bb] = it;  // the converging step is not counted
+            } else {
+                st->biters[bb] = it + 1;
+            }
+        }
+        if (bb == 0) {
+            if (!per_batch) st->err = sqrt((double)m);
+            if (all) {
+                st->status = CG_CONVERGED;
+                st->iters = it;
+            } else {
+                st->iters = it + 1;
+                if (it + 1 >= st->max_iter) st->status = CG_MAXIT;
+            }
+        }
+        // with the reference's rule a converged batch keeps iterating until
+        // all have converged, so its r.z is recorded like any other
+        if (!(per_batch && own_conv)) st->absNew[(it + 1) & 1][bb] = rz_s[bb];
+    }
+    if (per_batch ? own_conv : all) return;
     const T beta = rz_s[bb] / absOld;
     while (i0 < n) {
 #pragma unroll
@@ -299,43 +358,56 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
     }
 }
 
+// x = 0 for a zero right-hand side (whole solve, or per batch).
 template <typename T>
-__global__ void cg_finalize_kernel(long long total, T* x, const CgState<T>* st)
+__global__ void cg_finalize_kernel(int n, T* x, const CgState<T>* st)
 {
-    if (st->status != CG_EARLY_ZERO) return;
+    const int bb = blockIdx.y;
+    const bool zero = st->status == CG_EARLY_ZERO || (st->per_batch && st->bdone[bb] == CG_EARLY_ZERO);
+    if (!zero) return;
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < total) x[i] = T(0);
+    if (i < n) x[(long long)bb * n + i] = T(0);
 }
 
 template <typename T>
-__global__ void cg_state_init_kernel(CgState<T>* st, int max_iter, int batches)
+__global__ void cg_state_init_kernel(CgState<T>* st, int max_iter, int batches, int per_batch)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->status = CG_RUNNING;
         st->iters = 0;
         st->max_iter = max_iter;
         st->batches = batches;
+        st->per_batch = per_batch;
         st->err = 0.0;
     }
 }
 
 template <typename T>
-__global__ void invdiag_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA, const T* __restrict__ vals,
-                               T* __restrict__ inv)
+__global__ void invdiag_kernel(int n, long long nnz, const int* __restrict__ JA, const int* __restrict__ IA,
+                               const T* __restrict__ vals, T* __restrict__ inv)
 {
     int rr = blockIdx.x * blockDim.x + threadIdx.x;
     if (rr >= n) return;
+    const T* v = vals + blockIdx.y * nnz;
     T d = T(0);
     for (int k = JA[rr]; k < JA[rr + 1]; ++k)
-        if (IA[k] == rr) { d = vals[k]; break; }
-    inv[rr] = (d == T(0)) ? T(1) : T(1) / d;
+        if (IA[k] == rr) { d = v[k]; break; }
+    inv[(long long)blockIdx.y * n + rr] = (d == T(0)) ? T(1) : T(1) / d;
 }
 
 // ------------------------------------------------------------- host driver
+struct CgBatchOpts {
+    int vals_batched = 0;   // matrix values per batch (stride nnz)
+    int per_batch = 0;      // per-batch convergence
+    int* biters = nullptr;  // host, B entries (per-batch mode)
+    double* berr = nullptr; // host, B entries (per-batch mode)
+};
+
 template <typename VT, typename T>
 static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, const int* IA, const VT* vals,
                            const T* invd, const T* b, T* x, int use_guess, double tol, int max_iter, int* iters_out,
-                           double* err_out, cm_cg_stats* stats, const T* diag = nullptr)
+                           double* err_out, cm_cg_stats* stats, const T* diag = nullptr,
+                           CgBatchOpts opt = CgBatchOpts())
 {
     constexpr bool same_type = std::is_same<VT, T>::value;
     CM_REQUIRE(invd || same_type, CM_E_ARG, "CG: mixed storage needs an explicit inverse diagonal");
@@ -347,11 +419,13 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     CM_TRY(tmp.get(sizeof(T) * N, &p_));
     CM_TRY(tmp.get(sizeof(T) * N, &t_));
     CM_TRY(tmp.get(sizeof(CgState<T>), &st_));
-    // grids: multiples of 8 (one share per XCD), at most CG_GMAX blocks;
-    // K1 walks 64-row chunks with 4 wavefronts per block, the setup walks
-    // 512-row blocks with the same grid
-    const int G1 = walk_grid(div_up(div_up(n, 64), CG_BS / 64), CG_GMAX);
-    const int G2 = walk_grid(div_up(n, (long long)CG_BS * CG_VPT), CG_GMAX);
+    // grids: multiples of 8 (one share per XCD); with B batches each gets at
+    // most CG_GMAX/B blocks so that every reduction folds <= CG_GMAX partials.
+    // K1 walks 64-row chunks with 4 wavefronts per block; the setup walks
+    // 512-row blocks with the same grid.
+    const int gcap = std::max(8, CG_GMAX / B);
+    const int G1 = walk_grid(div_up(div_up(n, 64), CG_BS / 64), gcap);
+    const int G2 = walk_grid(div_up(n, (long long)CG_BS * CG_VPT), gcap);
     CM_TRY(tmp.get(sizeof(T) * 3 * (size_t)B * std::max(G1, G2), &p1_));
     CM_TRY(tmp.get(sizeof(T) * 2 * (size_t)B * G2, &p2_));
     T* r = (T*)r_;
@@ -360,19 +434,26 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     T* part1 = (T*)p1_;
     T* part2 = (T*)p2_;
     CgState<T>* st = (CgState<T>*)st_;
+    const long long vstride = opt.vals_batched ? (long long)nnz : 0;
+    // a caller-given inverse diagonal has one row of n per batch when the
+    // values are batched
+    int istride = opt.vals_batched ? n : 0;
     if (!invd) {
         void* id_;
-        CM_TRY(tmp.get(sizeof(T) * n, &id_));
-        invdiag_kernel<T><<<div_up(n, 256), 256, 0, ctx->stream>>>(n, JA, IA, (const T*)vals, (T*)id_);
+        const int nb = opt.vals_batched ? B : 1;
+        CM_TRY(tmp.get(sizeof(T) * (size_t)n * nb, &id_));
+        invdiag_kernel<T><<<dim3(div_up(n, 256), nb), 256, 0, ctx->stream>>>(n, vstride, JA, IA, (const T*)vals,
+                                                                             (T*)id_);
         CM_LAUNCH_CHECK();
         invd = (const T*)id_;
+        istride = opt.vals_batched ? n : 0;
     }
     int launches = 0;
     auto t0 = std::chrono::steady_clock::now();
-    cg_state_init_kernel<T><<<1, 64, 0, ctx->stream>>>(st, max_iter, B);
+    cg_state_init_kernel<T><<<1, 64, 0, ctx->stream>>>(st, max_iter, B, opt.per_batch);
     CM_LAUNCH_CHECK();
-    cg_setup_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, invd, b, x, r, p, use_guess,
-                                                                   part1, diag);
+    cg_setup_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, invd, istride, b, x, r,
+                                                                   p, use_guess, part1, diag);
     CM_LAUNCH_CHECK();
     cg_setup_finalize_kernel<T><<<1, CG_BS, 0, ctx->stream>>>(G1, B, (T)tol, part1, st);
     CM_LAUNCH_CHECK();
@@ -388,9 +469,12 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     while (it < max_iter) {
         int todo = std::min(CHUNK, max_iter - it);
         for (int k = 0; k < todo; ++k, ++it) {
-            cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, p, t, part1, st, diag);
-            cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, p, t, x, r, part2, st);
-            cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, B, G2, part2, invd, r, p, st);
+            cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1, st,
+                                                                          diag);
+            cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, istride, p, t, x, r,
+                                                                        part2, st);
+            cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, B, G2, part2, invd, istride, r, p,
+                                                                         st);
         }
         CM_LAUNCH_CHECK();
         launches += 3 * todo;
@@ -404,7 +488,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         CM_HIP(hipEventRecord(ev, ctx->stream));
         pending = true;
     }
-    cg_finalize_kernel<T><<<div_up(N, 256), 256, 0, ctx->stream>>>(N, x, st);
+    cg_finalize_kernel<T><<<dim3(div_up(n, 256), B), 256, 0, ctx->stream>>>(n, x, st);
     CM_LAUNCH_CHECK();
     launches += 1;
     CgState<T> hs;
@@ -412,8 +496,23 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     CM_HIP(hipStreamSynchronize(ctx->stream));
     CM_HIP(hipEventDestroy(ev));
     auto t1 = std::chrono::steady_clock::now();
-    if (iters_out) *iters_out = hs.iters;
-    if (err_out) *err_out = hs.err;
+    if (opt.per_batch) {
+        double m = 0;
+        int mi = 0;
+        for (int k = 0; k < B; ++k) {
+            // a batch still running at the end stopped at the iteration limit
+            int itk = (hs.bdone[k] == CG_RUNNING) ? hs.iters : hs.biters[k];
+            if (opt.biters) opt.biters[k] = itk;
+            if (opt.berr) opt.berr[k] = hs.berr[k];
+            m = std::max(m, hs.berr[k]);
+            mi = std::max(mi, itk);
+        }
+        if (iters_out) *iters_out = mi;
+        if (err_out) *err_out = m;
+    } else {
+        if (iters_out) *iters_out = hs.iters;
+        if (err_out) *err_out = hs.err;
+    }
     if (stats) {
         stats->path_used = CM_CG_MULTIKERNEL;
         stats->launches = launches;

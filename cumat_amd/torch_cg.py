@@ -1,0 +1,125 @@
+"""Differentiable CG for PyTorch (no reference counterpart).
+
+x = solve(A(theta), b) on a family of variable-coefficient edge Laplacians
+(A_ii = sum of incident theta_e, A_ij = -theta_e, Dirichlet edges on the
+diagonal only). The backward pass needs one more CG solve with A^T = A and one
+fused per-edge kernel; only x and the assembled values are kept, not the
+iterates:
+    lam          = A^{-1} dL/dx
+    dL/db        = lam
+    dL/dtheta_e  = -lam^T (dA/dtheta_e) x = -(lam_i - lam_j)(x_i - x_j)
+                   (boundary edge: -lam_i x_i)
+Everything runs in the HIP library through ctypes on the caller's torch
+stream; torch only provides device memory and the stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _capi as capi
+
+_DT = {torch.float32: capi.F32, torch.float64: capi.F64}
+
+
+class _StreamCtx:
+    """One library context per device whose stream follows torch's current
+    stream."""
+    _by_dev: dict = {}
+
+    @classmethod
+    def get(cls, device: torch.device):
+        idx = device.index if device.index is not None else torch.cuda.current_device()
+        c = cls._by_dev.get(idx)
+        if c is None:
+            h = C.c_void_p()
+            capi.call("cm_ctx_create", idx, C.byref(h))
+            c = {"h": h.value, "stream": None}
+            cls._by_dev[idx] = c
+        s = torch.cuda.current_stream(idx).cuda_stream
+        if c["stream"] != s:
+            capi.call("cm_ctx_set_stream", c["h"], C.c_void_p(s))
+            c["stream"] = s
+        return c["h"]
+
+
+def _p(t: torch.Tensor):
+    return C.c_void_p(t.data_ptr())
+
+
+class EdgeLaplacian:
+    """Pattern and edge maps of A(theta) on one device (from
+    cumat_amd.problems.grid_laplacian)."""
+
+    def __init__(self, P: dict, device="cuda"):
+        dev = torch.device(device)
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.int32), device=dev)  # noqa: E731
+        self.n, self.nnz, self.nedges = int(P["n"]), int(P["nnz"]), int(P["nedges"])
+        self.JA, self.IA = t(P["JA"]), t(P["IA"])
+        self.eptr, self.eidx = t(P["eptr"]), t(P["eidx"])
+        self.ei, self.ej = t(P["ei"]), t(P["ej"])
+        self.device = dev
+
+    def assemble(self, theta: torch.Tensor) -> torch.Tensor:
+        """CSR values of A(theta) per batch: (B, nnz)."""
+        theta = theta.contiguous()
+        B = theta.shape[0]
+        vals = torch.empty(B, self.nnz, dtype=theta.dtype, device=theta.device)
+        h = _StreamCtx.get(theta.device)
+        capi.call("cm_edge_assemble", h, _DT[theta.dtype], self.nnz, B, self.nedges, _p(self.eptr),
+                  _p(self.eidx), _p(theta), _p(vals))
+        return vals
+
+    def solve(self, vals: torch.Tensor, rhs: torch.Tensor, tol: float, max_iter: int):
+        """Per-batch Jacobi-PCG with batch-specific values; returns
+        (x, iterations per batch, error per batch)."""
+        vals, rhs = vals.contiguous(), rhs.contiguous()
+        B = rhs.shape[0]
+        x = torch.empty_like(rhs)
+        its = (C.c_int * B)()
+        errs = (C.c_double * B)()
+        st = capi.CgStats()
+        h = _StreamCtx.get(rhs.device)
+        capi.call("cm_pcg_solve_ex", h, _DT[rhs.dtype], self.n, self.nnz, B, _p(self.JA), _p(self.IA),
+                  _p(vals), 1, None, _p(rhs), _p(x), 0, float(tol), int(max_iter), 1, its, errs,
+                  C.byref(st))
+        return x, list(its), list(errs)
+
+    def edge_grad(self, lam: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        lam, x = lam.contiguous(), x.contiguous()
+        B = x.shape[0]
+        g = torch.empty(B, self.nedges, dtype=x.dtype, device=x.device)
+        h = _StreamCtx.get(x.device)
+        capi.call("cm_edge_grad", h, _DT[x.dtype], self.nedges, B, self.n, _p(self.ei), _p(self.ej),
+                  _p(lam), _p(x), _p(g))
+        return g
+
+
+class CGSolve(torch.autograd.Function):
+    """x = A(theta)^{-1} b, differentiable in theta and b."""
+
+    @staticmethod
+    def forward(ctx, theta, b, lap: EdgeLaplacian, tol: float, max_iter: int):
+        vals = lap.assemble(theta)
+        x, its, errs = lap.solve(vals, b, tol, max_iter)
+        ctx.save_for_backward(vals, x)
+        ctx.lap, ctx.tol, ctx.max_iter = lap, tol, max_iter
+        ctx.forward_stats = (its, errs)
+        CGSolve.last_forward_stats = (its, errs)
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        vals, x = ctx.saved_tensors
+        lap = ctx.lap
+        lam, its, errs = lap.solve(vals, gx, ctx.tol, ctx.max_iter)  # A^T = A
+        CGSolve.last_backward_stats = (its, errs)
+        gtheta = lap.edge_grad(lam, x)
+        return gtheta, lam, None, None, None
+
+
+def cg_solve(theta: torch.Tensor, b: torch.Tensor, lap: EdgeLaplacian, tol: float = 1e-7,
+             max_iter: int = 20000) -> torch.Tensor:
+    return CGSolve.apply(theta, b, lap, tol, max_iter)

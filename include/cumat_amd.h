@@ -307,6 +307,22 @@ int cm_pcg_solve(cm_ctx_t ctx, int dtype,
                  int* iterations_out, double* error_out,
                  cm_cg_stats* stats);
 
+/* Extension of cm_pcg_solve (multi-kernel path, batches <= 64) with no
+ * reference counterpart: vals_batched = 1 gives every batch its own matrix
+ * values (stride nnz, same pattern; invdiag, if given, has stride n), and
+ * per_batch = 1 lets each batch stop at its own convergence (each then has
+ * the plain solver's semantics on its own; iterations_each / error_each,
+ * host arrays of 'batches' entries, receive them). With per_batch = 0 the
+ * reference's rule holds: all batches iterate until all have converged. */
+int cm_pcg_solve_ex(cm_ctx_t ctx, int dtype,
+                    int n, int nnz, int batches,
+                    const int* JA, const int* IA, const void* vals,
+                    int vals_batched, const void* invdiag,
+                    const void* b, void* x, int use_guess,
+                    double tol, int max_iter, int per_batch,
+                    int* iterations_each, double* error_each,
+                    cm_cg_stats* stats);
+
 /* ======================================================================
  * Addition 2 (no reference counterpart): persistent batched CG over many
  * independent, uneven CSR systems with per-system convergence. System s

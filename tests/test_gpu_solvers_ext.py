@@ -1,0 +1,166 @@
+"""GPU tests of the three solver additions (no reference counterpart). Their
+oracle is the plain method restated in oracle/ (FP64 CG per system), closed
+forms, and central finite differences for the gradient."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _csr_dense(JA, IA, V, n):
+    A = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(JA))
+    A[rows, IA] = V
+    return A
+
+
+# ------------------------------------------------------------ batched values
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_solve_ex_batched_values_per_batch(gpu, orc, dtype):
+    """Each batch has its own matrix (same pattern) and stops at its own
+    convergence: per batch identical to the plain solver on that system."""
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    g, B = 30, 5
+    n = g * g
+    shifts = [0.0, 0.05, 0.3, 1.0, 4.0]
+    JA, IA, V0 = poisson2d(g, 0.0, np.float64)
+    vals = np.concatenate([poisson2d(g, s, np.float64)[2] for s in shifts]).astype(dtype)
+    rng = np.random.default_rng(17)
+    b = rng.uniform(-1, 1, n * B).astype(dtype)
+    b[2 * n:3 * n] = 0.0  # a zero right-hand side: early out for that batch only
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    dJA = cm.DeviceArray.from_numpy(JA)
+    dIA = cm.DeviceArray.from_numpy(IA)
+    dV = cm.DeviceArray.from_numpy(vals)
+    db = cm.DeviceArray.from_numpy(b)
+    dx = cm.DeviceArray(n * B, dtype=dtype)
+    its = (C.c_int * B)()
+    errs = (C.c_double * B)()
+    st = cm.capi.CgStats()
+    cm.capi.call("cm_pcg_solve_ex", None, cm.capi.F64 if dtype == np.float64 else cm.capi.F32, n, IA.size, B,
+                 dJA.ptr, dIA.ptr, dV.ptr, 1, None, db.ptr, dx.ptr, 0, tol, 5000, 1, its, errs, C.byref(st))
+    x = dx.numpy()
+    for k in range(B):
+        xk, itk, errk = orc.pcg(JA, IA, vals[k * IA.size:(k + 1) * IA.size], b[k * n:(k + 1) * n], n, tol=tol,
+                                max_iter=5000)
+        if k == 2:
+            assert its[k] == 0 and errs[k] == 0 and not x[k * n:(k + 1) * n].any()
+            continue
+        assert abs(its[k] - itk) <= (1 if dtype == np.float64 else 3), (k, its[k], itk)
+        assert errs[k] <= tol
+        rel = np.linalg.norm(x[k * n:(k + 1) * n] - xk) / np.linalg.norm(xk)
+        assert rel <= (1e-8 if dtype == np.float64 else 1e-4), (k, rel)
+    # iteration counts differ per system (shift 4 converges much faster)
+    assert its[4] < its[0]
+
+
+# ------------------------------------------------------------ persistent batched CG
+def _batched_problem(nsys, lo, hi, seed):
+    from cumat_amd.problems import batched_poisson
+    return batched_poisson(nsys, lo, hi, seed)
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_batched_persistent_cg(gpu, orc, use_graph):
+    cm = gpu
+    P = _batched_problem(300, 16, 40, seed=5)
+    nsys = 300
+    ro = P["row_off"]
+    d = {k: cm.DeviceArray.from_numpy(P[k]) for k in ("row_off", "JA", "IA", "vals", "b")}
+    x = cm.DeviceArray(P["b"].size, dtype=np.float32)
+    its = cm.DeviceArray(nsys, dtype=np.int32)
+    errs = cm.DeviceArray(nsys, dtype=np.float32)
+    for rep in range(2 if use_graph else 1):  # a graph replays the same result
+        x.fill_zero()
+        cm.pcg_batched(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], x, 1e-6, 2000, its, errs,
+                       use_graph=bool(use_graph))
+        xh, ih, eh = x.numpy(), its.numpy(), errs.numpy()
+        for s in range(0, nsys, 7):
+            lo, hi = ro[s], ro[s + 1]
+            n = hi - lo
+            JA = P["JA"][lo:hi + 1] - P["JA"][lo]
+            IA = P["IA"][P["JA"][lo]:P["JA"][hi]] - lo
+            V = P["vals"][P["JA"][lo]:P["JA"][hi]]
+            xo, ito, erro = orc.pcg(JA, IA, V, P["b"][lo:hi], n, tol=1e-6, max_iter=2000)
+            x64, _, _ = orc.pcg(JA, IA, V.astype(np.float64), P["b"][lo:hi].astype(np.float64), n, tol=1e-13,
+                                max_iter=5000)
+            assert abs(int(ih[s]) - ito) <= 3, (s, ih[s], ito)
+            assert eh[s] <= 1e-6
+            assert np.linalg.norm(xh[lo:hi] - x64) / np.linalg.norm(x64) <= 1e-5
+
+
+def test_batched_autotune_and_block_sizes(gpu, orc):
+    cm = gpu
+    P = _batched_problem(64, 30, 70, seed=9)
+    d = {k: cm.DeviceArray.from_numpy(P[k]) for k in ("row_off", "JA", "IA", "vals", "b")}
+    from cumat_amd.solvers import pcg_batched_autotune
+    bs, ms = pcg_batched_autotune(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], 1e-6, 2000)
+    assert bs in (256, 512, 1024) and ms > 0
+    res = []
+    for b_s in (256, 512, 1024):
+        x = cm.DeviceArray(P["b"].size, dtype=np.float32)
+        its = cm.DeviceArray(64, dtype=np.int32)
+        errs = cm.DeviceArray(64, dtype=np.float32)
+        cm.pcg_batched(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], x, 1e-6, 2000, its, errs, block_size=b_s)
+        res.append((x.numpy(), its.numpy()))
+    # block size changes the reduction tree only: same iterations within 2
+    assert np.abs(res[0][1] - res[2][1]).max() <= 2
+    np.testing.assert_allclose(res[0][0], res[2][0], atol=1e-4)
+
+
+# ------------------------------------------------------------ mixed precision IR
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_mixed_precision_ir_reaches_1e12(gpu, orc, storage):
+    """3D 7-point variable-coefficient Poisson (log-normal(0,1) edges): FP64
+    iterative refinement around FP32/BF16 inner CG reaches a true FP64
+    relative residual <= 1e-12 and the FP64 CG oracle's solution."""
+    cm = gpu
+    from cumat_amd.problems import grid_laplacian, lognormal_sampler
+    g = 24
+    P = grid_laplacian((g, g, g), lognormal_sampler(np.random.default_rng(160), 0.0, 1.0))
+    n = P["n"]
+    b = np.random.default_rng(161).uniform(-1, 1, n)
+    dJA, dIA = cm.DeviceArray.from_numpy(P["JA"]), cm.DeviceArray.from_numpy(P["IA"])
+    dV = cm.DeviceArray.from_numpy(P["vals"])
+    db = cm.DeviceArray.from_numpy(b)
+    dx = cm.DeviceArray(n, dtype=np.float64)
+    st = cm.pcg_ir(dJA, dIA, dV, db, dx, storage=storage, tol=1e-12, inner_tol=1e-3 if storage == "bf16" else 1e-5)
+    x = dx.numpy()
+    assert st["rel_residual"] <= 1e-12, st
+    A = _csr_dense(P["JA"], P["IA"], P["vals"], n) if n <= 20000 else None
+    r = b - orc.csr_spmv(P["JA"], P["IA"], P["vals"], x, n, n)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.5e-12
+    x64, it64, err64 = orc.pcg(P["JA"], P["IA"], P["vals"], b, n, tol=1e-13, max_iter=20000)
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) <= 1e-9
+    assert st["outer"] >= 2 and st["inner"] > 0
+    del A
+
+
+# ------------------------------------------------------------ edge kernels
+def test_edge_assemble_and_grad(gpu, orc):
+    cm = gpu
+    from cumat_amd.problems import grid_laplacian
+    rng = np.random.default_rng(21)
+    P = grid_laplacian((17, 23), lambda s: np.exp(0.5 * rng.standard_normal(s)))
+    B = 3
+    thetas = np.stack([P["theta"] * (1 + 0.1 * k) for k in range(B)])
+    d = {k: cm.DeviceArray.from_numpy(P[k]) for k in ("eptr", "eidx", "ei", "ej")}
+    dth = cm.DeviceArray.from_numpy(thetas.reshape(-1))
+    dv = cm.DeviceArray(P["nnz"] * B, dtype=np.float64)
+    cm.capi.call("cm_edge_assemble", None, cm.capi.F64, P["nnz"], B, P["nedges"], d["eptr"].ptr, d["eidx"].ptr,
+                 dth.ptr, dv.ptr)
+    got = dv.numpy().reshape(B, -1)
+    for k in range(B):
+        np.testing.assert_allclose(got[k], P["vals"] * (1 + 0.1 * k), rtol=1e-14, atol=1e-14)
+    lam = rng.standard_normal((B, P["n"]))
+    x = rng.standard_normal((B, P["n"]))
+    dl, dx = cm.DeviceArray.from_numpy(lam.reshape(-1)), cm.DeviceArray.from_numpy(x.reshape(-1))
+    dg = cm.DeviceArray(P["nedges"] * B, dtype=np.float64)
+    cm.capi.call("cm_edge_grad", None, cm.capi.F64, P["nedges"], B, P["n"], d["ei"].ptr, d["ej"].ptr, dl.ptr,
+                 dx.ptr, dg.ptr)
+    g = dg.numpy().reshape(B, -1)
+    for k in range(B):
+        np.testing.assert_array_equal(g[k], orc.edge_grad(P["ei"], P["ej"], lam[k], x[k]))
