@@ -249,7 +249,153 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
         out[label] = {"iterations": cg.iterations(), "error": cg.error(), "ms_to_solution": round(t * 1e3, 3),
                       "iters_per_s": round(cg.iterations() / t, 1)}
         log(label, out[label])
+    for name, fn in (("batched_cg_4096", bench_batched), ("mixed_ir_160cubed", bench_mixed),
+                     ("adjoint_64x512sq", bench_adjoint)):
+        try:
+            out[name] = fn(cm, capi, ctx)
+            log(name, out[name])
+        except Exception as ex:  # reported, the headline line is still printed
+            out[name] = {"error": repr(ex)}
+            log(name, "failed:", repr(ex))
     return out
+
+
+def _sub_csr(P, s):
+    ro = P["row_off"]
+    lo, hi = int(ro[s]), int(ro[s + 1])
+    a, b = int(P["JA"][lo]), int(P["JA"][hi])
+    return (P["JA"][lo:hi + 1] - a).astype(np.int32), (P["IA"][a:b] - lo).astype(np.int32), P["vals"][a:b], \
+        P["b"][lo:hi], hi - lo
+
+
+def bench_batched(cm, capi, ctx):
+    """4096 independent shifted 2D Poisson systems, side ~ U{16..96}, one
+    persistent launch (autotuned block size, graph replay) vs looping the
+    plain GPU solver over a sample of the systems."""
+    from cumat_amd.problems import batched_poisson
+    from cumat_amd.solvers import pcg_batched, pcg_batched_autotune
+    nsys, tol, maxit = 4096, 1e-6, 5000
+    P = batched_poisson(nsys, 16, 96, seed=20261205)
+    d = {k: cm.DeviceArray.from_numpy(P[k], ctx=ctx) for k in ("row_off", "JA", "IA", "vals", "b")}
+    x = cm.DeviceArray(P["b"].size, dtype=np.float32, ctx=ctx)
+    its = cm.DeviceArray(nsys, dtype=np.int32, ctx=ctx)
+    errs = cm.DeviceArray(nsys, dtype=np.float32, ctx=ctx)
+    bs, tune_ms = pcg_batched_autotune(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], tol, maxit, ctx=ctx)
+    pcg_batched(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], x, tol, maxit, its, errs, use_graph=True, ctx=ctx)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        pcg_batched(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], x, tol, maxit, its, errs, use_graph=True,
+                    ctx=ctx)
+        ts.append(time.perf_counter() - t0)
+    tb = min(ts)
+    ih, eh = its.numpy(), errs.numpy()
+    # looping the plain solver (cm_pcg_solve, its automatic path) on every 64th system
+    sample = list(range(0, nsys, 64))
+    loop_t = 0.0
+    for s in sample:
+        JA, IA, V, b, n = _sub_csr(P, s)
+        A = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
+        bd = cm.DeviceArray.from_numpy(b, ctx=ctx)
+        xd = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
+        cg = cm.ConjugateGradient(A).setTolerance(tol).setMaxIterations(maxit)
+        cg.solve(bd, xd, ctx=ctx)
+        ctx.sync()
+        t0 = time.perf_counter()
+        cg.solve(bd, xd, ctx=ctx)
+        ctx.sync()
+        loop_t += time.perf_counter() - t0
+    loop_sps = len(sample) / loop_t
+    return {"systems": nsys, "rows": int(P["b"].size), "nnz": int(P["IA"].size), "block_size": bs,
+            "seconds": round(tb, 4), "solves_per_s": round(nsys / tb, 1),
+            "plain_loop_solves_per_s": round(loop_sps, 1), "speedup_vs_plain_loop": round(nsys / tb / loop_sps, 2),
+            "iterations_total": int(ih.sum()), "iterations_max": int(ih.max()), "max_error": float(eh.max())}
+
+
+def bench_mixed(cm, capi, ctx):
+    """3D 7-point variable-coefficient Poisson 160^3, log-normal(0,1) edges:
+    FP64 iterative refinement around FP32 / BF16-storage inner CG against
+    plain FP64 CG, all to relative residual 1e-12."""
+    from cumat_amd.problems import grid_laplacian, lognormal_sampler
+    from cumat_amd.solvers import pcg_ir
+    P = grid_laplacian((160, 160, 160), lognormal_sampler(np.random.default_rng(20261206), 0.0, 1.0), edges=False)
+    n = P["n"]
+    b = np.random.default_rng(20261207).uniform(-1, 1, n)
+    dV = cm.DeviceArray.from_numpy(P["vals"], ctx=ctx)
+    db = cm.DeviceArray.from_numpy(b, ctx=ctx)
+    A = cm.SparseMatrix(cm.SparsityPattern(n, n, P["JA"], P["IA"], ctx=ctx), dV)
+    dJA, dIA = A.pattern.JA, A.pattern.IA
+    out = {"n": n, "nnz": P["nnz"]}
+
+    def true_rel(xd):
+        r = A.matvec(xd, ctx=ctx).numpy() - b
+        return float(np.linalg.norm(r) / np.linalg.norm(b))
+    x64 = cm.DeviceArray(n, dtype=np.float64, ctx=ctx)
+    cg = cm.ConjugateGradient(A, path=capi.CG_MULTIKERNEL).setTolerance(1e-12).setMaxIterations(40000)
+    t0 = time.perf_counter()
+    cg.solve(db, x64, ctx=ctx)
+    ctx.sync()
+    t64 = time.perf_counter() - t0
+    out["fp64_cg"] = {"seconds": round(t64, 3), "iterations": cg.iterations(), "rel_residual": true_rel(x64)}
+    xref = x64.numpy()
+    for storage, itol in (("f32", 1e-4), ("bf16", 1e-3)):
+        xd = cm.DeviceArray(n, dtype=np.float64, ctx=ctx)
+        st = pcg_ir(dJA, dIA, dV, db, xd, storage=storage, tol=1e-12, inner_tol=itol, max_outer=60, ctx=ctx)
+        xh = xd.numpy()
+        out["ir_" + storage] = {"seconds": round(st["seconds"], 3), "outer": st["outer"], "inner": st["inner"],
+                                "rel_residual": true_rel(xd),
+                                "err_vs_fp64_cg": float(np.linalg.norm(xh - xref) / np.linalg.norm(xref)),
+                                "speedup_vs_fp64_cg": round(t64 / st["seconds"], 3)}
+    return out
+
+
+def bench_adjoint(cm, capi, ctx):
+    """64 variable-coefficient 2D 512^2 Poisson systems, theta = exp(N(0, 0.5^2))
+    per edge: FP32 forward CG (tol 1e-7) + adjoint CG + fused edge gradient of
+    L = 0.5 |x - x*|^2, through the C ABI (the kernels the autograd function
+    calls)."""
+    import ctypes as C
+    from cumat_amd.problems import grid_laplacian
+    B, g, tol = 64, 512, 1e-7
+    P = grid_laplacian((g, g), lambda s: np.ones(s))
+    rng = np.random.default_rng(20261208)
+    n, nnz, ne = P["n"], P["nnz"], P["nedges"]
+    theta = np.exp(0.5 * rng.standard_normal((B, ne))).astype(np.float32)
+    b = rng.uniform(-1, 1, (B, n)).astype(np.float32)
+    xs = rng.standard_normal((B, n)).astype(np.float32)
+    d = {k: cm.DeviceArray.from_numpy(P[k], ctx=ctx) for k in ("JA", "IA", "eptr", "eidx", "ei", "ej")}
+    dth = cm.DeviceArray.from_numpy(theta.reshape(-1), ctx=ctx)
+    db = cm.DeviceArray.from_numpy(b.reshape(-1), ctx=ctx)
+    dxs = cm.DeviceArray.from_numpy(xs.reshape(-1), ctx=ctx)
+    vals = cm.DeviceArray(B * nnz, dtype=np.float32, ctx=ctx)
+    x = cm.DeviceArray(B * n, dtype=np.float32, ctx=ctx)
+    gx = cm.DeviceArray(B * n, dtype=np.float32, ctx=ctx)
+    lam = cm.DeviceArray(B * n, dtype=np.float32, ctx=ctx)
+    grad = cm.DeviceArray(B * ne, dtype=np.float32, ctx=ctx)
+    its = (C.c_int * B)()
+    errs = (C.c_double * B)()
+    st = capi.CgStats()
+
+    def solve(rhs, out):
+        capi.call("cm_pcg_solve_ex", ctx.handle, capi.F32, n, nnz, B, d["JA"].ptr, d["IA"].ptr, vals.ptr, 1, None,
+                  rhs.ptr, out.ptr, 0, tol, 20000, 1, its, errs, C.byref(st))
+        return list(its), list(errs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    capi.call("cm_edge_assemble", ctx.handle, capi.F32, nnz, B, ne, d["eptr"].ptr, d["eidx"].ptr, dth.ptr, vals.ptr)
+    f_its, f_errs = solve(db, x)
+    ctx.sync()
+    t1 = time.perf_counter()
+    cm.assign(gx, x.expr() - dxs.expr(), ctx=ctx)
+    b_its, b_errs = solve(gx, lam)
+    capi.call("cm_edge_grad", ctx.handle, capi.F32, ne, B, n, d["ei"].ptr, d["ej"].ptr, lam.ptr, x.ptr, grad.ptr)
+    ctx.sync()
+    t2 = time.perf_counter()
+    gh = grad.numpy()
+    return {"systems": B, "n_each": n, "nnz_each": nnz, "edges_each": ne, "forward_s": round(t1 - t0, 3),
+            "backward_s": round(t2 - t1, 3), "forward_iters_max": max(f_its), "backward_iters_max": max(b_its),
+            "forward_err_max": max(f_errs), "backward_err_max": max(b_errs),
+            "grad_finite": bool(np.isfinite(gh).all())}
 
 
 def cpu_baseline(JA, IA, V, b, n, nnz):

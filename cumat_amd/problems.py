@@ -31,15 +31,15 @@ def poisson2d(g: int, shift: float = 0.0, dtype=np.float32):
     return JA, cols[valid].astype(np.int32), vals[valid].astype(dtype)
 
 
-def grid_laplacian(dims, coef_sampler, dtype=np.float64):
+def grid_laplacian(dims, coef_sampler, dtype=np.float64, edges=True):
     """Variable-coefficient edge Laplacian on a grid of shape 'dims' (C order:
     the last axis is the fastest). coef_sampler(shape) returns the edge
     coefficients of one axis, an array of 'dims' with dims[a]+1 on axis a
     (entries 0 and dims[a] along that axis are the Dirichlet boundary edges).
 
-    Returns dict with JA, IA, vals, n, nedges, theta (edge coefficients),
-    ei, ej (edge endpoints, ej = -1 for boundary edges), eptr, eidx (gather
-    map of cm_edge_assemble)."""
+    Returns dict with JA, IA, vals, n, nedges, theta (edge coefficients) and,
+    if edges, ei, ej (edge endpoints, ej = -1 for boundary edges), eptr, eidx
+    (gather map of cm_edge_assemble)."""
     dims = tuple(int(d) for d in dims)
     d = len(dims)
     n = int(np.prod(dims))
@@ -93,6 +93,9 @@ def grid_laplacian(dims, coef_sampler, dtype=np.float64):
     JA[1:] = np.cumsum(valid.sum(1))
     IA = cols[valid].astype(np.int32)
     V = vals[valid].astype(dtype)
+    del cols, vals
+    if not edges:
+        return {"JA": JA, "IA": IA, "vals": V, "n": n, "nnz": int(IA.size), "nedges": nedges, "theta": theta}
     # gather map: off-diagonal entry -> -(edge+1); diagonal -> +(edge+1) x 2d
     cnt = np.where(np.arange(ns) == d, 2 * d, 1)
     per_entry = np.broadcast_to(cnt, (n, ns))[valid]
