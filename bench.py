@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the SpMV roofline launches and a copy kernel of known size (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
@@ -97,6 +99,8 @@ def main():
     cg = cm.ConjugateGradient(A, path=capi.CG_MULTIKERNEL).setTolerance(0.0)
 
     # ------------------------------------------------------------ timed CG
+    if args.kernels_only:
+        args.steps = args.warmup = 1
     cg.setMaxIterations(args.warmup)
     cg.solve(bd, xd, ctx=ctx)
     ctx.sync()
@@ -134,6 +138,22 @@ def main():
     spmv_bytes = 8 * nnz + 12 * n
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     log(f"SpMV 2048^2: {spmv_ms * 1000:.1f} us/launch, {spmv_gbs:.0f} GB/s ({spmv_gbs / HBM_PEAK_GBS:.1%})")
+    if args.kernels_only:
+        # calibration for the HBM byte counters: dst = src over 2^26 floats
+        # (one cwise launch, 256 MiB read + 256 MiB written)
+        src = cm.DeviceArray.from_numpy(rng.uniform(-1, 1, 1 << 26).astype(np.float32), ctx=ctx)
+        dst = cm.DeviceArray(1 << 26, dtype=np.float32, ctx=ctx)
+        cm.assign(dst, src.expr(), ctx=ctx)
+        ctx.sync()
+        timer.start(stream)
+        for _ in range(20):
+            cm.assign(dst, src.expr(), ctx=ctx)
+        copy_ms = timer.stop_ms(stream) / 20
+        log(f"copy 2^26 f32: {copy_ms * 1000:.1f} us/launch, {2 * 4 * (1 << 26) / (copy_ms * 1e-3) / 1e9:.0f} GB/s")
+        print(json.dumps({"spmv_us": round(spmv_ms * 1000, 2), "spmv_algorithmic_bytes": spmv_bytes,
+                          "copy_us": round(copy_ms * 1000, 2), "copy_bytes_read": 4 << 26,
+                          "copy_bytes_written": 4 << 26}), flush=True)
+        return
     traffic = None
     tf = os.path.join(ROOT, "profiles", "spmv_poisson2048_traffic.json")
     if os.path.exists(tf):
