@@ -1,0 +1,312 @@
+#ifndef CUMAT_AMD_SPARSEMATRIX_H
+#define CUMAT_AMD_SPARSEMATRIX_H
+
+// Sparse matrices of the host layer: SparsityPattern<CSR> with cuMat's
+// names JA (row pointer, rows+1) and IA (column index, nnz)
+// (cuMat/src/SparseMatrixBase.h:89-147), SparsityPattern<ELLPACK>
+// (cuMat/src/SparseMatrixBase.h:149-206), and SparseMatrix<Scalar, Batches,
+// Flags> whose values are an nnz x 1 x Batches matrix sharing one pattern
+// (cuMat/src/SparseMatrix.h:48-414). A * x with a dense vector x is the
+// sparse product of cuMat/src/SparseProductEvaluation.h:43-139 (CSR) and
+// 194-324 (ELLPACK), evaluated by one cm_csr_spmv / cm_ell_spmv launch.
+// Assigning a dense component-wise expression to a sparse matrix evaluates it
+// at the stored entries only (cuMat/src/SparseEvaluation.h:14-34,83-138).
+
+#include <vector>
+
+#include "ReductionOps.h"
+
+CUMAT_NAMESPACE_BEGIN
+
+template <int _SparseFlags> struct SparsityPattern;
+
+template <>
+struct SparsityPattern<SparseFlags::CSR> {
+    typedef int StorageIndex;
+    typedef Matrix<StorageIndex, Dynamic, 1, 1, ColumnMajor> IndexVector;
+    template <typename _Scalar, int _Batches>
+    using DataMatrix = Matrix<_Scalar, Dynamic, 1, _Batches, ColumnMajor>;
+
+    Index nnz = 0;
+    Index rows = 0;
+    Index cols = 0;
+    IndexVector IA;
+    IndexVector JA;
+
+    // cuMat's dimension checks plus the content checks every kernel relies on
+    // (JA non-decreasing from 0 to nnz, IA within [0, cols)): a malformed
+    // pattern is rejected on the host instead of reaching a kernel.
+    void assertValid() const
+    {
+        CUMAT_ASSERT_DIMENSION(JA.size() == rows + 1);
+        CUMAT_ASSERT_DIMENSION(rows > 0);
+        CUMAT_ASSERT_DIMENSION(cols > 0);
+        CUMAT_ASSERT_DIMENSION(IA.size() == nnz);
+        std::vector<int> ja((size_t)JA.size()), ia((size_t)IA.size());
+        JA.copyToHost(ja.data());
+        IA.copyToHost(ia.data());
+        CUMAT_ASSERT_ARGUMENT(ja[0] == 0 && ja[(size_t)rows] == nnz);
+        for (Index r = 0; r < rows; ++r) CUMAT_ASSERT_ARGUMENT(ja[(size_t)r] <= ja[(size_t)r + 1]);
+        for (int c : ia) CUMAT_ASSERT_ARGUMENT(c >= 0 && c < cols);
+    }
+    Index getNNZ() const { return nnz; }
+    template <typename _Scalar, int _Batches>
+    DataMatrix<_Scalar, _Batches> allocateDataMatrix(Index batches) const
+    {
+        return DataMatrix<_Scalar, _Batches>(nnz, 1, batches);
+    }
+    SparsityPattern deepClone() const
+    {
+        SparsityPattern c;
+        c.nnz = nnz;
+        c.rows = rows;
+        c.cols = cols;
+        c.IA = IA.deepClone();
+        c.JA = JA.deepClone();
+        return c;
+    }
+};
+
+template <>
+struct SparsityPattern<SparseFlags::ELLPACK> {
+    typedef int StorageIndex;
+    // rows x nnzPerRow, column major; -1 marks padding
+    typedef Matrix<StorageIndex, Dynamic, Dynamic, 1, ColumnMajor> IndexMatrix;
+    template <typename _Scalar, int _Batches>
+    using DataMatrix = Matrix<_Scalar, Dynamic, Dynamic, _Batches, ColumnMajor>;
+
+    Index rows = 0;
+    Index cols = 0;
+    Index nnzPerRow = 0;
+    IndexMatrix indices;
+
+    void assertValid() const
+    {
+        CUMAT_ASSERT_DIMENSION(rows > 0);
+        CUMAT_ASSERT_DIMENSION(cols > 0);
+        CUMAT_ASSERT_DIMENSION(indices.rows() == rows && indices.cols() == nnzPerRow);
+        std::vector<int> ind((size_t)indices.size());
+        indices.copyToHost(ind.data());
+        for (int c : ind) CUMAT_ASSERT_ARGUMENT(c >= -1 && c < cols);
+    }
+    Index getNNZ() const { return rows * nnzPerRow; }
+    template <typename _Scalar, int _Batches>
+    DataMatrix<_Scalar, _Batches> allocateDataMatrix(Index batches) const
+    {
+        return DataMatrix<_Scalar, _Batches>(rows, nnzPerRow, batches);
+    }
+};
+
+template <typename _SMatrix, typename _Vector>
+class SparseProductOp;
+
+template <typename _Scalar, int _Batches, int _SparseFlags>
+class SparseMatrix {
+    static_assert(_SparseFlags == SparseFlags::CSR || _SparseFlags == SparseFlags::ELLPACK,
+                  "SparseFlags must be CSR or ELLPACK");
+
+public:
+    typedef _Scalar Scalar;
+    typedef SparsityPattern<_SparseFlags> Pattern;
+    typedef typename Pattern::template DataMatrix<_Scalar, _Batches> DataMatrix;
+    enum { Batches = _Batches, SFlags = _SparseFlags };
+
+private:
+    Pattern sparsity_;
+    Index batches_ = 0;
+    DataMatrix A_;
+
+public:
+    SparseMatrix() = default;
+    explicit SparseMatrix(const Pattern& pattern, Index batches = (_Batches == Dynamic ? 1 : _Batches))
+        : sparsity_(pattern), batches_(batches), A_(pattern.template allocateDataMatrix<_Scalar, _Batches>(batches))
+    {
+        sparsity_.assertValid();
+    }
+
+    DataMatrix& getData() { return A_; }
+    const DataMatrix& getData() const { return A_; }
+    const Pattern& getSparsityPattern() const { return sparsity_; }
+    Index rows() const { return sparsity_.rows; }
+    Index cols() const { return sparsity_.cols; }
+    Index batches() const { return batches_; }
+    Index nnz() const { return sparsity_.getNNZ(); }
+    Index size() const { return rows() * cols() * batches_; }
+    bool isExclusiveUse() const { return A_.isExclusiveUse(); }
+    void makeExclusiveUse() { A_.makeExclusiveUse(); }
+    SparseMatrix deepClone() const
+    {
+        SparseMatrix m;
+        m.sparsity_ = sparsity_;
+        m.batches_ = batches_;
+        m.A_ = A_.deepClone();
+        return m;
+    }
+
+    // sparse * dense column vector (batched: matrix and/or vector batches)
+    template <typename V>
+    SparseProductOp<SparseMatrix, V> operator*(const MatrixBase<V>& v) const
+    {
+        return SparseProductOp<SparseMatrix, V>(*this, v.derived());
+    }
+
+    // dense expression evaluated at the stored entries (CSR)
+    template <typename E>
+    void assignCwise(const MatrixBase<E>& e, AssignmentMode mode)
+    {
+        static_assert(_SparseFlags == SparseFlags::CSR, "component-wise evaluation into sparse matrices is CSR only");
+        static_assert(std::is_same<typename E::Scalar, _Scalar>::value, "No implicit casting in assignments");
+        const E& x = e.derived();
+        CUMAT_ASSERT_DIMENSION(x.rows() == rows() || x.rows() == 1);
+        CUMAT_ASSERT_DIMENSION(x.cols() == cols() || x.cols() == 1);
+        CUMAT_ASSERT_DIMENSION(x.batches() == batches_ || x.batches() == 1);
+        internal::ProgramBuilder b;
+        x.emit(b, 0);
+        std::vector<cm_leaf> leaves(b.leaves.size() ? b.leaves.size() : 1);
+        for (size_t k = 0; k < b.leaves.size(); ++k) {
+            const internal::LeafDesc& l = b.leaves[k];
+            leaves[k].ptr = l.ptr;
+            leaves[k].dtype = l.dtype;
+            leaves[k].kind = l.kind;
+            leaves[k].sr = (l.rows == 1 && rows() != 1) ? 0 : l.sr;
+            leaves[k].sc = (l.cols == 1 && cols() != 1) ? 0 : l.sc;
+            leaves[k].sb = (l.batches == 1 && batches_ != 1) ? 0 : l.sb;
+        }
+        CUMAT_AMD_CALL(cm_csr_cwise_eval(Context::current().handle(), leaves.data(), (int)b.leaves.size(),
+                                         b.code.data(), (int)b.code.size(), (int)rows(), (int)cols(), (int)nnz(),
+                                         (int)batches_, sparsity_.JA.data(), sparsity_.IA.data(), A_.data(),
+                                         internal::DType<_Scalar>::value, (int)mode));
+    }
+    template <typename E> SparseMatrix& operator=(const MatrixBase<E>& e) { assignCwise(e, AssignmentMode::ASSIGN); return *this; }
+    template <typename E> SparseMatrix& operator+=(const MatrixBase<E>& e) { assignCwise(e, AssignmentMode::ADD); return *this; }
+    template <typename E> SparseMatrix& operator-=(const MatrixBase<E>& e) { assignCwise(e, AssignmentMode::SUB); return *this; }
+    template <typename E> SparseMatrix& operator*=(const MatrixBase<E>& e) { assignCwise(e, AssignmentMode::MUL); return *this; }
+    template <typename E> SparseMatrix& operator/=(const MatrixBase<E>& e) { assignCwise(e, AssignmentMode::DIV); return *this; }
+    // compound operations with another matrix of the same pattern, or a scalar
+    // (cuMat/src/SparseMatrix.h:334-386)
+    SparseMatrix& operator+=(const SparseMatrix& o) { assignCwise(o.direct(), AssignmentMode::ADD); return *this; }
+    SparseMatrix& operator-=(const SparseMatrix& o) { assignCwise(o.direct(), AssignmentMode::SUB); return *this; }
+    SparseMatrix& operator*=(const SparseMatrix& o) { assignCwise(o.direct(), AssignmentMode::MUL); return *this; }
+    template <typename S, typename = internal::if_arith<S>>
+    SparseMatrix& operator*=(const S& s) { assignCwise(HostScalar<_Scalar>((_Scalar)s), AssignmentMode::MUL); return *this; }
+    template <typename S, typename = internal::if_arith<S>>
+    SparseMatrix& operator+=(const S& s) { assignCwise(HostScalar<_Scalar>((_Scalar)s), AssignmentMode::ADD); return *this; }
+
+    // the stored values as an expression leaf evaluated at each stored entry
+    // (SparseMatrix::direct(), cuMat/src/SparseMatrix.h:230-233); valid in
+    // assignments to a sparse matrix of the same pattern
+    StridedView<_Scalar> direct() const
+    {
+        static_assert(_SparseFlags == SparseFlags::CSR, "direct() is CSR only");
+        internal::LeafDesc d{};
+        d.ptr = A_.data();
+        d.dtype = internal::DType<_Scalar>::value;
+        d.kind = CM_LEAF_STRIDED;
+        d.sr = -1;  // marks the value array of the destination pattern
+        d.sc = -1;
+        d.sb = nnz();
+        d.rows = rows();
+        d.cols = cols();
+        d.batches = batches_;
+        return StridedView<_Scalar>(A_.dataPointer(), d);
+    }
+
+    // identity pattern values: 1 on the diagonal, 0 elsewhere, as an
+    // expression (evaluated at the stored entries when assigned)
+    static StridedView<_Scalar> Identity(Index rows, Index cols)
+    {
+        Matrix<_Scalar, 1, 1, 1, ColumnMajor> one;
+        one.inplace() = HostScalar<_Scalar>(_Scalar(1));
+        internal::LeafDesc d{};
+        d.ptr = one.data();
+        d.dtype = internal::DType<_Scalar>::value;
+        d.kind = CM_LEAF_AS_DIAGONAL;
+        d.sr = d.sc = d.sb = 0;
+        d.rows = rows;
+        d.cols = cols;
+        d.batches = 1;
+        return StridedView<_Scalar>(one.dataPointer(), d);
+    }
+
+    // host copy of the values as a dense rows x cols x batches array
+    // ([batch][row][col] order), for tests and printing
+    std::vector<_Scalar> toDenseHost() const
+    {
+        static_assert(_SparseFlags == SparseFlags::CSR, "CSR only");
+        std::vector<int> ja((size_t)(rows() + 1)), ia((size_t)nnz());
+        std::vector<_Scalar> v((size_t)(nnz() * batches_)), out((size_t)size(), _Scalar(0));
+        sparsity_.JA.copyToHost(ja.data());
+        sparsity_.IA.copyToHost(ia.data());
+        A_.copyToHost(v.data());
+        for (Index b = 0; b < batches_; ++b)
+            for (Index r = 0; r < rows(); ++r)
+                for (int k = ja[(size_t)r]; k < ja[(size_t)r + 1]; ++k)
+                    out[(size_t)((b * rows() + r) * cols() + ia[(size_t)k])] = v[(size_t)(k + nnz() * b)];
+        return out;
+    }
+};
+
+// y (mode)= A * x, evaluated when assigned to a dense matrix
+template <typename _SMatrix, typename _Vector>
+class SparseProductOp {
+    _SMatrix A_;
+    _Vector x_;
+
+public:
+    typedef typename _SMatrix::Scalar Scalar;
+    static_assert(std::is_same<Scalar, typename _Vector::Scalar>::value,
+                  "No implicit casting between the sparse matrix and the vector");
+
+    SparseProductOp(const _SMatrix& A, const _Vector& x) : A_(A), x_(x)
+    {
+        CUMAT_ASSERT_DIMENSION(x.cols() == 1);
+        CUMAT_ASSERT_DIMENSION(x.rows() == A.cols());
+        CUMAT_ASSERT_DIMENSION(A.batches() == x.batches() || A.batches() == 1 || x.batches() == 1);
+    }
+    Index rows() const { return A_.rows(); }
+    Index cols() const { return 1; }
+    Index batches() const { return A_.batches() > x_.batches() ? A_.batches() : x_.batches(); }
+
+    void evalTo(Scalar* y, Index y_rows, Index y_batches, AssignmentMode mode) const
+    {
+        CUMAT_ASSERT_DIMENSION(y_rows == rows() && y_batches == batches());
+        auto x = internal::materialize(x_);
+        const auto& P = A_.getSparsityPattern();
+        evalImpl(P, x, y, mode);
+    }
+
+private:
+    template <typename X>
+    void evalImpl(const SparsityPattern<SparseFlags::CSR>& P, const X& x, Scalar* y, AssignmentMode mode) const
+    {
+        const int dt = internal::DType<Scalar>::value;
+        CUMAT_AMD_CALL(cm_csr_spmv(Context::current().handle(), dt, dt, (int)P.rows, (int)P.cols, (int)P.nnz,
+                                   (int)batches(), P.JA.data(), P.IA.data(), A_.getData().data(),
+                                   A_.batches() > 1 ? 1 : 0, x.data(), x.batches() > 1 ? 1 : 0, y, (int)mode));
+    }
+    template <typename X>
+    void evalImpl(const SparsityPattern<SparseFlags::ELLPACK>& P, const X& x, Scalar* y, AssignmentMode mode) const
+    {
+        CUMAT_AMD_CALL(cm_ell_spmv(Context::current().handle(), internal::DType<Scalar>::value, (int)P.rows,
+                                   (int)P.cols, (int)P.nnzPerRow, (int)batches(), P.indices.data(),
+                                   A_.getData().data(), A_.batches() > 1 ? 1 : 0, x.data(),
+                                   x.batches() > 1 ? 1 : 0, y, (int)mode));
+    }
+};
+
+// typedefs (cuMat/src/SparseMatrix.h:606-623)
+#define CUMAT_AMD_SPARSE_TYPEDEFS(S, s) \
+    typedef SparseMatrix<S, 1, SparseFlags::CSR> SMatrixX##s; \
+    typedef SparseMatrix<S, Dynamic, SparseFlags::CSR> BSMatrixX##s; \
+    typedef SparseMatrix<S, 1, SparseFlags::CSR> SMatrixX##s##_CSR; \
+    typedef SparseMatrix<S, Dynamic, SparseFlags::CSR> BSMatrixX##s##_CSR; \
+    typedef SparseMatrix<S, 1, SparseFlags::ELLPACK> SMatrixX##s##_ELLPACK; \
+    typedef SparseMatrix<S, Dynamic, SparseFlags::ELLPACK> BSMatrixX##s##_ELLPACK;
+CUMAT_AMD_SPARSE_TYPEDEFS(int, i)
+CUMAT_AMD_SPARSE_TYPEDEFS(float, f)
+CUMAT_AMD_SPARSE_TYPEDEFS(double, d)
+#undef CUMAT_AMD_SPARSE_TYPEDEFS
+
+CUMAT_NAMESPACE_END
+
+#endif
