@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cg-path", type=int, default=1, choices=[1, 3],
+                    help="multi-kernel CG form: 1 = three launches per iteration, 3 = two (fused p update)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the SpMV roofline launches and a copy kernel of known size (counter runs)")
     args = ap.parse_args()
@@ -96,7 +98,7 @@ def main():
     A = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
     bd = cm.DeviceArray.from_numpy(b, ctx=ctx)
     xd = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
-    cg = cm.ConjugateGradient(A, path=capi.CG_MULTIKERNEL).setTolerance(0.0)
+    cg = cm.ConjugateGradient(A, path=args.cg_path).setTolerance(0.0)
 
     # ------------------------------------------------------------ timed CG
     if args.kernels_only:
@@ -269,6 +271,25 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
         out[label] = {"iterations": cg.iterations(), "error": cg.error(), "ms_to_solution": round(t * 1e3, 3),
                       "iters_per_s": round(cg.iterations() / t, 1)}
         log(label, out[label])
+    # ---- small systems: iterations/s of both launch shapes (200 iterations, tol 0)
+    sweep = {}
+    for g in (16, 32, 48, 64, 96, 128, 192):
+        JA, IA, V = poisson2d(g)
+        Ag = cm.SparseMatrix.from_csr(g * g, g * g, JA, IA, V, ctx=ctx)
+        bg = cm.DeviceArray.from_numpy(np.ones(g * g, np.float32), ctx=ctx)
+        xg = cm.DeviceArray(g * g, dtype=np.float32, ctx=ctx)
+        row = {}
+        for path, label in ((capi.CG_RESIDENT, "resident"), (capi.CG_MULTIKERNEL, "multikernel")):
+            if path == capi.CG_RESIDENT and g * g > 16384:
+                continue
+            cg = cm.ConjugateGradient(Ag, path=path).setTolerance(0.0).setMaxIterations(200)
+            cg.solve(bg, xg, ctx=ctx)
+            t0 = time.perf_counter()
+            cg.solve(bg, xg, ctx=ctx)
+            row[label] = round(cg.iterations() / (time.perf_counter() - t0), 1)
+        sweep[f"{g}^2"] = row
+    out["cg_small_iters_per_s"] = sweep
+    log("cg_small_iters_per_s", sweep)
     for name, fn in (("batched_cg_4096", bench_batched), ("mixed_ir_160cubed", bench_mixed),
                      ("adjoint_64x512sq", bench_adjoint)):
         try:

@@ -27,7 +27,7 @@ SRC_NONE, SRC_LEAF, SRC_IMM, SRC_STACK = range(4)
 (UN_NEG, UN_ABS, UN_ABS2, UN_INVERSE, UN_INVERSE_CHECK, UN_SQRT, UN_RSQRT, UN_CBRT, UN_EXP,
  UN_LOG, UN_LOG1P, UN_LOG10, UN_SIN, UN_COS, UN_TAN, UN_ASIN, UN_ACOS, UN_ATAN, UN_SINH,
  UN_COSH, UN_TANH, UN_FLOOR, UN_CEIL, UN_ROUND, UN_NOT, UN_BITNOT, UN_SIGN) = range(27)
-CG_AUTO, CG_MULTIKERNEL, CG_RESIDENT = 0, 1, 2
+CG_AUTO, CG_MULTIKERNEL, CG_RESIDENT, CG_FUSED = 0, 1, 2, 3
 MAX_LEAVES, MAX_CODE, MAX_STACK = 12, 48, 4
 
 

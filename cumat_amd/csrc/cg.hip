@@ -9,12 +9,14 @@
 //  MULTIKERNEL path (any n, up to 16 right-hand sides; cg_multi.hpp):
 //    setup   r = b - A x, p = M^-1 r, |b|^2, |r|^2, r.p, early outs
 //    per iteration, three launches (K1 SpMV + p.Ap partials, K2 x/r update
-//    + |r|^2, r.z partials, K3 convergence test + p update); consumers
+//    + |r|^2, r.z partials, K3 convergence test + p update) or, on the
+//    FUSED path, two (KA: convergence test of the previous iteration, p
+//    update fused into the SpMV, p.Ap partials; KB: as K2); consumers
 //    re-reduce the producers' partials in a fixed order, so no device-scope
 //    fence is needed. Kernels launched after convergence return at once;
 //    the host checks the status word every few iterations through pinned
 //    memory and stops launching.
-//  RESIDENT path (n <= 32768, one right-hand side): one workgroup runs the
+//  RESIDENT path (n <= 16384, one right-hand side): one workgroup runs the
 //    whole solve; p lives in LDS, x / r / M^-1 in registers, the reductions
 //    are DPP wave reductions + LDS, and no kernel boundary is crossed.
 //
@@ -663,7 +665,10 @@ int cm_pcg_solve(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int
     }
     CM_REQUIRE(JA && IA && vals && b && x, CM_E_ARG, "cm_pcg_solve: NULL pointer");
     const bool resident_ok = batches == 1 && n <= 16384;
-    if (path == CM_CG_AUTO) path = (resident_ok && n <= 16384) ? CM_CG_RESIDENT : CM_CG_MULTIKERNEL;
+    // measured on MI355X (FP32 2D Poisson, iterations/s): the one-workgroup
+    // solve wins up to 64^2 = 4096 rows (138 k vs 97 k), the multi-kernel
+    // path from 96^2 = 9216 rows (96 k vs 69 k); the crossover is ~6000 rows
+    if (path == CM_CG_AUTO) path = (resident_ok && n <= 6144) ? CM_CG_RESIDENT : CM_CG_MULTIKERNEL;
     if (path == CM_CG_RESIDENT) {
         CM_REQUIRE(resident_ok, CM_E_UNSUPPORTED, "resident CG needs one rhs and n <= 16384");
         if (dtype == CM_F32)
@@ -675,14 +680,17 @@ int cm_pcg_solve(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int
                                             error_out, stats);
         return cm::fail(CM_E_UNSUPPORTED, "cm_pcg_solve: dtype %d", dtype);
     }
+    CM_REQUIRE(path == CM_CG_MULTIKERNEL || path == CM_CG_FUSED, CM_E_ARG, "cm_pcg_solve: unknown path %d", path);
+    cm::CgBatchOpts opt;
+    opt.fused = path == CM_CG_FUSED;
     if (dtype == CM_F32)
         return cm::pcg_multikernel<float, float>(ctx, n, nnz, batches, JA, IA, (const float*)vals,
                                                  (const float*)invdiag, (const float*)b, (float*)x, use_guess, tol,
-                                                 max_iter, iterations_out, error_out, stats);
+                                                 max_iter, iterations_out, error_out, stats, nullptr, opt);
     if (dtype == CM_F64)
         return cm::pcg_multikernel<double, double>(ctx, n, nnz, batches, JA, IA, (const double*)vals,
                                                    (const double*)invdiag, (const double*)b, (double*)x, use_guess,
-                                                   tol, max_iter, iterations_out, error_out, stats);
+                                                   tol, max_iter, iterations_out, error_out, stats, nullptr, opt);
     return cm::fail(CM_E_UNSUPPORTED, "cm_pcg_solve: dtype %d", dtype);
     CM_API_END
 }

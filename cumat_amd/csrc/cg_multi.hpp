@@ -1,7 +1,7 @@
 // cg_multi.hpp -- the multi-kernel Jacobi-PCG (any n, up to CG_MAXB
 // right-hand sides), fence-free.
 //
-// Per iteration i, three launches:
+// Per iteration i, three launches (the default form):
 //   K1  tmp = A p, per-block partial p.tmp
 //   K2  every block reduces K1's partials of its batch in the same fixed
 //       order -> the same alpha = absNew[i&1] / p.tmp; x += alpha p,
@@ -9,6 +9,21 @@
 //   K3  every block reduces K2's partials of all batches -> convergence test
 //       and beta = rz / absNew[i&1]; p = M^-1 r + beta p; block 0 records
 //       absNew[(i+1)&1] = rz, the status, the iteration count and the error.
+// The FUSED form has two launches per iteration (p alternates between two
+// buffers); it saves the p re-read and a launch but triples the gathers of
+// the SpMV, which costs more on MI355X than it saves (94 us for KA against
+// 48 + 18 us for K1 + K3 at 2048^2), so it is not the default:
+//   KA  (i >= 1) every block reduces KB's partials of iteration i-1 for all
+//       batches in the same fixed order -> convergence test of iteration
+//       i-1 and beta = rz / absNew[(i-1)&1]; block 0 records absNew[i&1] = rz,
+//       the status, the iteration count and the error. Then
+//       p_i = M^-1 r + beta p_{i-1} and tmp = A p_i in one pass (gathered
+//       p_i values are recomputed from r, M^-1, p_{i-1}), partial p_i.tmp.
+//       (i == 0: p_0 from the setup, no test.)
+//   KB  every block reduces KA's partials of its batch -> the same
+//       alpha = absNew[i&1] / p.tmp; x += alpha p_i, r -= alpha tmp;
+//       per-block partials of |r|^2 and r.(M^-1 r)
+// After the last iteration one more KA completes its convergence test.
 // Cross-block data only flows through kernel boundaries (no device-scope
 // fences, which on a multi-XCD part write back the L2 each time), and every
 // reduction is done in one fixed order, so runs are bit-reproducible.
@@ -22,6 +37,8 @@
 // The grid shrinks as the batch count grows so that each reduction folds at
 // most CG_GMAX partials.
 #pragma once
+#include <cstdlib>
+
 #include "spmv_core.hpp"
 #include "spmv_wave.hpp"
 
@@ -55,12 +72,52 @@ struct CgState {
 
 // Sum of G partials (stride 'stride' between consecutive ones) in a fixed
 // order; identical in every block. lds: BS/64 entries.
+// All of a thread's loads (G <= CG_GMAX = 8 * CG_BS) are issued before the
+// first add, so the partials written by the previous launch (often in another
+// XCD's cache) cost one memory latency, not eight; the adds keep the order
+// i = tid, tid + BS, ... so the result does not depend on the unrolling.
+constexpr int CG_PPT = CG_GMAX / CG_BS;  // partials per thread
+static_assert(CG_PPT * CG_BS == CG_GMAX, "partials must fit one unrolled pass");
+
 template <typename T>
 __device__ __forceinline__ T reduce_partials(const T* __restrict__ part, int G, int stride, T* lds)
 {
+    T v[CG_PPT];
+#pragma unroll
+    for (int j = 0; j < CG_PPT; ++j) {
+        const int i = threadIdx.x + j * CG_BS;
+        v[j] = (i < G) ? part[(long long)i * stride] : T(0);
+    }
     T a = T(0);
-    for (int i = threadIdx.x; i < G; i += CG_BS) a += part[(long long)i * stride];
+#pragma unroll
+    for (int j = 0; j < CG_PPT; ++j)
+        if (threadIdx.x + j * CG_BS < G) a += v[j];
     return block_reduce<OpSum, CG_BS>(a, lds);
+}
+
+// Two interleaved partial sets (part[2i], part[2i+1]) reduced together;
+// lds needs 2*CG_BS/64 entries. Same order and results as two calls of
+// reduce_partials with stride 2.
+template <typename T>
+__device__ __forceinline__ void reduce_partials_pair(const T* __restrict__ part, int G, T& a, T& b, T* lds)
+{
+    T va[CG_PPT], vb[CG_PPT];
+#pragma unroll
+    for (int j = 0; j < CG_PPT; ++j) {
+        const int i = threadIdx.x + j * CG_BS;
+        const bool ok = i < G;
+        va[j] = ok ? part[2ll * i] : T(0);
+        vb[j] = ok ? part[2ll * i + 1] : T(0);
+    }
+    a = T(0);
+    b = T(0);
+#pragma unroll
+    for (int j = 0; j < CG_PPT; ++j)
+        if (threadIdx.x + j * CG_BS < G) {
+            a += va[j];
+            b += vb[j];
+        }
+    block_sum2<CG_BS>(a, b, lds);
 }
 
 template <typename T>
@@ -165,7 +222,8 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_finalize_kernel(int G, int B, 
     }
 }
 
-// ---------------------------------------------------------------- K1
+// ---------------------------------------------------------------- K1 (three-launch form)
+// tmp = A p, per-block partial p.tmp
 template <typename VT, typename T>
 __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
                                                         const VT* __restrict__ vals, long long vstride,
@@ -207,6 +265,138 @@ __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __rest
     }
     s = block_reduce<OpSum, CG_BS>(s, red);
     if (threadIdx.x == 0) part[(long long)bb * gridDim.x + blockIdx.x] = s;
+}
+
+// p_{i} = M^-1 r_i + beta p_{i-1} with one explicit rounding pattern, so that
+// the value a lane writes for its own row and the value other lanes
+// recompute for that column are bit-identical
+__device__ __forceinline__ float cg_pnext(float beta, float pold, float d, float r)
+{
+    return __builtin_fmaf(beta, pold, d * r);
+}
+__device__ __forceinline__ double cg_pnext(double beta, double pold, double d, double r)
+{
+    return __builtin_fma(beta, pold, d * r);
+}
+
+// ---------------------------------------------------------------- KA
+// Iteration 'it' (>= 1) first completes iteration it-1: every block folds
+// KB's partials of |r|^2 and r.(M^-1 r) of all batches in the same order ->
+// convergence test and beta; block (0, b) records batch b's state and block
+// (0, 0) the global status (the launch with it == max_iter only evaluates).
+// Then p_it = M^-1 r + beta p_{it-1} (written for own rows into the other p
+// buffer) and Ap_it: the gathered p_it[c] are recomputed from r, M^-1 and
+// p_{it-1} at column c, so the p update and the SpMV share one launch.
+// FIRST (it == 0): p_0 comes from the setup, no evaluation.
+// A block that returns early because block (0,0) already flagged the status
+// in this same launch would have returned anyway: within it < max_iter the
+// status only changes to "converged", which every block computes alike.
+template <typename VT, typename T, bool FIRST>
+__global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, int G2, const int* __restrict__ JA,
+                                                        const int* __restrict__ IA, const VT* __restrict__ vals,
+                                                        long long vstride, const T* __restrict__ invd, int istride,
+                                                        const T* __restrict__ r, const T* __restrict__ pprev,
+                                                        T* __restrict__ pnext, T* __restrict__ tmp,
+                                                        T* __restrict__ part1, const T* __restrict__ part2,
+                                                        CgState<T>* st, const T* __restrict__ diag)
+{
+    const int bb = blockIdx.y;
+    if (st->status != CG_RUNNING) return;
+    const bool per_batch = st->per_batch != 0;
+    // a batch that finished earlier has nothing to do; block (0,0) still
+    // evaluates, it owns the global status
+    if (per_batch && st->bdone[bb] != CG_RUNNING && !(FIRST == false && blockIdx.x == 0 && bb == 0)) return;
+    constexpr int WPB = CG_BS / 64;
+    __shared__ T lds[WPB][CG_EPT * 64];
+    __shared__ T red[2 * CG_BS / 64];
+    __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB];
+    T* slice = lds[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const long long off = (long long)bb * n;
+    const VT* vb = vals + bb * vstride;
+    const T* ib = invd + (long long)bb * istride;
+    const T* rb = r + off;
+    const T* pp = pprev + off;
+    T* pn = pnext + off;
+    ChunkWalk w(div_up_dev(n, 64), WPB);
+    WaveChunk<VT, CG_EPT> cur, nxt;
+    // the first chunk's matrix loads overlap the partial reductions below
+    if (w.valid()) chunk_load(cur, JA, IA, vb, w.cur * 64, n);
+    T beta = T(0);
+    if (!FIRST) {
+        const int prev = it - 1;
+        for (int k = 0; k < B; ++k) {
+            T a, c;
+            reduce_partials_pair(part2 + 2ll * k * G2, G2, a, c, red);
+            if (threadIdx.x == 0) { rr_s[k] = a; rz_s[k] = c; }
+        }
+        __syncthreads();
+        bool all = true;
+        T m = 0;
+        bool have_m = false;
+        for (int k = 0; k < B; ++k) {
+            const bool done = per_batch && st->bdone[k] != CG_RUNNING;
+            all = all && (done || rr_s[k] < st->thr[k]);
+            if (!done) {
+                T q = rr_s[k] / st->rhs2[k];
+                m = (!have_m || q > m) ? q : m;
+                have_m = true;
+            }
+        }
+        const bool own_prev = per_batch && st->bdone[bb] != CG_RUNNING;
+        const bool own_conv = own_prev || rr_s[bb] < st->thr[bb];
+        const T absOld = st->absNew[prev & 1][bb];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (per_batch && !own_prev) {
+                st->berr[bb] = sqrt((double)(rr_s[bb] / st->rhs2[bb]));
+                if (own_conv) {
+                    st->bdone[bb] = CG_CONVERGED;
+                    st->biters[bb] = prev;  // the converging step is not counted
+                } else {
+                    st->biters[bb] = it;
+                }
+            }
+            if (bb == 0) {
+                if (!per_batch) st->err = sqrt((double)m);
+                if (all) {
+                    st->status = CG_CONVERGED;
+                    st->iters = prev;
+                } else {
+                    st->iters = it;
+                    if (it >= st->max_iter) st->status = CG_MAXIT;
+                }
+            }
+            // with the reference's rule a converged batch keeps iterating until
+            // all have converged, so its r.z is recorded like any other
+            if (!(per_batch && own_conv)) st->absNew[it & 1][bb] = rz_s[bb];
+        }
+        if (per_batch ? own_conv : all) return;
+        if (it >= st->max_iter) return;
+        beta = rz_s[bb] / absOld;
+    }
+    T s = 0;
+    if (w.valid()) {
+        auto pval = [=](int c) -> T { return FIRST ? pp[c] : cg_pnext(beta, pp[c], ib[c], rb[c]); };
+        for (;;) {
+            const int nc = w.peek_next();
+            const bool more = nc < w.hi;
+            if (more) chunk_load(nxt, JA, IA, vb, nc * 64, n);
+            T ap = chunk_compute<VT, T, CG_EPT>(cur, JA, IA, vb, pval, slice);
+            if (lane < cur.nr) {
+                const int row = cur.r0 + lane;
+                const T pv = pval(row);
+                if (!FIRST) pn[row] = pv;
+                if (diag) ap += diag[row] * pv;
+                tmp[off + row] = ap;
+                s += pv * ap;
+            }
+            if (!more) break;
+            cur = nxt;
+            w.next();
+        }
+    }
+    s = block_reduce<OpSum, CG_BS>(s, red);
+    if (threadIdx.x == 0) part1[(long long)bb * gridDim.x + blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------- K2
@@ -267,7 +457,8 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1,
     }
 }
 
-// ---------------------------------------------------------------- K3
+// ---------------------------------------------------------------- K3 (three-launch form)
+// convergence test of iteration it and p = M^-1 r + beta p
 template <typename T>
 __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B, int G2, const T* __restrict__ part2,
                                                            const T* __restrict__ invd, int istride,
@@ -280,7 +471,7 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
     // a batch that finished in an earlier iteration has nothing to update;
     // block (0,0) still runs, it owns the global status
     if (per_batch && st->bdone[bb] != CG_RUNNING && !(blockIdx.x == 0 && bb == 0)) return;
-    __shared__ T red[CG_BS / 64];
+    __shared__ T red[2 * CG_BS / 64];
     __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB];
     const long long off = (long long)bb * n;
     const T* ib = invd + (long long)bb * istride;
@@ -301,8 +492,8 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
     // every block computes the same totals (fixed order); the partials of a
     // batch that finished earlier are stale and are not used
     for (int k = 0; k < B; ++k) {
-        T a = reduce_partials(part2 + 2ll * k * G2, G2, 2, red);
-        T c = reduce_partials(part2 + 2ll * k * G2 + 1, G2, 2, red);
+        T a, c;
+        reduce_partials_pair(part2 + 2ll * k * G2, G2, a, c, red);
         if (threadIdx.x == 0) { rr_s[k] = a; rz_s[k] = c; }
     }
     __syncthreads();
@@ -401,6 +592,9 @@ struct CgBatchOpts {
     int per_batch = 0;      // per-batch convergence
     int* biters = nullptr;  // host, B entries (per-batch mode)
     double* berr = nullptr; // host, B entries (per-batch mode)
+    // two launches per iteration (p update fused into the SpMV, gathered p
+    // recomputed from r, M^-1, p_prev) instead of three
+    bool fused = false;
 };
 
 template <typename VT, typename T>
@@ -416,7 +610,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     const long long N = (long long)n * B;
     void *r_, *p_, *t_, *st_, *p1_, *p2_;
     CM_TRY(tmp.get(sizeof(T) * N, &r_));
-    CM_TRY(tmp.get(sizeof(T) * N, &p_));
+    CM_TRY(tmp.get(sizeof(T) * 2 * N, &p_));  // p of even / odd iterations
     CM_TRY(tmp.get(sizeof(T) * N, &t_));
     CM_TRY(tmp.get(sizeof(CgState<T>), &st_));
     // grids: multiples of 8 (one share per XCD); with B batches each gets at
@@ -429,7 +623,8 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     CM_TRY(tmp.get(sizeof(T) * 3 * (size_t)B * std::max(G1, G2), &p1_));
     CM_TRY(tmp.get(sizeof(T) * 2 * (size_t)B * G2, &p2_));
     T* r = (T*)r_;
-    T* p = (T*)p_;
+    T* pbuf[2] = {(T*)p_, (T*)p_ + N};
+    T* p = pbuf[0];
     T* t = (T*)t_;
     T* part1 = (T*)p1_;
     T* part2 = (T*)p2_;
@@ -448,6 +643,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         invd = (const T*)id_;
         istride = opt.vals_batched ? n : 0;
     }
+    const bool fused = opt.fused;
     int launches = 0;
     auto t0 = std::chrono::steady_clock::now();
     cg_state_init_kernel<T><<<1, 64, 0, ctx->stream>>>(st, max_iter, B, opt.per_batch);
@@ -469,15 +665,28 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     while (it < max_iter) {
         int todo = std::min(CHUNK, max_iter - it);
         for (int k = 0; k < todo; ++k, ++it) {
-            cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1, st,
-                                                                          diag);
-            cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, istride, p, t, x, r,
-                                                                        part2, st);
-            cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, B, G2, part2, invd, istride, r, p,
-                                                                         st);
+            if (!fused) {
+                cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1,
+                                                                              st, diag);
+                cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, istride, p, t,
+                                                                            x, r, part2, st);
+                cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, B, G2, part2, invd, istride, r,
+                                                                             p, st);
+                continue;
+            }
+            T* pcur = pbuf[it & 1];
+            if (it == 0)
+                cg_iter_kernel<VT, T, true><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(
+                    n, it, B, G2, JA, IA, vals, vstride, invd, istride, r, pcur, pcur, t, part1, part2, st, diag);
+            else
+                cg_iter_kernel<VT, T, false><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(
+                    n, it, B, G2, JA, IA, vals, vstride, invd, istride, r, pbuf[(it - 1) & 1], pcur, t, part1,
+                    part2, st, diag);
+            cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, istride, pcur, t, x,
+                                                                        r, part2, st);
         }
         CM_LAUNCH_CHECK();
-        launches += 3 * todo;
+        launches += (fused ? 2 : 3) * todo;
         // lagged status check: read the previous chunk's word without draining
         // the queue, then request the current one
         if (pending) {
@@ -487,6 +696,15 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         CM_HIP(hipMemcpyAsync(hstat, &st->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         CM_HIP(hipEventRecord(ev, ctx->stream));
         pending = true;
+    }
+    // completes the last launched iteration (convergence test / iteration
+    // limit); a no-op once the status is final
+    if (fused && it >= 1) {
+        cg_iter_kernel<VT, T, false><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(
+            n, it, B, G2, JA, IA, vals, vstride, invd, istride, r, pbuf[(it - 1) & 1], pbuf[it & 1], t, part1, part2,
+            st, diag);
+        CM_LAUNCH_CHECK();
+        launches += 1;
     }
     cg_finalize_kernel<T><<<dim3(div_up(n, 256), B), 256, 0, ctx->stream>>>(n, x, st);
     CM_LAUNCH_CHECK();
@@ -514,7 +732,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         if (err_out) *err_out = hs.err;
     }
     if (stats) {
-        stats->path_used = CM_CG_MULTIKERNEL;
+        stats->path_used = fused ? CM_CG_FUSED : CM_CG_MULTIKERNEL;
         stats->launches = launches;
         stats->seconds = std::chrono::duration<double>(t1 - t0).count();
     }

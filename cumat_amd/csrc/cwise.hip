@@ -680,6 +680,64 @@ static void launch_lincomb_k(cm_ctx_t ctx, const LincombArgs& A, float* out, lon
     lincomb_kernel<K><<<(unsigned)blocks, 256, 0, ctx->stream>>>(A, out, n4);
 }
 
+// out[0..n) = sum_j A.c[j] * A.v[j][0..n): 16-byte path for the aligned
+// prefix, scalar tail
+static int lincomb_run(cm_ctx_t ctx, const LincombArgs& A, int k, float* out, long long n)
+{
+    bool aligned = ((uintptr_t)out & 15) == 0;
+    for (int j = 0; j < k; ++j) aligned = aligned && (((uintptr_t)A.v[j] & 15) == 0);
+    long long n4 = aligned ? n / 4 : 0;
+    if (n4 > 0) {
+        switch (k) {
+#define LC_CASE(K) case K: launch_lincomb_k<K>(ctx, A, out, n4); break;
+            LC_CASE(1) LC_CASE(2) LC_CASE(3) LC_CASE(4) LC_CASE(5) LC_CASE(6) LC_CASE(7) LC_CASE(8)
+            LC_CASE(9) LC_CASE(10) LC_CASE(11) LC_CASE(12) LC_CASE(13) LC_CASE(14) LC_CASE(15) LC_CASE(16)
+#undef LC_CASE
+        default: break;
+        }
+        CM_LAUNCH_CHECK();
+    }
+    long long start = n4 * 4;
+    if (start < n) {
+        lincomb_tail_kernel<<<div_up(n - start, 256), 256, 0, ctx->stream>>>(A, k, out, start, n);
+        CM_LAUNCH_CHECK();
+    }
+    return CM_OK;
+}
+
+// A flat FP32 program of the form  v0 [* c0] + v1*c1 + ... (one LOAD, an
+// optional multiply by an immediate, then FMA terms) assigned with '=' is a
+// linear combination: it runs on the dedicated streaming kernel.
+static bool match_lincomb(const CwProgram& P, LincombArgs& A, int& k)
+{
+    auto vec_leaf = [&](int l) {
+        return l >= 0 && l < P.nleaves && P.leaves[l].dtype == CM_F32 && P.leaves[l].kind == CM_LEAF_STRIDED &&
+               ((P.leaf_direct >> l) & 1) && !((P.leaf_scalar >> l) & 1);
+    };
+    if (P.ncode < 1 || P.ncode > LC_MAX + 1) return false;
+    const cm_instr& I0 = P.code[0];
+    if (I0.opcode != CM_OP_LOAD || I0.src != CM_SRC_LEAF || I0.dtype != CM_F32 || !vec_leaf(I0.arg)) return false;
+    memset(&A, 0, sizeof(A));
+    A.v[0] = (const float*)P.leaves[I0.arg].ptr;
+    A.c[0] = 1.0f;
+    k = 1;
+    int pc = 1;
+    if (pc < P.ncode && P.code[pc].opcode == CM_OP_BINARY) {
+        const cm_instr& I = P.code[pc];
+        if (I.fn != CM_BIN_MUL || I.src != CM_SRC_IMM || I.dtype != CM_F32) return false;
+        A.c[0] = (float)I.fimm;
+        ++pc;
+    }
+    for (; pc < P.ncode; ++pc) {
+        const cm_instr& I = P.code[pc];
+        if (I.opcode != CM_OP_FMA || I.dtype != CM_F32 || !vec_leaf(I.arg) || k >= LC_MAX) return false;
+        A.v[k] = (const float*)P.leaves[I.arg].ptr;
+        A.c[k] = (float)I.fimm;
+        ++k;
+    }
+    return k >= 2;
+}
+
 }  // namespace cm
 
 extern "C" {
@@ -729,6 +787,11 @@ int cm_cwise_eval(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_ins
         else if (direct) P.leaf_direct |= 1u << l;
         else flat = false;
     }
+    if (!wide && flat && P.dst_direct && mode == CM_ASSIGN && dst_dtype == CM_F32) {
+        cm::LincombArgs A;
+        int k = 0;
+        if (cm::match_lincomb(P, A, k)) return cm::lincomb_run(ctx, A, k, (float*)dst, n);
+    }
     if (wide) return cm::launch_dense_w<true>(ctx, P, flat);
     return cm::launch_dense_w<false>(ctx, P, flat);
     CM_API_END
@@ -745,30 +808,12 @@ int cm_lincomb_f32(cm_ctx_t ctx, int k, const float* const* vecs, const float* c
     if (n == 0) return CM_OK;
     cm::LincombArgs A;
     memset(&A, 0, sizeof(A));
-    bool aligned = ((uintptr_t)out & 15) == 0;
     for (int j = 0; j < k; ++j) {
         CM_REQUIRE(vecs[j], CM_E_ARG, "cm_lincomb_f32: vector %d is NULL", j);
         A.v[j] = vecs[j];
         A.c[j] = coeffs[j];
-        aligned = aligned && (((uintptr_t)vecs[j] & 15) == 0);
     }
-    long long n4 = aligned ? n / 4 : 0;
-    if (n4 > 0) {
-        switch (k) {
-#define LC_CASE(K) case K: cm::launch_lincomb_k<K>(ctx, A, out, n4); break;
-            LC_CASE(1) LC_CASE(2) LC_CASE(3) LC_CASE(4) LC_CASE(5) LC_CASE(6) LC_CASE(7) LC_CASE(8)
-            LC_CASE(9) LC_CASE(10) LC_CASE(11) LC_CASE(12) LC_CASE(13) LC_CASE(14) LC_CASE(15) LC_CASE(16)
-#undef LC_CASE
-        default: break;
-        }
-        CM_LAUNCH_CHECK();
-    }
-    long long start = n4 * 4;
-    if (start < n) {
-        cm::lincomb_tail_kernel<<<cm::div_up(n - start, 256), 256, 0, ctx->stream>>>(A, k, out, start, n);
-        CM_LAUNCH_CHECK();
-    }
-    return CM_OK;
+    return cm::lincomb_run(ctx, A, k, out, n);
     CM_API_END
 }
 

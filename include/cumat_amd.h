@@ -290,7 +290,13 @@ int cm_csr_inverse_diagonal(cm_ctx_t ctx, int dtype, int rows, int nnz,
  * then computed). x holds the guess on entry if use_guess, else is zeroed.
  * iterations_out / error_out are host pointers.
  * ====================================================================== */
-enum cm_cg_path { CM_CG_AUTO = 0, CM_CG_MULTIKERNEL = 1, CM_CG_RESIDENT = 2 };
+/* Launch shapes: AUTO picks RESIDENT for one small system (n <= 6144) and
+ * MULTIKERNEL otherwise. MULTIKERNEL: three launches per iteration (SpMV,
+ * x/r update, p update). FUSED: two launches per iteration (the p update is
+ * folded into the SpMV launch, each gathered p value being recomputed from
+ * r, M^-1 and the previous p). RESIDENT: the whole solve in one workgroup
+ * (n <= 16384, one right-hand side). All give the reference's semantics. */
+enum cm_cg_path { CM_CG_AUTO = 0, CM_CG_MULTIKERNEL = 1, CM_CG_RESIDENT = 2, CM_CG_FUSED = 3 };
 
 typedef struct cm_cg_stats {
     int32_t path_used;        /* cm_cg_path that ran                         */

@@ -14,7 +14,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-PATHS = [1, 2]  # multikernel, resident
+PATHS = [1, 2, 3]  # multikernel, resident, fused multikernel
 
 
 def _poisson(cm, g, shift=0.0, dtype=np.float32):
@@ -159,3 +159,35 @@ def test_large_poisson_2048_residual(gpu):
     assert cg.error() <= 1e-3 and 0 < cg.iterations() < 20000
     r = A.matvec(xd).numpy().astype(np.float64) - b
     assert np.linalg.norm(r) / np.linalg.norm(b) < 3e-3
+
+
+def test_auto_path_choice(gpu):
+    """AUTO: one workgroup for a small single system, the multi-kernel
+    launches for larger ones and for several right-hand sides."""
+    cm = gpu
+    for g, expect in ((32, 2), (64, 2), (96, 1), (200, 1)):
+        A, JA, IA, V, n = _poisson(cm, g)
+        cg = cm.ConjugateGradient(A).setTolerance(1e-4)
+        cg.solve(cm.DeviceArray.from_numpy(np.ones(n, np.float32)))
+        assert cg.last_stats["path"] == expect, (g, cg.last_stats)
+    A, JA, IA, V, n = _poisson(cm, 16)
+    cg = cm.ConjugateGradient(A).setTolerance(1e-4)
+    cg.solve(cm.DeviceArray.from_numpy(np.ones((2, n, 1), np.float32)))
+    assert cg.last_stats["path"] == 1
+
+
+def test_fused_and_three_launch_agree(gpu):
+    """The two-launch iteration computes p with an explicit multiply-add and
+    recomputes gathered p values, so iterates agree with the three-launch
+    form to rounding, with the same iteration count within 1 (FP64)."""
+    cm = gpu
+    A, JA, IA, V, n = _poisson(cm, 150, 0.01, np.float64)
+    b = np.random.default_rng(15).uniform(-1, 1, (3, n, 1))
+    out = {}
+    for path in (1, 3):
+        cg = cm.ConjugateGradient(A, path=path).setTolerance(1e-11).setMaxIterations(5000)
+        x = cg.solve(cm.DeviceArray.from_numpy(b)).numpy()
+        out[path] = (x, cg.iterations(), cg.error())
+        assert cg.last_stats["path"] == path
+    assert abs(out[1][1] - out[3][1]) <= 1
+    np.testing.assert_allclose(out[1][0], out[3][0], rtol=0, atol=1e-8)

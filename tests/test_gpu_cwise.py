@@ -170,6 +170,12 @@ def test_lincomb8_fused(gpu, orc):
     exp = sum(v.astype(np.float64) * float(c) for v, c in zip(vs, cs))
     err = np.abs(got - exp).max() / np.abs(exp).max()
     assert err < 1e-6
+    # '+=' into zeros runs the general program interpreter (the '=' form is
+    # dispatched to the streaming linear-combination kernel): same values
+    acc = cm.DeviceArray(n, dtype=np.float32)
+    acc.fill_zero()
+    cm.assign(acc, e, mode=cm.capi.ADD)
+    assert np.abs(acc.numpy() - exp).max() / np.abs(exp).max() < 1e-6
     out = cm.DeviceArray(n, dtype=np.float32)
     ptrs = (C.c_void_p * 8)(*[d.ptr for d in dv])
     coef = (C.c_float * 8)(*cs)

@@ -134,6 +134,33 @@ def test_spmv_ragged_and_empty_rows(gpu, orc):
     assert np.array_equal(y[short], exp[short])
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_spmv_staging_boundaries_batched(gpu, orc, seed):
+    """Rows of 0..16 entries put the 64-row chunks on both sides of the
+    staging capacity (512 entries) with every start offset mod 4 (16-byte
+    and element-wise staging, and the wavefront-per-row path); three value
+    batches with nnz % 4 != 0 make every batch but the first misaligned for
+    16-byte loads. Integer-valued operands: exact in FP32, so bit-exact."""
+    cm = gpu
+    rng = np.random.default_rng(100 + seed)
+    rows, cols, B = 64 * 150 + 17, 5000, 3
+    lens = rng.integers(0, 17, rows)
+    JA = np.zeros(rows + 1, np.int32)
+    JA[1:] = np.cumsum(lens)
+    if JA[-1] % 4 == 0:
+        lens[-1] += 1
+        JA[1:] = np.cumsum(lens)
+    IA = np.concatenate([np.sort(rng.choice(cols, l, replace=False)) for l in lens]).astype(np.int32)
+    nnz = IA.size
+    assert nnz % 4 != 0
+    V = rng.integers(-8, 9, nnz * B).astype(np.float32)
+    x = rng.integers(-8, 9, cols * B).astype(np.float32)
+    A = cm.SparseMatrix(cm.SparsityPattern(rows, cols, JA, IA), V, batches=B)
+    y = A.matvec(cm.DeviceArray.from_numpy(x, rows=cols, cols=1, batches=B)).numpy()
+    exp = orc.csr_spmv(JA, IA, V, x, rows, cols, batches=B, vals_batched=True, x_batched=True)
+    assert np.array_equal(y.reshape(-1), exp.reshape(-1))
+
+
 def test_spmv_int32_and_f64(gpu, orc):
     cm = gpu
     from cumat_amd.problems import poisson2d
