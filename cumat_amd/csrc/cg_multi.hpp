@@ -245,11 +245,11 @@ __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __rest
     ChunkWalk w(div_up_dev(n, 64), WPB);
     if (w.valid()) {
         WaveChunk<VT, CG_EPT> cur, nxt;
-        chunk_load(cur, JA, IA, vb, w.cur * 64, n);
+        chunk_load<VT, CG_EPT, false>(cur, JA, IA, vb, w.cur * 64, n);
         for (;;) {
             const int nc = w.peek_next();
             const bool more = nc < w.hi;
-            if (more) chunk_load(nxt, JA, IA, vb, nc * 64, n);
+            if (more) chunk_load<VT, CG_EPT, false>(nxt, JA, IA, vb, nc * 64, n);
             T ap = chunk_compute<VT, T, CG_EPT>(cur, JA, IA, vb, [pb](int c) { return pb[c]; }, slice);
             if (lane < cur.nr) {
                 const long long row = cur.r0 + lane;
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
     ChunkWalk w(div_up_dev(n, 64), WPB);
     WaveChunk<VT, CG_EPT> cur, nxt;
     // the first chunk's matrix loads overlap the partial reductions below
-    if (w.valid()) chunk_load(cur, JA, IA, vb, w.cur * 64, n);
+    if (w.valid()) chunk_load<VT, CG_EPT, false>(cur, JA, IA, vb, w.cur * 64, n);
     T beta = T(0);
     if (!FIRST) {
         const int prev = it - 1;
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
         for (;;) {
             const int nc = w.peek_next();
             const bool more = nc < w.hi;
-            if (more) chunk_load(nxt, JA, IA, vb, nc * 64, n);
+            if (more) chunk_load<VT, CG_EPT, false>(nxt, JA, IA, vb, nc * 64, n);
             T ap = chunk_compute<VT, T, CG_EPT>(cur, JA, IA, vb, pval, slice);
             if (lane < cur.nr) {
                 const int row = cur.r0 + lane;

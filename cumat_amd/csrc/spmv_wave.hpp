@@ -4,7 +4,7 @@
 // (lane l sums row r0 + l). For a chunk whose non-zeros fit the lane budget
 // (EPT per lane, i.e. EPT average non-zeros per row), the wavefront
 //   1. loads the chunk's column indices and values with fully coalesced
-//      (non-temporal) loads, EPT per lane, plus each lane's row bounds;
+//      loads, EPT per lane, plus each lane's row bounds;
 //   2. gathers x at those columns and parks the products in its private LDS
 //      slice (no workgroup barrier: only this wavefront touches the slice);
 //   3. sums each lane's row sequentially in CSR order (the reference's
@@ -49,7 +49,18 @@ struct ChunkWalk {
     __device__ void next() { cur += step; }
 };
 
-template <typename VT, int EPT>
+template <bool NT, typename X>
+__device__ __forceinline__ X ld_matrix(const X* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+// NT: non-temporal (streaming) loads of the matrix. A single product streams
+// a matrix that is not read again soon; an iterative solver reads the same
+// matrix every iteration, and plain loads let the 256 MB last-level cache
+// keep part of it from one iteration to the next.
+template <typename VT, int EPT, bool NT = true>
 __device__ __forceinline__ void chunk_load(WaveChunk<VT, EPT>& C, const int* __restrict__ JA,
                                            const int* __restrict__ IA, const VT* __restrict__ vals, int r0,
                                            int nrows)
@@ -66,8 +77,8 @@ __device__ __forceinline__ void chunk_load(WaveChunk<VT, EPT>& C, const int* __r
         for (int j = 0; j < EPT; ++j) {
             const int k = C.s + lane + 64 * j;
             const bool ok = k < C.e;
-            C.col[j] = ok ? __builtin_nontemporal_load(IA + k) : 0;
-            C.v[j] = ok ? __builtin_nontemporal_load(vals + k) : VT(0);
+            C.col[j] = ok ? ld_matrix<NT>(IA + k) : 0;
+            C.v[j] = ok ? ld_matrix<NT>(vals + k) : VT(0);
         }
     }
 }

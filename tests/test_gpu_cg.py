@@ -191,3 +191,4 @@ def test_fused_and_three_launch_agree(gpu):
         assert cg.last_stats["path"] == path
     assert abs(out[1][1] - out[3][1]) <= 1
     np.testing.assert_allclose(out[1][0], out[3][0], rtol=0, atol=1e-8)
+
