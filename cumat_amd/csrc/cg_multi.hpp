@@ -95,19 +95,21 @@ __device__ __forceinline__ T reduce_partials(const T* __restrict__ part, int G, 
     return block_reduce<OpSum, CG_BS>(a, lds);
 }
 
-// Two interleaved partial sets (part[2i], part[2i+1]) reduced together;
-// lds needs 2*CG_BS/64 entries. Same order and results as two calls of
+// Two interleaved partial sets (part[2i], part[2i+1]) reduced together
+// (part must be 2*sizeof(T)-aligned); lds needs 2*CG_BS/64 entries. Same order and results as two calls of
 // reduce_partials with stride 2.
 template <typename T>
 __device__ __forceinline__ void reduce_partials_pair(const T* __restrict__ part, int G, T& a, T& b, T* lds)
 {
+    typedef T T2 __attribute__((ext_vector_type(2)));
     T va[CG_PPT], vb[CG_PPT];
 #pragma unroll
     for (int j = 0; j < CG_PPT; ++j) {
         const int i = threadIdx.x + j * CG_BS;
-        const bool ok = i < G;
-        va[j] = ok ? part[2ll * i] : T(0);
-        vb[j] = ok ? part[2ll * i + 1] : T(0);
+        T2 v = {T(0), T(0)};
+        if (i < G) v = *reinterpret_cast<const T2*>(part + 2ll * i);  // one load per pair
+        va[j] = v.x;
+        vb[j] = v.y;
     }
     a = T(0);
     b = T(0);
@@ -429,9 +431,10 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1,
             dv[j] = ok ? ib[i] : T(0);
         }
     };
+    const T rz = st->absNew[it & 1][bb];
     load(i0);
     const T pap = reduce_partials(part1 + (long long)bb * G1, G1, 1, red);
-    const T alpha = st->absNew[it & 1][bb] / pap;
+    const T alpha = rz / pap;
     T srr = 0, srz = 0;
     while (i0 < n) {
 #pragma unroll
@@ -472,7 +475,18 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
     // block (0,0) still runs, it owns the global status
     if (per_batch && st->bdone[bb] != CG_RUNNING && !(blockIdx.x == 0 && bb == 0)) return;
     __shared__ T red[2 * CG_BS / 64];
-    __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB];
+    __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB], thr_s[CG_MAXB], rhs_s[CG_MAXB];
+    __shared__ int done_s[CG_MAXB];
+    // the state the decisions below read, fetched before the reductions so
+    // that its latency overlaps theirs (and every block sees the values as
+    // they were when this launch started)
+    if (threadIdx.x < B) {
+        thr_s[threadIdx.x] = st->thr[threadIdx.x];
+        rhs_s[threadIdx.x] = st->rhs2[threadIdx.x];
+        done_s[threadIdx.x] = per_batch ? st->bdone[threadIdx.x] : (int)CG_RUNNING;
+    }
+    const T absOld = st->absNew[it & 1][bb];
+    const int max_iter = st->max_iter;
     const long long off = (long long)bb * n;
     const T* ib = invd + (long long)bb * istride;
     const long long stride = (long long)gridDim.x * CG_BS;
@@ -501,20 +515,19 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
     T m = 0;
     bool have_m = false;
     for (int k = 0; k < B; ++k) {
-        const bool prev = per_batch && st->bdone[k] != CG_RUNNING;
-        all = all && (prev || rr_s[k] < st->thr[k]);
+        const bool prev = done_s[k] != CG_RUNNING;
+        all = all && (prev || rr_s[k] < thr_s[k]);
         if (!prev) {
-            T q = rr_s[k] / st->rhs2[k];
+            T q = rr_s[k] / rhs_s[k];
             m = (!have_m || q > m) ? q : m;
             have_m = true;
         }
     }
-    const bool own_prev = per_batch && st->bdone[bb] != CG_RUNNING;
-    const bool own_conv = own_prev || rr_s[bb] < st->thr[bb];
-    const T absOld = st->absNew[it & 1][bb];
+    const bool own_prev = done_s[bb] != CG_RUNNING;
+    const bool own_conv = own_prev || rr_s[bb] < thr_s[bb];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (per_batch && !own_prev) {
-            st->berr[bb] = sqrt((double)(rr_s[bb] / st->rhs2[bb]));
+            st->berr[bb] = sqrt((double)(rr_s[bb] / rhs_s[bb]));
             if (own_conv) {
                 st->bdone[bb] = CG_CONVERGED;
                 st->biters[bb] = it;  // the converging step is not counted
@@ -529,7 +542,7 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B,
                 st->iters = it;
             } else {
                 st->iters = it + 1;
-                if (it + 1 >= st->max_iter) st->status = CG_MAXIT;
+                if (it + 1 >= max_iter) st->status = CG_MAXIT;
             }
         }
         // with the reference's rule a converged batch keeps iterating until
