@@ -38,6 +38,8 @@
 // most CG_GMAX partials.
 #pragma once
 #include <cstdlib>
+#include <map>
+#include <type_traits>
 
 #include "spmv_core.hpp"
 #include "spmv_wave.hpp"
@@ -64,6 +66,7 @@ struct CgState {
     double berr[CG_MAXB];
     int status;
     int iters;
+    int it;        // index of the running iteration (three-launch form)
     int max_iter;
     int batches;
     int per_batch;
@@ -230,10 +233,15 @@ template <typename VT, typename T>
 __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
                                                         const VT* __restrict__ vals, long long vstride,
                                                         const T* __restrict__ p, T* __restrict__ tmp,
-                                                        T* __restrict__ part, const CgState<T>* st,
+                                                        T* __restrict__ part, CgState<T>* st,
                                                         const T* __restrict__ diag)
 {
     const int bb = blockIdx.y;
+    // K1 owns the iteration counter that K2 and K3 read: no launch argument
+    // changes from one iteration to the next, so iterations can be replayed
+    // from a captured graph (K1 itself never reads it, K2 / K3 read it after
+    // this launch has completed)
+    if (blockIdx.x == 0 && bb == 0 && threadIdx.x == 0 && st->status == CG_RUNNING) st->it = st->it + 1;
     if (batch_inactive(st, bb)) return;
     constexpr int WPB = CG_BS / 64;
     __shared__ T lds[WPB][CG_EPT * 64];
@@ -405,7 +413,7 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
 // The first sweep's operands are loaded before the partials of K1 are
 // reduced, so the streaming loads overlap that reduction's latency.
 template <typename T>
-__global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1, const T* __restrict__ part1,
+__global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int G1, const T* __restrict__ part1,
                                                           const T* __restrict__ invd, int istride,
                                                           const T* __restrict__ p, const T* __restrict__ tmp,
                                                           T* __restrict__ x, T* __restrict__ r,
@@ -413,6 +421,7 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1,
 {
     const int bb = blockIdx.y;
     if (batch_inactive(st, bb)) return;
+    const int it = it_arg >= 0 ? it_arg : st->it;  // < 0: the counter kept by K1
     __shared__ T red[2 * CG_BS / 64];
     const long long off = (long long)bb * n;
     const T* ib = invd + (long long)bb * istride;
@@ -463,13 +472,14 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it, int G1,
 // ---------------------------------------------------------------- K3 (three-launch form)
 // convergence test of iteration it and p = M^-1 r + beta p
 template <typename T>
-__global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it, int B, int G2, const T* __restrict__ part2,
+__global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, int B, int G2, const T* __restrict__ part2,
                                                            const T* __restrict__ invd, int istride,
                                                            const T* __restrict__ r, T* __restrict__ p,
                                                            CgState<T>* st)
 {
     const int bb = blockIdx.y;
     if (st->status != CG_RUNNING) return;
+    const int it = it_arg >= 0 ? it_arg : st->it;  // < 0: the counter kept by K1
     const bool per_batch = st->per_batch != 0;
     // a batch that finished in an earlier iteration has nothing to update;
     // block (0,0) still runs, it owns the global status
@@ -579,6 +589,7 @@ __global__ void cg_state_init_kernel(CgState<T>* st, int max_iter, int batches, 
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->status = CG_RUNNING;
         st->iters = 0;
+        st->it = -1;
         st->max_iter = max_iter;
         st->batches = batches;
         st->per_batch = per_batch;
@@ -600,6 +611,86 @@ __global__ void invdiag_kernel(int n, long long nnz, const int* __restrict__ JA,
 }
 
 // ------------------------------------------------------------- host driver
+// One three-launch iteration. The iteration index lives on the device (K1
+// advances it), so consecutive iterations have identical launch arguments.
+template <typename VT, typename T>
+static void launch_cg_iteration(cm_ctx_t ctx, int n, int B, int G1, int G2, const int* JA, const int* IA,
+                                const VT* vals, long long vstride, const T* invd, int istride, T* p, T* t, T* x, T* r,
+                                T* part1, T* part2, CgState<T>* st, const T* diag)
+{
+    cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1, st, diag);
+    cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x, r, part2,
+                                                                st);
+    cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r, p, st);
+}
+
+// Captured graphs of 'chunk' iterations, keyed by every launch argument (the
+// solver's work buffers come from the caching allocator, so repeated solves
+// of one size find their graph again).
+struct CgGraphKey {
+    cm_ctx_t ctx;
+    const void* ptrs[13];
+    long long vstride;
+    int n, B, G1, G2, istride, chunk, vt_size, t_size, vt_bf16;
+    bool operator<(const CgGraphKey& o) const { return memcmp(this, &o, sizeof(CgGraphKey)) < 0; }
+};
+static std::mutex g_cg_graph_mutex;
+static std::map<CgGraphKey, hipGraphExec_t> g_cg_graphs;
+constexpr size_t CG_GRAPH_CACHE_MAX = 64;
+
+template <typename VT, typename T>
+static int cg_chunk_graph(cm_ctx_t ctx, int chunk, int n, int B, int G1, int G2, const int* JA, const int* IA,
+                          const VT* vals, long long vstride, const T* invd, int istride, T* p, T* t, T* x, T* r,
+                          T* part1, T* part2, CgState<T>* st, const T* diag, hipGraphExec_t* out)
+{
+    CgGraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.ctx = ctx;
+    const void* ps[13] = {JA, IA, vals, invd, p, t, x, r, part1, part2, st, diag, ctx->stream};
+    memcpy(key.ptrs, ps, sizeof(ps));
+    key.vstride = vstride;
+    key.n = n;
+    key.B = B;
+    key.G1 = G1;
+    key.G2 = G2;
+    key.istride = istride;
+    key.chunk = chunk;
+    key.vt_size = (int)sizeof(VT);
+    key.t_size = (int)sizeof(T);
+    key.vt_bf16 = std::is_same<VT, unsigned short>::value ? 1 : 0;
+    std::lock_guard<std::mutex> lk(g_cg_graph_mutex);
+    auto it = g_cg_graphs.find(key);
+    if (it != g_cg_graphs.end()) {
+        *out = it->second;
+        return CM_OK;
+    }
+    if (g_cg_graphs.size() >= CG_GRAPH_CACHE_MAX) {
+        for (auto& kv : g_cg_graphs) (void)hipGraphExecDestroy(kv.second);
+        g_cg_graphs.clear();
+    }
+    *out = nullptr;
+    // the legacy default stream cannot be captured: such callers keep
+    // ordinary launches
+    if (ctx->stream == nullptr) return CM_OK;
+    hipGraph_t g;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return CM_OK;
+    }
+    for (int k = 0; k < chunk; ++k)
+        launch_cg_iteration<VT, T>(ctx, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x, r, part1, part2,
+                                   st, diag);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    CM_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    CM_HIP(e);
+    g_cg_graphs.emplace(key, exec);
+    *out = exec;
+    return CM_OK;
+}
+
 struct CgBatchOpts {
     int vals_batched = 0;   // matrix values per batch (stride nnz)
     int per_batch = 0;      // per-batch convergence
@@ -675,16 +766,31 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     const int CHUNK = 8;
     bool pending = false;
     int it = 0;
+    // long solves replay a captured graph of CHUNK three-launch iterations
+    // (one launch call instead of 3*CHUNK)
+    hipGraphExec_t chunk_graph = nullptr;
+    const bool use_graph = !fused && max_iter >= 2 * CHUNK;
+    bool graph_tried = false;
     while (it < max_iter) {
         int todo = std::min(CHUNK, max_iter - it);
-        for (int k = 0; k < todo; ++k, ++it) {
+        // captured after the first chunk has run with ordinary launches, so
+        // every kernel has been loaded before the capture
+        if (use_graph && !graph_tried && it >= CHUNK) {
+            graph_tried = true;
+            CM_TRY((cg_chunk_graph<VT, T>(ctx, CHUNK, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x,
+                                          r, part1, part2, st, diag, &chunk_graph)));
+        }
+        if (chunk_graph) {
+            // CHUNK iterations in one graph launch; iterations past max_iter
+            // are no-ops (the status is final by then)
+            CM_HIP(hipGraphLaunch(chunk_graph, ctx->stream));
+            it += CHUNK;
+            todo = CHUNK;
+        }
+        for (int k = 0; k < todo && !chunk_graph; ++k, ++it) {
             if (!fused) {
-                cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1,
-                                                                              st, diag);
-                cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, istride, p, t,
-                                                                            x, r, part2, st);
-                cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, B, G2, part2, invd, istride, r,
-                                                                             p, st);
+                launch_cg_iteration<VT, T>(ctx, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x, r, part1,
+                                           part2, st, diag);
                 continue;
             }
             T* pcur = pbuf[it & 1];
