@@ -4,11 +4,15 @@
 // Per iteration i, three launches (the default form):
 //   K1  tmp = A p, per-block partial p.tmp
 //   K2  every block reduces K1's partials of its batch in the same fixed
-//       order -> the same alpha = absNew[i&1] / p.tmp; x += alpha p,
-//       r -= alpha tmp; per-block partials of |r|^2 and r.(M^-1 r)
+//       order -> the same alpha = absNew[i&1] / p.tmp; r -= alpha tmp;
+//       per-block partials of |r|^2 and r.(M^-1 r)
 //   K3  every block reduces K2's partials of all batches -> convergence test
-//       and beta = rz / absNew[i&1]; p = M^-1 r + beta p; block 0 records
-//       absNew[(i+1)&1] = rz, the status, the iteration count and the error.
+//       and beta = rz / absNew[i&1]; x += alpha p (alpha re-derived from K1's
+//       partials), p = M^-1 r + beta p; block 0 records absNew[(i+1)&1] = rz,
+//       the status, the iteration count and the error. On a batch's last
+//       iteration the x update is recorded and applied by the finalize
+//       kernel, so no kernel depends on a decision another block of the
+//       same launch writes.
 // The FUSED form has two launches per iteration (p alternates between two
 // buffers); it saves the p re-read and a launch but triples the gathers of
 // the SpMV, which costs more on MI355X than it saves (94 us for KA against
@@ -64,6 +68,8 @@ struct CgState {
     int bdone[CG_MAXB];   // per-batch mode: CgStatus of each batch
     int biters[CG_MAXB];
     double berr[CG_MAXB];
+    T alpha_fin[CG_MAXB]; // alpha of a batch's last iteration ...
+    int xpend[CG_MAXB];   // ... whose x update the finalize kernel applies
     int status;
     int iters;
     int it;        // index of the running iteration (three-launch form)
@@ -412,7 +418,9 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
 // ---------------------------------------------------------------- K2
 // The first sweep's operands are loaded before the partials of K1 are
 // reduced, so the streaming loads overlap that reduction's latency.
-template <typename T>
+// UPDATE_X = false (three-launch form): x += alpha p is left to K3, which
+// reads p anyway, so this pass reads r, Ap, M^-1 and writes r only.
+template <typename T, bool UPDATE_X>
 __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int G1, const T* __restrict__ part1,
                                                           const T* __restrict__ invd, int istride,
                                                           const T* __restrict__ p, const T* __restrict__ tmp,
@@ -433,8 +441,10 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = base + j * stride;
             const bool ok = i < n;
-            xv[j] = ok ? x[off + i] : T(0);
-            pv[j] = ok ? p[off + i] : T(0);
+            if (UPDATE_X) {
+                xv[j] = ok ? x[off + i] : T(0);
+                pv[j] = ok ? p[off + i] : T(0);
+            }
             rv[j] = ok ? r[off + i] : T(0);
             tv[j] = ok ? tmp[off + i] : T(0);
             dv[j] = ok ? ib[i] : T(0);
@@ -450,9 +460,8 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = i0 + j * stride;
             if (i < n) {
-                const T xn = xv[j] + alpha * pv[j];
                 const T rn = rv[j] - alpha * tv[j];
-                x[off + i] = xn;
+                if (UPDATE_X) x[off + i] = xv[j] + alpha * pv[j];
                 r[off + i] = rn;
                 srr += rn * rn;
                 srz += rn * (dv[j] * rn);
@@ -475,7 +484,8 @@ template <typename T>
 __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, int B, int G2, const T* __restrict__ part2,
                                                            const T* __restrict__ invd, int istride,
                                                            const T* __restrict__ r, T* __restrict__ p,
-                                                           CgState<T>* st)
+                                                           CgState<T>* st, int G1, const T* __restrict__ part1,
+                                                           T* __restrict__ x)
 {
     const int bb = blockIdx.y;
     if (st->status != CG_RUNNING) return;
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
     const T* ib = invd + (long long)bb * istride;
     const long long stride = (long long)gridDim.x * CG_BS;
     long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
-    T pv[CG_VPT], rv[CG_VPT], dv[CG_VPT];
+    T pv[CG_VPT], rv[CG_VPT], dv[CG_VPT], xv[CG_VPT];
     auto load = [&](long long base) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
@@ -510,9 +520,14 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
             pv[j] = ok ? p[off + i] : T(0);
             rv[j] = ok ? r[off + i] : T(0);
             dv[j] = ok ? ib[i] : T(0);
+            xv[j] = ok ? x[off + i] : T(0);
         }
     };
     load(i0);
+    // alpha of this iteration, recomputed from K1's partials exactly as K2
+    // computed it (same partials, same order)
+    const T pap = reduce_partials(part1 + (long long)bb * G1, G1, 1, red);
+    const T alpha = absOld / pap;
     // every block computes the same totals (fixed order); the partials of a
     // batch that finished earlier are stale and are not used
     for (int k = 0; k < B; ++k) {
@@ -559,28 +574,46 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
         // all have converged, so its r.z is recorded like any other
         if (!(per_batch && own_conv)) st->absNew[(it + 1) & 1][bb] = rz_s[bb];
     }
-    if (per_batch ? own_conv : all) return;
+    // The last iteration of this batch (converged, or the iteration limit):
+    // its x update is left to cg_finalize_kernel, which applies the recorded
+    // alpha to the (then unchanged) p. Every block reaches the same decision
+    // from the same partials, so a block that returned at the top because
+    // block 0 already flagged the end in this launch skips nothing needed.
+    const bool last = (per_batch ? own_conv : all) || it + 1 >= max_iter;
+    if (last) {
+        if (blockIdx.x == 0 && threadIdx.x == 0 && !own_prev) {
+            st->alpha_fin[bb] = alpha;
+            st->xpend[bb] = 1;
+        }
+        return;
+    }
     const T beta = rz_s[bb] / absOld;
     while (i0 < n) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = i0 + j * stride;
-            if (i < n) p[off + i] = dv[j] * rv[j] + beta * pv[j];
+            if (i < n) {
+                x[off + i] = xv[j] + alpha * pv[j];
+                p[off + i] = dv[j] * rv[j] + beta * pv[j];
+            }
         }
         i0 += stride * CG_VPT;
         if (i0 < n) load(i0);
     }
 }
 
-// x = 0 for a zero right-hand side (whole solve, or per batch).
+// End of a solve, per batch: x = 0 for a zero right-hand side, or the
+// pending x += alpha p of the batch's last iteration (three-launch form).
 template <typename T>
-__global__ void cg_finalize_kernel(int n, T* x, const CgState<T>* st)
+__global__ void cg_finalize_kernel(int n, T* x, const T* p, const CgState<T>* st)
 {
     const int bb = blockIdx.y;
     const bool zero = st->status == CG_EARLY_ZERO || (st->per_batch && st->bdone[bb] == CG_EARLY_ZERO);
-    if (!zero) return;
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[(long long)bb * n + i] = T(0);
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long k = (long long)bb * n + i;
+    if (zero) x[k] = T(0);
+    else if (st->xpend[bb]) x[k] = x[k] + st->alpha_fin[bb] * p[k];
 }
 
 template <typename T>
@@ -594,6 +627,7 @@ __global__ void cg_state_init_kernel(CgState<T>* st, int max_iter, int batches, 
         st->batches = batches;
         st->per_batch = per_batch;
         st->err = 0.0;
+        for (int k = 0; k < batches && k < CG_MAXB; ++k) st->xpend[k] = 0;
     }
 }
 
@@ -612,83 +646,20 @@ __global__ void invdiag_kernel(int n, long long nnz, const int* __restrict__ JA,
 
 // ------------------------------------------------------------- host driver
 // One three-launch iteration. The iteration index lives on the device (K1
-// advances it), so consecutive iterations have identical launch arguments.
+// advances it), so consecutive iterations have identical launch arguments
+// (replaying chunks of them from a captured hipGraph was measured: 72.8
+// against 70.3 us per iteration with direct launches at 2048^2, so the
+// solver launches directly).
 template <typename VT, typename T>
 static void launch_cg_iteration(cm_ctx_t ctx, int n, int B, int G1, int G2, const int* JA, const int* IA,
                                 const VT* vals, long long vstride, const T* invd, int istride, T* p, T* t, T* x, T* r,
                                 T* part1, T* part2, CgState<T>* st, const T* diag)
 {
     cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1, st, diag);
-    cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x, r, part2,
-                                                                st);
-    cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r, p, st);
-}
-
-// Captured graphs of 'chunk' iterations, keyed by every launch argument (the
-// solver's work buffers come from the caching allocator, so repeated solves
-// of one size find their graph again).
-struct CgGraphKey {
-    cm_ctx_t ctx;
-    const void* ptrs[13];
-    long long vstride;
-    int n, B, G1, G2, istride, chunk, vt_size, t_size, vt_bf16;
-    bool operator<(const CgGraphKey& o) const { return memcmp(this, &o, sizeof(CgGraphKey)) < 0; }
-};
-static std::mutex g_cg_graph_mutex;
-static std::map<CgGraphKey, hipGraphExec_t> g_cg_graphs;
-constexpr size_t CG_GRAPH_CACHE_MAX = 64;
-
-template <typename VT, typename T>
-static int cg_chunk_graph(cm_ctx_t ctx, int chunk, int n, int B, int G1, int G2, const int* JA, const int* IA,
-                          const VT* vals, long long vstride, const T* invd, int istride, T* p, T* t, T* x, T* r,
-                          T* part1, T* part2, CgState<T>* st, const T* diag, hipGraphExec_t* out)
-{
-    CgGraphKey key;
-    memset(&key, 0, sizeof(key));
-    key.ctx = ctx;
-    const void* ps[13] = {JA, IA, vals, invd, p, t, x, r, part1, part2, st, diag, ctx->stream};
-    memcpy(key.ptrs, ps, sizeof(ps));
-    key.vstride = vstride;
-    key.n = n;
-    key.B = B;
-    key.G1 = G1;
-    key.G2 = G2;
-    key.istride = istride;
-    key.chunk = chunk;
-    key.vt_size = (int)sizeof(VT);
-    key.t_size = (int)sizeof(T);
-    key.vt_bf16 = std::is_same<VT, unsigned short>::value ? 1 : 0;
-    std::lock_guard<std::mutex> lk(g_cg_graph_mutex);
-    auto it = g_cg_graphs.find(key);
-    if (it != g_cg_graphs.end()) {
-        *out = it->second;
-        return CM_OK;
-    }
-    if (g_cg_graphs.size() >= CG_GRAPH_CACHE_MAX) {
-        for (auto& kv : g_cg_graphs) (void)hipGraphExecDestroy(kv.second);
-        g_cg_graphs.clear();
-    }
-    *out = nullptr;
-    // the legacy default stream cannot be captured: such callers keep
-    // ordinary launches
-    if (ctx->stream == nullptr) return CM_OK;
-    hipGraph_t g;
-    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        (void)hipGetLastError();
-        return CM_OK;
-    }
-    for (int k = 0; k < chunk; ++k)
-        launch_cg_iteration<VT, T>(ctx, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x, r, part1, part2,
-                                   st, diag);
-    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-    CM_HIP(e);
-    hipGraphExec_t exec = nullptr;
-    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    CM_HIP(e);
-    g_cg_graphs.emplace(key, exec);
-    *out = exec;
-    return CM_OK;
+    cg_update_kernel<T, false><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x, r,
+                                                                       part2, st);
+    cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r, p, st, G1,
+                                                                 part1, x);
 }
 
 struct CgBatchOpts {
@@ -766,28 +737,9 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     const int CHUNK = 8;
     bool pending = false;
     int it = 0;
-    // long solves replay a captured graph of CHUNK three-launch iterations
-    // (one launch call instead of 3*CHUNK)
-    hipGraphExec_t chunk_graph = nullptr;
-    const bool use_graph = !fused && max_iter >= 2 * CHUNK;
-    bool graph_tried = false;
     while (it < max_iter) {
         int todo = std::min(CHUNK, max_iter - it);
-        // captured after the first chunk has run with ordinary launches, so
-        // every kernel has been loaded before the capture
-        if (use_graph && !graph_tried && it >= CHUNK) {
-            graph_tried = true;
-            CM_TRY((cg_chunk_graph<VT, T>(ctx, CHUNK, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x,
-                                          r, part1, part2, st, diag, &chunk_graph)));
-        }
-        if (chunk_graph) {
-            // CHUNK iterations in one graph launch; iterations past max_iter
-            // are no-ops (the status is final by then)
-            CM_HIP(hipGraphLaunch(chunk_graph, ctx->stream));
-            it += CHUNK;
-            todo = CHUNK;
-        }
-        for (int k = 0; k < todo && !chunk_graph; ++k, ++it) {
+        for (int k = 0; k < todo; ++k, ++it) {
             if (!fused) {
                 launch_cg_iteration<VT, T>(ctx, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x, r, part1,
                                            part2, st, diag);
@@ -801,8 +753,8 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
                 cg_iter_kernel<VT, T, false><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(
                     n, it, B, G2, JA, IA, vals, vstride, invd, istride, r, pbuf[(it - 1) & 1], pcur, t, part1,
                     part2, st, diag);
-            cg_update_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, istride, pcur, t, x,
-                                                                        r, part2, st);
+            cg_update_kernel<T, true><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, istride, pcur,
+                                                                              t, x, r, part2, st);
         }
         CM_LAUNCH_CHECK();
         launches += (fused ? 2 : 3) * todo;
@@ -825,7 +777,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         CM_LAUNCH_CHECK();
         launches += 1;
     }
-    cg_finalize_kernel<T><<<dim3(div_up(n, 256), B), 256, 0, ctx->stream>>>(n, x, st);
+    cg_finalize_kernel<T><<<dim3(div_up(n, 256), B), 256, 0, ctx->stream>>>(n, x, p, st);
     CM_LAUNCH_CHECK();
     launches += 1;
     CgState<T> hs;
