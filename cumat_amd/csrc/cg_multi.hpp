@@ -4,15 +4,16 @@
 // Per iteration i, three launches (the default form):
 //   K1  tmp = A p, per-block partial p.tmp
 //   K2  every block reduces K1's partials of its batch in the same fixed
-//       order -> the same alpha = absNew[i&1] / p.tmp; r -= alpha tmp;
-//       per-block partials of |r|^2 and r.(M^-1 r)
+//       order -> the same alpha = absNew[i&1] / p.tmp; r -= alpha tmp
+//       (and x += alpha p for small systems); per-block partials of |r|^2
+//       and r.(M^-1 r)
 //   K3  every block reduces K2's partials of all batches -> convergence test
-//       and beta = rz / absNew[i&1]; x += alpha p (alpha re-derived from K1's
-//       partials), p = M^-1 r + beta p; block 0 records absNew[(i+1)&1] = rz,
-//       the status, the iteration count and the error. On a batch's last
-//       iteration the x update is recorded and applied by the finalize
-//       kernel, so no kernel depends on a decision another block of the
-//       same launch writes.
+//       and beta = rz / absNew[i&1]; p = M^-1 r + beta p; for large systems
+//       also x += alpha p (alpha re-derived from K1's partials), which saves
+//       K2 the x and p passes; block 0 records absNew[(i+1)&1] = rz, the
+//       status, the iteration count and the error. In that form a batch's
+//       last x update is recorded and applied by the finalize kernel, so no
+//       kernel depends on a decision another block of the same launch writes.
 // The FUSED form has two launches per iteration (p alternates between two
 // buffers); it saves the p re-read and a launch but triples the gathers of
 // the SpMV, which costs more on MI355X than it saves (94 us for KA against
@@ -57,6 +58,9 @@ constexpr int CG_CAP = 4096;   // staged non-zeros per setup row block
 constexpr int CG_EPT = 8;      // staged non-zeros per lane in K1
 constexpr int CG_VPT = 4;      // vector elements per thread per sweep
 constexpr int CG_GMAX = 2048;  // max blocks (= max partials) per kernel
+// from this many unknowns (n x batches) on, x is updated in K3 instead of K2
+// (measured on MI355X: +6 % at 1024^2, -14 % at 512^2 where latency rules)
+constexpr long long CG_XK3_MIN = 1 << 20;
 
 enum CgStatus : int { CG_RUNNING = 0, CG_CONVERGED = 1, CG_EARLY_ZERO = 2, CG_EARLY_GUESS = 3, CG_MAXIT = 4 };
 
@@ -129,6 +133,62 @@ __device__ __forceinline__ void reduce_partials_pair(const T* __restrict__ part,
             b += vb[j];
         }
     block_sum2<CG_BS>(a, b, lds);
+}
+
+// One right-hand side: the p.Ap partials (stride 1) and the (|r|^2, r.z)
+// pairs loaded together and summed with shared barriers; lds needs
+// 3*CG_BS/64 entries. Same order and results as reduce_partials followed by
+// reduce_partials_pair, with one memory latency and one barrier round less.
+template <typename T>
+__device__ __forceinline__ void reduce_partials_three(const T* __restrict__ part1, int G1,
+                                                      const T* __restrict__ part2, int G2, T& pap, T& a, T& b,
+                                                      T* lds)
+{
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    T v1[CG_PPT], va[CG_PPT], vb[CG_PPT];
+#pragma unroll
+    for (int j = 0; j < CG_PPT; ++j) {
+        const int i = threadIdx.x + j * CG_BS;
+        v1[j] = (i < G1) ? part1[i] : T(0);
+        T2 v = {T(0), T(0)};
+        if (i < G2) v = *reinterpret_cast<const T2*>(part2 + 2ll * i);
+        va[j] = v.x;
+        vb[j] = v.y;
+    }
+    pap = T(0);
+    a = T(0);
+    b = T(0);
+#pragma unroll
+    for (int j = 0; j < CG_PPT; ++j) {
+        const int i = threadIdx.x + j * CG_BS;
+        if (i < G1) pap += v1[j];
+        if (i < G2) {
+            a += va[j];
+            b += vb[j];
+        }
+    }
+    constexpr int NW = CG_BS / 64;
+    pap = wave_reduce<OpSum>(pap);
+    a = wave_reduce<OpSum>(a);
+    b = wave_reduce<OpSum>(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        lds[w] = pap;
+        lds[NW + w] = a;
+        lds[2 * NW + w] = b;
+    }
+    __syncthreads();
+    T r0 = lds[0], r1 = lds[NW], r2 = lds[2 * NW];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) {
+        r0 += lds[i];
+        r1 += lds[NW + i];
+        r2 += lds[2 * NW + i];
+    }
+    __syncthreads();
+    pap = r0;
+    a = r1;
+    b = r2;
 }
 
 template <typename T>
@@ -480,7 +540,10 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int
 
 // ---------------------------------------------------------------- K3 (three-launch form)
 // convergence test of iteration it and p = M^-1 r + beta p
-template <typename T>
+// XK3: K3 also performs x += alpha p (K2 then skips x and p: one vector pass
+// less per iteration; pays off for large systems, where the pass costs more
+// than the extra partial reduction K3 needs for alpha).
+template <typename T, bool XK3>
 __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, int B, int G2, const T* __restrict__ part2,
                                                            const T* __restrict__ invd, int istride,
                                                            const T* __restrict__ r, T* __restrict__ p,
@@ -494,7 +557,7 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
     // a batch that finished in an earlier iteration has nothing to update;
     // block (0,0) still runs, it owns the global status
     if (per_batch && st->bdone[bb] != CG_RUNNING && !(blockIdx.x == 0 && bb == 0)) return;
-    __shared__ T red[2 * CG_BS / 64];
+    __shared__ T red[3 * CG_BS / 64];
     __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB], thr_s[CG_MAXB], rhs_s[CG_MAXB];
     __shared__ int done_s[CG_MAXB];
     // the state the decisions below read, fetched before the reductions so
@@ -520,22 +583,29 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
             pv[j] = ok ? p[off + i] : T(0);
             rv[j] = ok ? r[off + i] : T(0);
             dv[j] = ok ? ib[i] : T(0);
-            xv[j] = ok ? x[off + i] : T(0);
+            if (XK3) xv[j] = ok ? x[off + i] : T(0);
         }
     };
     load(i0);
     // alpha of this iteration, recomputed from K1's partials exactly as K2
-    // computed it (same partials, same order)
-    const T pap = reduce_partials(part1 + (long long)bb * G1, G1, 1, red);
-    const T alpha = absOld / pap;
-    // every block computes the same totals (fixed order); the partials of a
-    // batch that finished earlier are stale and are not used
-    for (int k = 0; k < B; ++k) {
+    // computed it (same partials, same order); every block computes the same
+    // totals (fixed order); the partials of a batch that finished earlier are
+    // stale and are not used
+    T pap = T(1);
+    if (XK3 && B == 1) {
         T a, c;
-        reduce_partials_pair(part2 + 2ll * k * G2, G2, a, c, red);
-        if (threadIdx.x == 0) { rr_s[k] = a; rz_s[k] = c; }
+        reduce_partials_three(part1, G1, part2, G2, pap, a, c, red);
+        if (threadIdx.x == 0) { rr_s[0] = a; rz_s[0] = c; }
+    } else {
+        if (XK3) pap = reduce_partials(part1 + (long long)bb * G1, G1, 1, red);
+        for (int k = 0; k < B; ++k) {
+            T a, c;
+            reduce_partials_pair(part2 + 2ll * k * G2, G2, a, c, red);
+            if (threadIdx.x == 0) { rr_s[k] = a; rz_s[k] = c; }
+        }
     }
     __syncthreads();
+    const T alpha = absOld / pap;
     bool all = true;
     T m = 0;
     bool have_m = false;
@@ -581,7 +651,7 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
     // block 0 already flagged the end in this launch skips nothing needed.
     const bool last = (per_batch ? own_conv : all) || it + 1 >= max_iter;
     if (last) {
-        if (blockIdx.x == 0 && threadIdx.x == 0 && !own_prev) {
+        if (XK3 && blockIdx.x == 0 && threadIdx.x == 0 && !own_prev) {
             st->alpha_fin[bb] = alpha;
             st->xpend[bb] = 1;
         }
@@ -593,7 +663,7 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = i0 + j * stride;
             if (i < n) {
-                x[off + i] = xv[j] + alpha * pv[j];
+                if (XK3) x[off + i] = xv[j] + alpha * pv[j];
                 p[off + i] = dv[j] * rv[j] + beta * pv[j];
             }
         }
@@ -656,10 +726,17 @@ static void launch_cg_iteration(cm_ctx_t ctx, int n, int B, int G1, int G2, cons
                                 T* part1, T* part2, CgState<T>* st, const T* diag)
 {
     cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1, st, diag);
-    cg_update_kernel<T, false><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x, r,
-                                                                       part2, st);
-    cg_pupdate_kernel<T><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r, p, st, G1,
-                                                                 part1, x);
+    if ((long long)n * B >= CG_XK3_MIN) {
+        cg_update_kernel<T, false><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x,
+                                                                           r, part2, st);
+        cg_pupdate_kernel<T, true><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r, p,
+                                                                           st, G1, part1, x);
+    } else {
+        cg_update_kernel<T, true><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x,
+                                                                          r, part2, st);
+        cg_pupdate_kernel<T, false><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r,
+                                                                            p, st, G1, part1, x);
+    }
 }
 
 struct CgBatchOpts {

@@ -164,3 +164,46 @@ def test_edge_assemble_and_grad(gpu, orc):
     g = dg.numpy().reshape(B, -1)
     for k in range(B):
         np.testing.assert_array_equal(g[k], orc.edge_grad(P["ei"], P["ej"], lam[k], x[k]))
+
+
+@pytest.mark.parametrize("per_batch", [1, 0])
+def test_large_batched_solve_x_update_in_p_kernel(gpu, orc, per_batch):
+    """From 2^20 unknowns on, x += alpha p moves into the p-update launch and a
+    batch's last update is applied at the end: per-batch and all-batch
+    stopping both match the oracle per system (3 x 600^2 = 1.08 M unknowns)."""
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    g, B = 600, 3
+    n = g * g
+    JA, IA, _ = poisson2d(g, 0.0, np.float64)
+    shifts = [0.02, 0.2, 1.0]
+    vals = np.concatenate([poisson2d(g, s, np.float64)[2] for s in shifts])
+    b = np.random.default_rng(31).uniform(-1, 1, n * B)
+    dJA, dIA, dV = cm.DeviceArray.from_numpy(JA), cm.DeviceArray.from_numpy(IA), cm.DeviceArray.from_numpy(vals)
+    db = cm.DeviceArray.from_numpy(b)
+    dx = cm.DeviceArray(n * B, dtype=np.float64)
+    its = (C.c_int * B)()
+    errs = (C.c_double * B)()
+    st = cm.capi.CgStats()
+    tol = 1e-8
+    cm.capi.call("cm_pcg_solve_ex", None, cm.capi.F64, n, IA.size, B, dJA.ptr, dIA.ptr, dV.ptr, 1, None, db.ptr,
+                 dx.ptr, 0, tol, 4000, per_batch, its, errs, C.byref(st))
+    x = dx.numpy()
+    ref = [orc.pcg(JA, IA, vals[k * IA.size:(k + 1) * IA.size], b[k * n:(k + 1) * n], n, tol=tol, max_iter=4000)
+           for k in range(B)]
+    # the reduction order differs from the oracle's sequential sums; near the
+    # threshold that can move a 360 K-row FP64 solve by an iteration or two
+    if per_batch:
+        for k in range(B):
+            xk, itk, _ = ref[k]
+            assert abs(its[k] - itk) <= 3, (k, its[k], itk)
+            assert errs[k] <= tol
+            assert np.linalg.norm(x[k * n:(k + 1) * n] - xk) / np.linalg.norm(xk) <= 1e-6
+        assert its[2] < its[0]
+    else:
+        # all batches iterate until the slowest converges: same count for all
+        assert len(set(its)) == 1
+        assert abs(its[0] - max(r[1] for r in ref)) <= 3
+        for k in range(B):
+            r = orc.csr_spmv(JA, IA, vals[k * IA.size:(k + 1) * IA.size], x[k * n:(k + 1) * n], n, n)
+            assert np.linalg.norm(b[k * n:(k + 1) * n] - r) / np.linalg.norm(b[k * n:(k + 1) * n]) <= 2 * tol
