@@ -291,7 +291,8 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
     out["cg_small_iters_per_s"] = sweep
     log("cg_small_iters_per_s", sweep)
     for name, fn in (("batched_cg_4096", bench_batched), ("mixed_ir_160cubed", bench_mixed),
-                     ("adjoint_64x512sq", bench_adjoint)):
+                     ("adjoint_64x512sq", bench_adjoint),
+                     ("reference_published_workloads", bench_reference_workloads)):
         try:
             out[name] = fn(cm, capi, ctx)
             log(name, out[name])
@@ -387,6 +388,90 @@ def bench_mixed(cm, capi, ctx):
                                 "rel_residual": true_rel(xd),
                                 "err_vs_fp64_cg": float(np.linalg.norm(xh - xref) / np.linalg.norm(xref)),
                                 "speedup_vs_fp64_cg": round(t64 / st["seconds"], 3)}
+    return out
+
+
+def bench_reference_workloads(cm, capi, ctx):
+    """The reference's own published benchmark workloads (NVIDIA RTX 2070,
+    numbers quoted from its benchmark result files), run here unchanged in
+    size and method so the two can be set side by side (different hardware:
+    informational, not a target)."""
+    import ctypes as C
+    from cumat_amd.problems import poisson2d
+    out = {}
+    # CG: 2D Poisson, rhs = rand()/RAND_MAX on the boundary nodes after
+    # srand(42) in the reference's loop order, tol 1e-4, default max iterations
+    # (benchmarks/conjugate_gradient/Implementation_cuMat.cu)
+    libc = C.CDLL("libc.so.6")
+    published_cg = {500: 201.583, 1000: 599.841}  # ms, Conjugate Gradient - 2D-Poisson Matrix.json:19,22
+    for g in (500, 1000):
+        JA, IA, V = poisson2d(g)
+        n = g * g
+        rhs = np.zeros(n, np.float32)
+        libc.srand(42)
+        for x in range(g):
+            for y in range(g):
+                if x == 0 or y == 0 or x == g - 1 or y == g - 1:
+                    rhs[y + x * g] = libc.rand() / float(2147483647)
+        A = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
+        bd = cm.DeviceArray.from_numpy(rhs, ctx=ctx)
+        xd = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
+        cm.ConjugateGradient(A).setTolerance(1e-4).solve(bd, xd, ctx=ctx)
+        ctx.sync()
+        ts = []
+        for _ in range(2):  # timed as the reference does: set-up (diagonal) + solve, 2 runs
+            t0 = time.perf_counter()
+            cg = cm.ConjugateGradient(A).setTolerance(1e-4)
+            cg.solve(bd, xd, ctx=ctx)
+            ctx.sync()
+            ts.append(time.perf_counter() - t0)
+        ms = 1000 * sum(ts) / len(ts)
+        out[f"cg_poisson{g}_tol1e-4"] = {"ms": round(ms, 3), "iterations": cg.iterations(), "error": cg.error(),
+                                          "reference_rtx2070_ms": published_cg[g],
+                                          "ratio": round(published_cg[g] / ms, 1)}
+    # CSR SpMV, grid 1000 (CSRMV - 2D-Poisson Matrix.json:24): 0.163467 ms
+    JA, IA, V = poisson2d(1000)
+    n = 1000 * 1000
+    A = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
+    xd = cm.DeviceArray.from_numpy(np.random.default_rng(1).uniform(-1, 1, n).astype(np.float32), ctx=ctx)
+    yd = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
+    A.matvec(xd, yd, ctx=ctx)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        A.matvec(xd, yd, ctx=ctx)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 10
+    out["spmv_poisson1000"] = {"ms": round(ms, 4), "reference_rtx2070_ms": 0.163467, "ratio": round(0.163467 / ms, 1)}
+    # dot, n = 1e8 FP32, result copied to the host each call (Dotproduct.json:77)
+    n = 100_000_000
+    a = cm.DeviceArray.from_numpy(np.ones(n, np.float32), ctx=ctx)
+    b = cm.DeviceArray.from_numpy(np.full(n, 0.5, np.float32), ctx=ctx)
+    d = cm.DeviceArray(1, dtype=np.float32, ctx=ctx)
+    cm.dot(a, b, out=d, ctx=ctx)
+    d.numpy()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        cm.dot(a, b, out=d, ctx=ctx)
+        v = d.numpy()
+    ms = (time.perf_counter() - t0) * 1000 / 20
+    out["dot_1e8_with_host_copy"] = {"ms": round(ms, 4), "value": float(v[0]), "reference_rtx2070_ms": 2.37392,
+                                     "ratio": round(2.37392 / ms, 1)}
+    del a, b
+    # linear combination k = 2, n = 5e7 (Linear Combination - Constant Count.json:45)
+    n = 50_000_000
+    v0 = cm.DeviceArray.from_numpy(np.ones(n, np.float32), ctx=ctx)
+    v1 = cm.DeviceArray.from_numpy(np.full(n, 2.0, np.float32), ctx=ctx)
+    o = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
+    e = v0.expr() * 0.5 + v1.expr() * 0.25
+    cm.assign(o, e, ctx=ctx)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(50):
+        cm.assign(o, e, ctx=ctx)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1000 / 50
+    out["lincomb_k2_5e7"] = {"ms": round(ms, 4), "reference_rtx2070_ms": 2.00751, "ratio": round(2.00751 / ms, 1)}
     return out
 
 
