@@ -21,12 +21,13 @@ from .expr import (Expr, as_expr, assign, cast, select, where_scalar,  # noqa: F
 from .sparse import SparseMatrix, SparsityPattern, ELLMatrix  # noqa: F401
 from .reductions import reduce, dot, squared_norm, norm  # noqa: F401
 from .solvers import ConjugateGradient, DiagonalPreconditioner, pcg_batched, pcg_ir  # noqa: F401
+from . import dense  # noqa: F401
 
 __all__ = [
     "capi", "DeviceArray", "Context", "default_context", "synchronize", "Expr", "as_expr",
     "assign", "cast", "select", "lower", "evaluate", "SparseMatrix", "SparsityPattern",
     "ELLMatrix", "reduce", "dot", "squared_norm", "norm", "ConjugateGradient",
-    "DiagonalPreconditioner", "pcg_batched", "pcg_ir", "counters", "reset_counters",
+    "DiagonalPreconditioner", "pcg_batched", "pcg_ir", "dense", "counters", "reset_counters",
     "device_info",
 ]
 
@@ -34,7 +35,7 @@ __all__ = [
 def counters() -> dict:
     """Profiling counters (cuMat::Profiling, cuMat/src/Profiling.h:23-109)."""
     names = ["DeviceMemAlloc", "DeviceMemFree", "EvalAny", "EvalCwise", "EvalCwiseSparse",
-             "EvalReduction", "EvalMatmulSparse", "KernelLaunch"]
+             "EvalReduction", "EvalMatmulSparse", "KernelLaunch", "EvalMatmul"]
     out = {}
     for i, n in enumerate(names):
         v = C.c_longlong(0)

@@ -98,6 +98,10 @@ _PROTOS = {
     "cm_csr_cwise_eval": ([_VP, C.POINTER(Leaf), _I32, C.POINTER(Instr), _I32, _I32, _I32, _I32,
                            _I32, _VP, _VP, _VP, _I32, _I32], _I32),
     "cm_csr_inverse_diagonal": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP], _I32),
+    "cm_dense_gemv": ([_VP, _I32, _I32, _I32, _I32, _VP, _I32, _I32, _VP, _I32, _VP, _I32], _I32),
+    "cm_dense_inverse_diagonal": ([_VP, _I32, _I32, _I32, _VP, _I32, _VP], _I32),
+    "cm_pcg_dense_solve": ([_VP, _I32, _I32, _I32, _VP, _I32, _I32, _VP, _VP, _VP, _I32, _F64, _I32,
+                            C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(CgStats)], _I32),
     "cm_pcg_solve": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _VP, _VP, _VP, _I32, _F64, _I32,
                       _I32, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(CgStats)], _I32),
     "cm_pcg_solve_ex": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP, _VP, _VP, _I32, _F64,

@@ -695,6 +695,55 @@ int cm_pcg_solve(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int
     CM_API_END
 }
 
+int cm_pcg_dense_solve(cm_ctx_t ctx, int dtype, int n, int batches, const void* A, int a_batched, int a_row_major,
+                       const void* invdiag, const void* b, void* x, int use_guess, double tol, int max_iter,
+                       int* iterations_out, double* error_out, cm_cg_stats* stats)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(n >= 0 && batches >= 1 && batches <= cm::CG_MAXB, CM_E_DIM,
+               "cm_pcg_dense_solve: bad sizes (batches 1..%d)", cm::CG_MAXB);
+    CM_REQUIRE(tol >= 0, CM_E_ARG, "cm_pcg_dense_solve: negative tolerance");
+    if (max_iter < 0) max_iter = 2 * n;  // IterativeSolverBase::maxIterations (cuMat/src/IterativeSolverBase.h:85)
+    if (n == 0) {
+        if (iterations_out) *iterations_out = 0;
+        if (error_out) *error_out = 0;
+        return CM_OK;
+    }
+    CM_REQUIRE(A && b && x, CM_E_ARG, "cm_pcg_dense_solve: NULL pointer");
+    CM_REQUIRE(dtype == CM_F32 || dtype == CM_F64, CM_E_UNSUPPORTED, "cm_pcg_dense_solve: dtype %d", dtype);
+    cm::Temps tmp(ctx);
+    const int nb = a_batched ? batches : 1;
+    if (!invdiag) {
+        void* id;
+        CM_TRY(tmp.get((dtype == CM_F32 ? 4 : 8) * (size_t)n * nb, &id));
+        CM_TRY(cm_dense_inverse_diagonal(ctx, dtype, n, batches, A, a_batched, id));
+        invdiag = id;
+    }
+    cm::CgBatchOpts opt;
+    opt.vals_batched = a_batched;  // the inverse diagonal then has one row of n per batch
+    const long long bstride = a_batched ? (long long)n * n : 0;
+    if (dtype == CM_F32) {
+        cm::DenseOp<float> op;
+        op.A = (const float*)A;
+        op.bstride = bstride;
+        op.row_major = a_row_major != 0;
+        opt.dense = &op;
+        return cm::pcg_multikernel<float, float>(ctx, n, 0, batches, nullptr, nullptr, nullptr,
+                                                 (const float*)invdiag, (const float*)b, (float*)x, use_guess, tol,
+                                                 max_iter, iterations_out, error_out, stats, nullptr, opt);
+    }
+    cm::DenseOp<double> op;
+    op.A = (const double*)A;
+    op.bstride = bstride;
+    op.row_major = a_row_major != 0;
+    opt.dense = &op;
+    return cm::pcg_multikernel<double, double>(ctx, n, 0, batches, nullptr, nullptr, nullptr, (const double*)invdiag,
+                                               (const double*)b, (double*)x, use_guess, tol, max_iter,
+                                               iterations_out, error_out, stats, nullptr, opt);
+    CM_API_END
+}
+
 int cm_pcg_solve_ex(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int* JA, const int* IA,
                     const void* vals, int vals_batched, const void* invdiag, const void* b, void* x, int use_guess,
                     double tol, int max_iter, int per_batch, int* iterations_each, double* error_each,

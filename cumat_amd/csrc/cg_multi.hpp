@@ -46,6 +46,7 @@
 #include <map>
 #include <type_traits>
 
+#include "dense.hpp"
 #include "spmv_core.hpp"
 #include "spmv_wave.hpp"
 
@@ -204,8 +205,10 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, const int* __res
                                                          const T* __restrict__ invd, int istride,
                                                          const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r,
                                                          T* __restrict__ p, int use_guess, T* __restrict__ part,
-                                                         const T* __restrict__ diag)
+                                                         const T* __restrict__ diag, const T* __restrict__ ax_pre)
 {
+    // ax_pre: A x of the guess computed beforehand (dense operators); JA, IA
+    // and vals are then unused
     __shared__ SpmvSmem<T, CG_RPB, CG_CAP> sm;
     __shared__ T red[2 * CG_BS / 64];
     const int G = gridDim.x;
@@ -219,7 +222,13 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, const int* __res
         const int r0 = w.cur * CG_RPB;
         const int nr = min(CG_RPB, n - r0);
         T ax[CG_RPB / CG_BS];
-        if (use_guess) {
+        if (use_guess && ax_pre) {
+#pragma unroll
+            for (int j = 0; j < CG_RPB / CG_BS; ++j) {
+                const int i = threadIdx.x + j * CG_BS;
+                ax[j] = (i < nr) ? ax_pre[off + r0 + i] : T(0);
+            }
+        } else if (use_guess) {
             block_csr_spmv<VT, T, CG_BS, CG_RPB, CG_CAP>(JA, IA, vb, x + off, r0, nr, sm, ax, diag);
             __syncthreads();  // sm is reused by the next row block
         } else {
@@ -340,6 +349,37 @@ __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __rest
         }
     }
     s = block_reduce<OpSum, CG_BS>(s, red);
+    if (threadIdx.x == 0) part[(long long)bb * gridDim.x + blockIdx.x] = s;
+}
+
+// K1 for a dense operator: tmp = A p, per-block partial p.tmp (same grid,
+// partial layout and iteration counter as cg_spmv_kernel)
+template <typename T, bool ROW_MAJOR>
+__global__ __launch_bounds__(CG_BS) void cg_dense_kernel(int n, const T* __restrict__ A, long long a_bstride,
+                                                         const T* __restrict__ p, T* __restrict__ tmp,
+                                                         T* __restrict__ part, CgState<T>* st)
+{
+    static_assert(CG_BS == DN_BS, "dense tiles use the CG block size");
+    const int bb = blockIdx.y;
+    if (blockIdx.x == 0 && bb == 0 && threadIdx.x == 0 && st->status == CG_RUNNING) st->it = st->it + 1;
+    if (batch_inactive(st, bb)) return;
+    __shared__ T red[4 * 64];
+    __shared__ T red2[CG_BS / 64];
+    const long long off = (long long)bb * n;
+    const T* Ab = A + bb * a_bstride;
+    const T* pb = p + off;
+    T s = T(0);
+    const int ntiles = div_up_dev(n, DN_TILE);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r0 = t * DN_TILE;
+        const T v = dense_tile<T, ROW_MAJOR>(n, n, Ab, pb, r0, red);
+        const int r = r0 + (int)threadIdx.x;
+        if (threadIdx.x < 64 && r < n) {
+            tmp[off + r] = v;
+            s += pb[r] * v;
+        }
+    }
+    s = block_reduce<OpSum, CG_BS>(s, red2);
     if (threadIdx.x == 0) part[(long long)bb * gridDim.x + blockIdx.x] = s;
 }
 
@@ -720,12 +760,27 @@ __global__ void invdiag_kernel(int n, long long nnz, const int* __restrict__ JA,
 // (replaying chunks of them from a captured hipGraph was measured: 72.8
 // against 70.3 us per iteration with direct launches at 2048^2, so the
 // solver launches directly).
+// A dense operator in place of the CSR matrix (cm_pcg_dense_solve): n x n,
+// batch stride bstride (0: one matrix for all right-hand sides)
+template <typename T>
+struct DenseOp {
+    const T* A = nullptr;
+    long long bstride = 0;
+    bool row_major = false;
+};
+
 template <typename VT, typename T>
 static void launch_cg_iteration(cm_ctx_t ctx, int n, int B, int G1, int G2, const int* JA, const int* IA,
                                 const VT* vals, long long vstride, const T* invd, int istride, T* p, T* t, T* x, T* r,
-                                T* part1, T* part2, CgState<T>* st, const T* diag)
+                                T* part1, T* part2, CgState<T>* st, const T* diag, const DenseOp<T>* dense = nullptr)
 {
-    cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1, st, diag);
+    if (dense && dense->row_major)
+        cg_dense_kernel<T, true><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, dense->A, dense->bstride, p, t, part1, st);
+    else if (dense)
+        cg_dense_kernel<T, false><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, dense->A, dense->bstride, p, t, part1,
+                                                                          st);
+    else
+        cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1, st, diag);
     if ((long long)n * B >= CG_XK3_MIN) {
         cg_update_kernel<T, false><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x,
                                                                            r, part2, st);
@@ -747,6 +802,7 @@ struct CgBatchOpts {
     // two launches per iteration (p update fused into the SpMV, gathered p
     // recomputed from r, M^-1, p_prev) instead of three
     bool fused = false;
+    const void* dense = nullptr;  // DenseOp<T>: a dense operator replaces the CSR matrix
 };
 
 template <typename VT, typename T>
@@ -757,6 +813,9 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
 {
     constexpr bool same_type = std::is_same<VT, T>::value;
     CM_REQUIRE(invd || same_type, CM_E_ARG, "CG: mixed storage needs an explicit inverse diagonal");
+    const DenseOp<T>* dense = (const DenseOp<T>*)opt.dense;
+    CM_REQUIRE(!dense || (invd && !opt.fused), CM_E_ARG,
+               "CG: a dense operator needs an inverse diagonal and the three-launch form");
     CM_REQUIRE(B >= 1 && B <= CG_MAXB, CM_E_UNSUPPORTED, "CG: %d right-hand sides (max %d)", B, CG_MAXB);
     Temps tmp(ctx);
     const long long N = (long long)n * B;
@@ -800,8 +859,21 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     auto t0 = std::chrono::steady_clock::now();
     cg_state_init_kernel<T><<<1, 64, 0, ctx->stream>>>(st, max_iter, B, opt.per_batch);
     CM_LAUNCH_CHECK();
+    const T* ax_pre = nullptr;
+    if (dense && use_guess) {
+        // t = A x_0, read by the setup kernel
+        if (dense->row_major)
+            dense_gemv_kernel<T, true><<<dim3(dense_grid(n), B), DN_BS, 0, ctx->stream>>>(
+                n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN);
+        else
+            dense_gemv_kernel<T, false><<<dim3(dense_grid(n), B), DN_BS, 0, ctx->stream>>>(
+                n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN);
+        CM_LAUNCH_CHECK();
+        ax_pre = t;
+        launches += 1;
+    }
     cg_setup_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, invd, istride, b, x, r,
-                                                                   p, use_guess, part1, diag);
+                                                                   p, use_guess, part1, diag, ax_pre);
     CM_LAUNCH_CHECK();
     cg_setup_finalize_kernel<T><<<1, CG_BS, 0, ctx->stream>>>(G1, B, (T)tol, part1, st);
     CM_LAUNCH_CHECK();
@@ -819,7 +891,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         for (int k = 0; k < todo; ++k, ++it) {
             if (!fused) {
                 launch_cg_iteration<VT, T>(ctx, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x, r, part1,
-                                           part2, st, diag);
+                                           part2, st, diag, dense);
                 continue;
             }
             T* pcur = pbuf[it & 1];

@@ -1,0 +1,112 @@
+// dense.hpp -- dense matrix-vector product y = A x and the Jacobi diagonal of
+// a dense matrix.
+//
+// Replaces the matrix-vector case of cuMat's dense ProductOp, which cuMat
+// evaluates with cuBLAS gemm (cuMat/src/ProductOp.h:318-420), and the
+// ExtractDiagonalOp + cwiseInverseCheck preconditioner setup on a dense
+// matrix (cuMat/src/ConjugateGradient.h:218-246, cuMat/src/UnaryOps.h:177).
+// A matrix-vector product reads every matrix element once and does one
+// multiply-add per element: it is HBM-bound, so it is a streaming kernel, not
+// a GEMM on the matrix cores.
+//
+// Work unit: a tile of 64 consecutive rows per 256-thread block.
+//   column-major A (cuMat's default): lane l owns row r0 + l, so each column
+//     of the tile is one 256-byte contiguous load per wavefront; wavefront w
+//     sums the columns w, w + 4, ... with 8 loads in flight, and the four
+//     partial sums are added in wavefront order through LDS;
+//   row-major A: each wavefront takes 16 rows of the tile, lanes stride the
+//     row with contiguous loads, fixed-order wavefront reduction.
+// The summation order depends only on the sizes, so results are
+// deterministic from run to run.
+#pragma once
+#include <algorithm>
+
+#include "cm_common.hpp"
+
+namespace cm {
+
+constexpr int DN_BS = 256;    // threads per block: 4 wavefronts
+constexpr int DN_TILE = 64;   // rows per tile
+constexpr int DN_UNROLL = 8;  // column loads in flight per wavefront
+
+// Value of row r0 + lane of (A x) on the lanes of wavefront 0 (0 elsewhere
+// and beyond the last row). red: 4*64 elements of LDS. Called by all threads
+// of the block.
+template <typename T, bool ROW_MAJOR>
+__device__ __forceinline__ T dense_tile(int rows, int cols, const T* __restrict__ A, const T* __restrict__ x,
+                                        int r0, T* red)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    if constexpr (!ROW_MAJOR) {
+        const int r = r0 + lane;
+        const bool ok = r < rows;
+        const T* a = A + (ok ? r : 0);
+        const long long lda = rows;
+        T acc = T(0);
+        int j = wave;
+        for (; j + 4 * (DN_UNROLL - 1) < cols; j += 4 * DN_UNROLL) {
+            T v[DN_UNROLL];
+#pragma unroll
+            for (int u = 0; u < DN_UNROLL; ++u) v[u] = a[(long long)(j + 4 * u) * lda];
+#pragma unroll
+            for (int u = 0; u < DN_UNROLL; ++u) acc += v[u] * x[j + 4 * u];
+        }
+        for (; j < cols; j += 4) acc += a[(long long)j * lda] * x[j];
+        red[wave * 64 + lane] = ok ? acc : T(0);
+    } else {
+        for (int i = wave; i < DN_TILE; i += 4) {
+            const int r = r0 + i;
+            T s = T(0);
+            if (r < rows) {  // uniform over the wavefront
+                const T* a = A + (long long)r * cols;
+                for (int c = lane; c < cols; c += 64) s += a[c] * x[c];
+            }
+            s = wave_reduce<OpSum>(s);
+            if (lane == 0) red[i] = s;
+        }
+    }
+    __syncthreads();
+    T y = T(0);
+    if (wave == 0) {
+        if constexpr (!ROW_MAJOR) y = ((red[lane] + red[64 + lane]) + red[128 + lane]) + red[192 + lane];
+        else y = red[lane];
+    }
+    __syncthreads();  // red is reused by the next tile
+    return y;
+}
+
+// y_b = A_b x_b (assignment mode as cm_assign_mode); A_b = A + b*a_bstride
+template <typename T, bool ROW_MAJOR>
+__global__ __launch_bounds__(DN_BS) void dense_gemv_kernel(int rows, int cols, const T* __restrict__ A,
+                                                           long long a_bstride, const T* __restrict__ x,
+                                                           int x_batched, T* __restrict__ y, int mode)
+{
+    __shared__ T red[4 * 64];
+    const int bb = blockIdx.y;
+    const T* Ab = A + bb * a_bstride;
+    const T* xb = x + (x_batched ? (long long)bb * cols : 0);
+    T* yb = y + (long long)bb * rows;
+    const int ntiles = div_up_dev(rows, DN_TILE);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r0 = t * DN_TILE;
+        const T v = dense_tile<T, ROW_MAJOR>(rows, cols, Ab, xb, r0, red);
+        const int r = r0 + (int)threadIdx.x;
+        if (threadIdx.x < 64 && r < rows) yb[r] = (mode == CM_ASSIGN) ? v : assign_mode<T>(mode, yb[r], v);
+    }
+}
+
+// inv_b[i] = A_b(i,i) == 0 ? 1 : 1 / A_b(i,i)
+template <typename T>
+__global__ void dense_inverse_diagonal_kernel(int n, const T* __restrict__ A, long long a_bstride,
+                                              T* __restrict__ inv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T d = A[blockIdx.y * a_bstride + (long long)i * n + i];
+    inv[(long long)blockIdx.y * n + i] = (d == T(0)) ? T(1) : T(1) / d;
+}
+
+inline int dense_grid(int rows) { return (int)std::min<long long>(div_up(rows, DN_TILE), 4096); }
+
+}  // namespace cm

@@ -1,7 +1,8 @@
 #ifndef CUMAT_AMD_CONJUGATEGRADIENT_H
 #define CUMAT_AMD_CONJUGATEGRADIENT_H
 
-// Jacobi-preconditioned Conjugate Gradient on CSR matrices, with the API of
+// Jacobi-preconditioned Conjugate Gradient on CSR or dense matrices
+// (ConjugateGradient<SMatrixXf> / <MatrixXf>), with the API of
 // cuMat's ConjugateGradient / IterativeSolverBase / SolverBase
 // (cuMat/src/ConjugateGradient.h:66-265, cuMat/src/IterativeSolverBase.h:11-227,
 // cuMat/src/SolverBase.h:34-126): default tolerance epsilon, default
@@ -35,11 +36,17 @@ public:
     DiagonalPreconditioner() = default;
     explicit DiagonalPreconditioner(const _MatrixType& A) : entries_(A.rows())
     {
-        static_assert((int)_MatrixType::SFlags == (int)SparseFlags::CSR, "the CG solver takes CSR matrices");
-        const auto& P = A.getSparsityPattern();
-        CUMAT_AMD_CALL(cm_csr_inverse_diagonal(Context::current().handle(), internal::DType<Scalar>::value,
-                                               (int)A.rows(), (int)A.nnz(), 1, P.JA.data(), P.IA.data(),
-                                               A.getData().data(), 0, entries_.data()));
+        if constexpr (internal::is_sparse_matrix<_MatrixType>::value) {
+            static_assert((int)_MatrixType::SFlags == (int)SparseFlags::CSR, "the CG solver takes CSR matrices");
+            const auto& P = A.getSparsityPattern();
+            CUMAT_AMD_CALL(cm_csr_inverse_diagonal(Context::current().handle(), internal::DType<Scalar>::value,
+                                                   (int)A.rows(), (int)A.nnz(), 1, P.JA.data(), P.IA.data(),
+                                                   A.getData().data(), 0, entries_.data()));
+        } else {
+            CUMAT_ASSERT_DIMENSION(A.rows() == A.cols());
+            CUMAT_AMD_CALL(cm_dense_inverse_diagonal(Context::current().handle(), internal::DType<Scalar>::value,
+                                                     (int)A.rows(), 1, A.data(), 0, entries_.data()));
+        }
     }
     const Vector& entries() const { return entries_; }
     template <typename R>
@@ -67,10 +74,18 @@ public:
     const R& solve(const MatrixBase<R>& b) const { return b.derived(); }
 };
 
+namespace internal {
+template <typename M, bool Sparse = is_sparse_matrix<M>::value>
+struct cg_matrix_ok : std::integral_constant<bool, (int)M::SFlags == (int)SparseFlags::CSR> {};
+template <typename M>
+struct cg_matrix_ok<M, false> : std::true_type {};
+}  // namespace internal
+
 template <typename _MatrixType, typename _Preconditioner = DiagonalPreconditioner<_MatrixType>>
 class ConjugateGradient {
-    static_assert((int)_MatrixType::SFlags == (int)SparseFlags::CSR, "the CG solver takes CSR matrices");
+    static_assert(internal::cg_matrix_ok<_MatrixType>::value, "the CG solver takes CSR or dense matrices");
     static_assert((int)_MatrixType::Batches == 1, "the matrix of the CG solver has one batch (cuMat/src/ConjugateGradient.h:223)");
+    static constexpr bool IsSparse = internal::is_sparse_matrix<_MatrixType>::value;
 
 public:
     typedef _MatrixType MatrixType;
@@ -156,14 +171,24 @@ private:
     {
         CUMAT_ASSERT_ERROR(initialized_);
         CUMAT_ASSERT_DIMENSION(rhs.rows() == matrix_.cols() && rhs.cols() == 1);
-        const auto& P = matrix_.getSparsityPattern();
         int it = 0;
         double err = 0;
-        CUMAT_AMD_CALL(cm_pcg_solve(Context::current().handle(), internal::DType<Scalar>::value, (int)matrix_.rows(),
-                                    (int)matrix_.nnz(), (int)rhs.batches(), P.JA.data(), P.IA.data(),
-                                    matrix_.getData().data(), preconditioner_.entries().data(), rhs.data(), x.data(),
-                                    use_guess ? 1 : 0, (double)tolerance_, (int)maxIterations(), path_, &it, &err,
-                                    &stats_));
+        if constexpr (IsSparse) {
+            const auto& P = matrix_.getSparsityPattern();
+            CUMAT_AMD_CALL(cm_pcg_solve(Context::current().handle(), internal::DType<Scalar>::value,
+                                        (int)matrix_.rows(), (int)matrix_.nnz(), (int)rhs.batches(), P.JA.data(),
+                                        P.IA.data(), matrix_.getData().data(), preconditioner_.entries().data(),
+                                        rhs.data(), x.data(), use_guess ? 1 : 0, (double)tolerance_,
+                                        (int)maxIterations(), path_, &it, &err, &stats_));
+        } else {
+            // the dense product A * p (cuMat/src/ProductOp.h:318-420) in place of the SpMV
+            CUMAT_AMD_CALL(cm_pcg_dense_solve(Context::current().handle(), internal::DType<Scalar>::value,
+                                              (int)matrix_.rows(), (int)rhs.batches(), matrix_.data(), 0,
+                                              ((int)MatrixType::Flags & (int)Flags::RowMajor) ? 1 : 0,
+                                              preconditioner_.entries().data(), rhs.data(), x.data(),
+                                              use_guess ? 1 : 0, (double)tolerance_, (int)maxIterations(), &it,
+                                              &err, &stats_));
+        }
         iterations_ = it;
         error_ = (RealScalar)err;
     }

@@ -21,8 +21,8 @@ CUMAT_NAMESPACE_BEGIN
 
 template <typename _Scalar, int _Rows, int _Columns, int _Batches, int _Flags>
 class Matrix;
-template <typename _SMatrix, typename _Vector>
-class SparseProductOp;
+template <typename _Left, typename _Right>
+class ProductOp;
 
 namespace internal {
 
@@ -426,12 +426,8 @@ template <typename T>
 using if_arith = typename std::enable_if<std::is_arithmetic<T>::value>::type;
 }
 
-#define CUMAT_AMD_BINARY_OPERATOR(op, fn) \
-    template <typename L, typename R> \
-    BinaryOp<L, R, fn> operator op(const MatrixBase<L>& l, const MatrixBase<R>& r) \
-    { \
-        return BinaryOp<L, R, fn>(l.derived(), r.derived()); \
-    } \
+// with a scalar on either side
+#define CUMAT_AMD_BINARY_OPERATOR_SCALAR(op, fn) \
     template <typename L, typename S, typename = internal::if_arith<S>> \
     BinaryOp<L, HostScalar<typename L::Scalar>, fn> operator op(const MatrixBase<L>& l, const S& s) \
     { \
@@ -444,10 +440,21 @@ using if_arith = typename std::enable_if<std::is_arithmetic<T>::value>::type;
         return BinaryOp<HostScalar<typename R::Scalar>, R, fn>(HostScalar<typename R::Scalar>((typename R::Scalar)s), \
                                                                r.derived()); \
     }
+// between two matrices / expressions, and with a scalar
+#define CUMAT_AMD_BINARY_OPERATOR(op, fn) \
+    template <typename L, typename R> \
+    BinaryOp<L, R, fn> operator op(const MatrixBase<L>& l, const MatrixBase<R>& r) \
+    { \
+        return BinaryOp<L, R, fn>(l.derived(), r.derived()); \
+    } \
+    CUMAT_AMD_BINARY_OPERATOR_SCALAR(op, fn)
+// As in cuMat (cuMat/src/BinaryOps.h:286-293), * and / between two matrices
+// are not component-wise: matrix * matrix is the matrix product (ProductOp.h)
+// and the component-wise forms are cwiseMul / cwiseDiv.
 CUMAT_AMD_BINARY_OPERATOR(+, CM_BIN_ADD)
 CUMAT_AMD_BINARY_OPERATOR(-, CM_BIN_SUB)
-CUMAT_AMD_BINARY_OPERATOR(*, CM_BIN_MUL)
-CUMAT_AMD_BINARY_OPERATOR(/, CM_BIN_DIV)
+CUMAT_AMD_BINARY_OPERATOR_SCALAR(*, CM_BIN_MUL)
+CUMAT_AMD_BINARY_OPERATOR_SCALAR(/, CM_BIN_DIV)
 CUMAT_AMD_BINARY_OPERATOR(%, CM_BIN_MOD)
 CUMAT_AMD_BINARY_OPERATOR(&, CM_BIN_AND)
 CUMAT_AMD_BINARY_OPERATOR(|, CM_BIN_OR)
@@ -461,6 +468,7 @@ CUMAT_AMD_BINARY_OPERATOR(>, CM_BIN_GT)
 CUMAT_AMD_BINARY_OPERATOR(<=, CM_BIN_LE)
 CUMAT_AMD_BINARY_OPERATOR(>=, CM_BIN_GE)
 #undef CUMAT_AMD_BINARY_OPERATOR
+#undef CUMAT_AMD_BINARY_OPERATOR_SCALAR
 
 template <typename E>
 UnaryOp<E, CM_UN_NEG> operator-(const MatrixBase<E>& e) { return UnaryOp<E, CM_UN_NEG>(e.derived()); }

@@ -78,14 +78,14 @@ public:
 
     // sparse products written into the window (inplace() = A * x)
     template <typename A, typename V>
-    DenseRef& operator=(const SparseProductOp<A, V>& p)
+    DenseRef& operator=(const ProductOp<A, V>& p)
     {
         CUMAT_ASSERT_DIMENSION(cols_ == 1);
         p.evalTo(data(), rows_, batches_, AssignmentMode::ASSIGN);
         return *this;
     }
     template <typename A, typename V>
-    DenseRef& operator+=(const SparseProductOp<A, V>& p)
+    DenseRef& operator+=(const ProductOp<A, V>& p)
     {
         CUMAT_ASSERT_DIMENSION(cols_ == 1);
         p.evalTo(data(), rows_, batches_, AssignmentMode::ADD);
@@ -187,13 +187,13 @@ public:
 
     // y = A * x with a sparse A: one SpMV launch
     template <typename A, typename V>
-    Matrix(const SparseProductOp<A, V>& p) : rows_(p.rows()), cols_(1), batches_(p.batches())
+    Matrix(const ProductOp<A, V>& p) : rows_(p.rows()), cols_(1), batches_(p.batches())
     {
         allocate();
         p.evalTo(data(), rows_, batches_, AssignmentMode::ASSIGN);
     }
     template <typename A, typename V>
-    Matrix& operator=(const SparseProductOp<A, V>& p)
+    Matrix& operator=(const ProductOp<A, V>& p)
     {
         if (!isExclusiveUse() || rows_ != p.rows() || cols_ != 1 || batches_ != p.batches()) {
             Matrix fresh(p.rows(), 1, p.batches());
@@ -205,14 +205,14 @@ public:
         return *this;
     }
     template <typename A, typename V>
-    Matrix& operator+=(const SparseProductOp<A, V>& p)
+    Matrix& operator+=(const ProductOp<A, V>& p)
     {
         CUMAT_ASSERT_DIMENSION(cols_ == 1);
         p.evalTo(data(), rows_, batches_, AssignmentMode::ADD);
         return *this;
     }
     template <typename A, typename V>
-    Matrix& operator-=(const SparseProductOp<A, V>& p)
+    Matrix& operator-=(const ProductOp<A, V>& p)
     {
         CUMAT_ASSERT_DIMENSION(cols_ == 1);
         p.evalTo(data(), rows_, batches_, AssignmentMode::SUB);

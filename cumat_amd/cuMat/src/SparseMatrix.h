@@ -14,7 +14,7 @@
 
 #include <vector>
 
-#include "ReductionOps.h"
+#include "ProductOp.h"
 
 CUMAT_NAMESPACE_BEGIN
 
@@ -97,9 +97,6 @@ struct SparsityPattern<SparseFlags::ELLPACK> {
     }
 };
 
-template <typename _SMatrix, typename _Vector>
-class SparseProductOp;
-
 template <typename _Scalar, int _Batches, int _SparseFlags>
 class SparseMatrix {
     static_assert(_SparseFlags == SparseFlags::CSR || _SparseFlags == SparseFlags::ELLPACK,
@@ -145,9 +142,9 @@ public:
 
     // sparse * dense column vector (batched: matrix and/or vector batches)
     template <typename V>
-    SparseProductOp<SparseMatrix, V> operator*(const MatrixBase<V>& v) const
+    ProductOp<SparseMatrix, V> operator*(const MatrixBase<V>& v) const
     {
-        return SparseProductOp<SparseMatrix, V>(*this, v.derived());
+        return ProductOp<SparseMatrix, V>(*this, v.derived());
     }
 
     // dense expression evaluated at the stored entries (CSR)
@@ -246,53 +243,32 @@ public:
     }
 };
 
-// y (mode)= A * x, evaluated when assigned to a dense matrix
-template <typename _SMatrix, typename _Vector>
-class SparseProductOp {
-    _SMatrix A_;
-    _Vector x_;
+namespace internal {
+template <typename T>
+struct is_sparse_matrix : std::false_type {};
+template <typename S, int B, int F>
+struct is_sparse_matrix<SparseMatrix<S, B, F>> : std::true_type {};
+}  // namespace internal
 
-public:
-    typedef typename _SMatrix::Scalar Scalar;
-    static_assert(std::is_same<Scalar, typename _Vector::Scalar>::value,
-                  "No implicit casting between the sparse matrix and the vector");
-
-    SparseProductOp(const _SMatrix& A, const _Vector& x) : A_(A), x_(x)
-    {
-        CUMAT_ASSERT_DIMENSION(x.cols() == 1);
-        CUMAT_ASSERT_DIMENSION(x.rows() == A.cols());
-        CUMAT_ASSERT_DIMENSION(A.batches() == x.batches() || A.batches() == 1 || x.batches() == 1);
-    }
-    Index rows() const { return A_.rows(); }
-    Index cols() const { return 1; }
-    Index batches() const { return A_.batches() > x_.batches() ? A_.batches() : x_.batches(); }
-
-    void evalTo(Scalar* y, Index y_rows, Index y_batches, AssignmentMode mode) const
-    {
-        CUMAT_ASSERT_DIMENSION(y_rows == rows() && y_batches == batches());
-        auto x = internal::materialize(x_);
-        const auto& P = A_.getSparsityPattern();
-        evalImpl(P, x, y, mode);
-    }
-
-private:
-    template <typename X>
-    void evalImpl(const SparsityPattern<SparseFlags::CSR>& P, const X& x, Scalar* y, AssignmentMode mode) const
-    {
-        const int dt = internal::DType<Scalar>::value;
+// y (mode)= A * x with a sparse A (ProductOp<SparseMatrix, V>, evaluated
+// when assigned to a dense matrix): one SpMV launch
+// (ProductAssignment over CSR / ELLPACK, cuMat/src/SparseProductEvaluation.h:43-139,194-324)
+template <typename S, int B, int F, typename X>
+void productEvalTo(const SparseMatrix<S, B, F>& A, const X& x, S* y, Index batches, AssignmentMode mode)
+{
+    const auto& P = A.getSparsityPattern();
+    if constexpr (F == SparseFlags::CSR) {
+        const int dt = internal::DType<S>::value;
         CUMAT_AMD_CALL(cm_csr_spmv(Context::current().handle(), dt, dt, (int)P.rows, (int)P.cols, (int)P.nnz,
-                                   (int)batches(), P.JA.data(), P.IA.data(), A_.getData().data(),
-                                   A_.batches() > 1 ? 1 : 0, x.data(), x.batches() > 1 ? 1 : 0, y, (int)mode));
-    }
-    template <typename X>
-    void evalImpl(const SparsityPattern<SparseFlags::ELLPACK>& P, const X& x, Scalar* y, AssignmentMode mode) const
-    {
-        CUMAT_AMD_CALL(cm_ell_spmv(Context::current().handle(), internal::DType<Scalar>::value, (int)P.rows,
-                                   (int)P.cols, (int)P.nnzPerRow, (int)batches(), P.indices.data(),
-                                   A_.getData().data(), A_.batches() > 1 ? 1 : 0, x.data(),
+                                   (int)batches, P.JA.data(), P.IA.data(), A.getData().data(),
+                                   A.batches() > 1 ? 1 : 0, x.data(), x.batches() > 1 ? 1 : 0, y, (int)mode));
+    } else {
+        CUMAT_AMD_CALL(cm_ell_spmv(Context::current().handle(), internal::DType<S>::value, (int)P.rows,
+                                   (int)P.cols, (int)P.nnzPerRow, (int)batches, P.indices.data(),
+                                   A.getData().data(), A.batches() > 1 ? 1 : 0, x.data(),
                                    x.batches() > 1 ? 1 : 0, y, (int)mode));
     }
-};
+}
 
 // typedefs (cuMat/src/SparseMatrix.h:606-623)
 #define CUMAT_AMD_SPARSE_TYPEDEFS(S, s) \

@@ -10,21 +10,27 @@ import ctypes as C
 import numpy as np
 
 from . import _capi as capi
+from . import dense
 from .device import DeviceArray, ctx_handle
 from .sparse import SparseMatrix
 
 
+def _scalar(matrix):
+    return matrix.np_dtype if dense.is_dense(matrix) else matrix.values.np_dtype
+
+
 class DiagonalPreconditioner:
     """Jacobi: entries = A.diagonal().cwiseInverseCheck()
-    (cuMat/src/ConjugateGradient.h:218-246)."""
+    (cuMat/src/ConjugateGradient.h:218-246); A is a SparseMatrix or a dense
+    DeviceArray."""
 
-    def __init__(self, matrix: SparseMatrix):
-        self.entries = matrix.inverse_diagonal()
+    def __init__(self, matrix):
+        self.entries = dense.inverse_diagonal(matrix) if dense.is_dense(matrix) else matrix.inverse_diagonal()
 
 
 class IdentityPreconditioner:
-    def __init__(self, matrix: SparseMatrix):
-        ones = np.ones(matrix.rows, matrix.values.np_dtype)
+    def __init__(self, matrix):
+        ones = np.ones(matrix.rows, _scalar(matrix))
         self.entries = DeviceArray.from_numpy(ones)
 
 
@@ -32,7 +38,9 @@ class ConjugateGradient:
     """x = ConjugateGradient(A).solve(b) with the reference's semantics:
     tolerance defaults to the scalar epsilon, maxIterations to 2*cols,
     error() = sqrt(max_b |r|^2/|b|^2), iterations() excludes the converging
-    step, and all right-hand-side batches iterate until all converge."""
+    step, and all right-hand-side batches iterate until all converge.
+    The matrix is a CSR SparseMatrix or a dense square DeviceArray
+    (ConjugateGradient<MatrixXf>, with the dense product A * p)."""
 
     def __init__(self, matrix: SparseMatrix | None = None, preconditioner=None, path: int = capi.CG_AUTO):
         self._tol = None
@@ -45,7 +53,7 @@ class ConjugateGradient:
         if matrix is not None:
             self.compute(matrix, preconditioner)
 
-    def compute(self, matrix: SparseMatrix, preconditioner=None):
+    def compute(self, matrix, preconditioner=None):
         if matrix.rows != matrix.cols:
             raise capi.CumatArgError(capi.CM_E_DIM, "Matrix must be square")
         if matrix.batches != 1:
@@ -58,7 +66,7 @@ class ConjugateGradient:
     # ---- IterativeSolverBase accessors (cuMat/src/IterativeSolverBase.h:54-111)
     def tolerance(self):
         if self._tol is None:
-            return float(np.finfo(self.matrix.values.np_dtype).eps)
+            return float(np.finfo(_scalar(self.matrix)).eps)
         return self._tol
 
     def setTolerance(self, tol):
@@ -96,6 +104,14 @@ class ConjugateGradient:
         it = C.c_int(0)
         err = C.c_double(0)
         st = capi.CgStats()
+        if dense.is_dense(A):
+            capi.call("cm_pcg_dense_solve", ctx_handle(ctx), A.dtype, A.rows, b.batches, A.ptr, 0,
+                      int(A.row_major), self.preconditioner.entries.ptr, b.ptr, x.ptr, int(use_guess),
+                      float(self.tolerance()), int(self.maxIterations()), C.byref(it), C.byref(err),
+                      C.byref(st))
+            self._iters, self._error = it.value, err.value
+            self.last_stats = {"path": st.path_used, "launches": st.launches, "seconds": st.seconds}
+            return x
         p = A.pattern
         capi.call("cm_pcg_solve", ctx_handle(ctx), A.dtype, p.rows, p.nnz, b.batches, p.JA.ptr, p.IA.ptr,
                   A.values.ptr, self.preconditioner.entries.ptr, b.ptr, x.ptr, int(use_guess),

@@ -88,7 +88,8 @@ enum cm_counter {
     CM_CNT_EVAL_REDUCTION = 5,
     CM_CNT_EVAL_MATMUL_SPARSE = 6,
     CM_CNT_KERNEL_LAUNCH = 7,
-    CM_CNT_COUNT = 8
+    CM_CNT_EVAL_MATMUL = 8,
+    CM_CNT_COUNT = 9
 };
 
 typedef struct cm_context* cm_ctx_t;
@@ -257,6 +258,24 @@ int cm_ell_spmv(cm_ctx_t ctx, int dtype, int rows, int cols, int nnz_per_row,
                 int vals_batched, const void* x, int x_batched, void* y,
                 int mode);
 
+/* Dense matrix-vector product y = A x per batch, the matrix-vector case of
+ * the dense ProductOp, which cuMat evaluates with cuBLAS gemm
+ * (ProductAssignment, cuMat/src/ProductOp.h:318-420). A is rows x cols,
+ * column-major (cuMat's default) or row-major (a_row_major = 1), batch
+ * stride rows*cols if a_batched, else shared by all batches; x has cols
+ * entries per batch (shared if x_batched = 0); y has rows per batch.
+ * mode: CM_ASSIGN, CM_ADD or CM_SUB. */
+int cm_dense_gemv(cm_ctx_t ctx, int dtype, int rows, int cols, int batches,
+                  const void* A, int a_batched, int a_row_major,
+                  const void* x, int x_batched, void* y, int mode);
+
+/* Jacobi inverse diagonal of a dense n x n matrix: inv[i] = A(i,i) == 0 ?
+ * 1 : 1 / A(i,i) (ExtractDiagonalOp + cwiseInverseCheck,
+ * cuMat/src/ConjugateGradient.h:228-230, cuMat/src/UnaryOps.h:177);
+ * 'batches' rows of n if a_batched, else one. */
+int cm_dense_inverse_diagonal(cm_ctx_t ctx, int dtype, int n, int batches,
+                              const void* A, int a_batched, void* inv);
+
 /* Component-wise evaluation into the non-zeros of a CSR matrix
  * (CwiseCSREvaluationKernel, cuMat/src/SparseEvaluation.h:14-34,83-138):
  * the program is evaluated at (row, IA[k], batch) for every stored k and
@@ -328,6 +347,20 @@ int cm_pcg_solve_ex(cm_ctx_t ctx, int dtype,
                     double tol, int max_iter, int per_batch,
                     int* iterations_each, double* error_each,
                     cm_cg_stats* stats);
+
+/* Jacobi-PCG with a dense n x n matrix: ConjugateGradient<MatrixXf> /
+ * <MatrixXd> (cuMat/src/ConjugateGradient.h:66-207 with the dense product
+ * A * p, cuMat/src/ProductOp.h:318-420), same semantics and outputs as
+ * cm_pcg_solve (multi-kernel path, batches <= 64). A is column-major or
+ * row-major (a_row_major = 1), one matrix for all batches unless a_batched
+ * (stride n*n); invdiag (n, or n per batch if a_batched) may be NULL, the
+ * solver then derives it from A. */
+int cm_pcg_dense_solve(cm_ctx_t ctx, int dtype, int n, int batches,
+                       const void* A, int a_batched, int a_row_major,
+                       const void* invdiag, const void* b, void* x,
+                       int use_guess, double tol, int max_iter,
+                       int* iterations_out, double* error_out,
+                       cm_cg_stats* stats);
 
 /* ======================================================================
  * Addition 2 (no reference counterpart): persistent batched CG over many

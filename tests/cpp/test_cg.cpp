@@ -252,4 +252,122 @@ TEST_CASE("Mixed-precision CG with FP64 refinement reaches 1e-12")
     }
 }
 
+// ---- dense matrices: ConjugateGradient<MatrixXf> / <MatrixXd> and A * x,
+// the four cases of tests/TestConjugateGradient.cu:11-99. The reference fills
+// A = U[0,1) + size*I, which is not symmetric; CG reaches the tolerance on it
+// only when its skew part happens to be small, so the solve cases take the
+// symmetric part (U + U^T)/2 + size*I.
+template <typename S>
+static std::vector<S> hostDense(int n, unsigned seed, bool spd)
+{
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    std::vector<double> a((size_t)n * n);
+    for (auto& v : a) v = u(gen);
+    std::vector<S> out((size_t)n * n);  // column-major
+    for (int c = 0; c < n; ++c)
+        for (int r = 0; r < n; ++r)
+            out[r + (size_t)n * c] = (S)(spd ? 0.5 * (a[r + (size_t)n * c] + a[c + (size_t)n * r]) + (r == c ? n : 0)
+                                             : a[r + (size_t)n * c]);
+    return out;
+}
+
+TEST_CASE("CG dense: early out on a zero right-hand side (TestConjugateGradient.cu:11-25)")
+{
+    for (int size : {5, 10, 50, 100}) {
+        auto ah = hostDense<float>(size, 10 + size, false);
+        MatrixXf A = MatrixXf::fromHost(ah.data(), size, size, 1);
+        VectorXf b = VectorXf::Zero(size);
+        ConjugateGradient<MatrixXf> cg(A);
+        VectorXf x = cg.solve(b);
+        REQUIRE(cg.iterations() == 0);
+        REQUIRE(cg.error() == 0);
+        REQUIRE(x.squaredNorm().getScalar() == 0);
+    }
+}
+
+TEST_CASE("CG dense: early out on an exact guess (TestConjugateGradient.cu:27-44)")
+{
+    for (int size : {5, 10, 50, 100}) {
+        auto ah = hostDense<float>(size, 20 + size, false);
+        MatrixXf A = MatrixXf::fromHost(ah.data(), size, size, 1);
+        VectorXf xTruth = VectorXf::fromHost(randomVector<float>(size, 30 + size));
+        VectorXf b = A * xTruth;
+        ConjugateGradient<MatrixXf> cg(A);
+        VectorXf x = cg.solveWithGuess(b, xTruth);
+        REQUIRE(cg.iterations() == 0);
+        REQUIRE(std::fabs(cg.error()) <= 1e-10);
+        auto hx = x.toHostRowMajor(), ht = xTruth.toHostRowMajor();
+        for (int i = 0; i < size; ++i) REQUIRE(hx[i] == ht[i]);
+    }
+}
+
+TEST_CASE("CG dense: solve (TestConjugateGradient.cu:46-71)")
+{
+    for (int size : {5, 10, 50, 100}) {
+        auto ah = hostDense<double>(size, 40 + size, true);
+        MatrixXd A = MatrixXd::fromHost(ah.data(), size, size, 1);
+        VectorXd xTruth = VectorXd::fromHost(randomVector<double>(size, 50 + size));
+        VectorXd b = A * xTruth;
+        ConjugateGradient<MatrixXd> cg(A);
+        cg.setMaxIterations(10 * size);
+        VectorXd x = cg.solve(b);
+        REQUIRE(cg.iterations() > 0);
+        REQUIRE(cg.error() > 0);
+        REQUIRE(cg.iterations() < cg.maxIterations());
+        REQUIRE(cg.error() <= cg.tolerance());
+        auto hx = x.toHostRowMajor(), ht = xTruth.toHostRowMajor();
+        for (int i = 0; i < size; ++i) REQUIRE(std::fabs(hx[i] - ht[i]) <= 1e-4 * std::fabs(ht[i]) + 1e-12);
+    }
+}
+
+TEST_CASE("CG dense: batched right-hand sides (TestConjugateGradient.cu:73-99)")
+{
+    constexpr int size = 10, batches = 3;
+    auto ah = hostDense<double>(size, 60, true);
+    MatrixXd A = MatrixXd::fromHost(ah.data(), size, size, 1);
+    typedef Matrix<double, Dynamic, 1, batches, ColumnMajor> Vec;
+    auto xh = randomVector<double>(size * batches, 61);
+    Vec xTruth = Vec::fromHost(xh.data(), size, 1, batches);
+    Vec b = A * xTruth;
+    ConjugateGradient<MatrixXd> cg(A);
+    cg.setMaxIterations(10 * size);
+    Vec x = cg.solve(b);
+    REQUIRE(cg.iterations() > 0);
+    REQUIRE(cg.error() > 0);
+    REQUIRE(cg.iterations() < cg.maxIterations());
+    REQUIRE(cg.error() <= cg.tolerance());
+    auto hx = x.toHostRowMajor();
+    for (int i = 0; i < size * batches; ++i) REQUIRE(std::fabs(hx[i] - xh[i]) <= 1e-4 * std::fabs(xh[i]) + 1e-12);
+}
+
+TEST_CASE("dense A * x: row-major and column-major against the host, += and -=")
+{
+    const int rows = 70, cols = 45;
+    std::vector<int> h((size_t)rows * cols), xv(cols);
+    for (int i = 0; i < rows * cols; ++i) h[i] = (i * 37) % 23 - 11;
+    for (int j = 0; j < cols; ++j) xv[j] = (j * 13) % 7 - 3;
+    MatrixXi Ac = MatrixXi::fromHost(h.data(), rows, cols, 1);    // column-major storage of h
+    MatrixXiR Ar = MatrixXiR::fromHost(h.data(), rows, cols, 1);  // row-major storage of h
+    VectorXi x = VectorXi::fromHost(xv);
+    VectorXi yc = Ac * x;
+    VectorXi yr = Ar * x;
+    auto hc = yc.toHostRowMajor(), hr = yr.toHostRowMajor();
+    for (int r = 0; r < rows; ++r) {
+        long long ec = 0, er = 0;
+        for (int c = 0; c < cols; ++c) {
+            ec += (long long)h[r + (size_t)rows * c] * xv[c];
+            er += (long long)h[(size_t)r * cols + c] * xv[c];
+        }
+        REQUIRE(hc[r] == ec);
+        REQUIRE(hr[r] == er);
+    }
+    yc += Ac * x;
+    yc -= Ac * x;
+    yc -= Ac * x;
+    auto h2 = yc.toHostRowMajor();
+    for (int r = 0; r < rows; ++r) REQUIRE(h2[r] == 0);
+    REQUIRE_THROWS_AS(VectorXi(Ac * VectorXi::fromHost(std::vector<int>(cols + 1, 1))), std::invalid_argument);
+}
+
 HARNESS_MAIN()

@@ -111,7 +111,7 @@ TEST_CASE("integer cwise ops are bit-exact (C semantics of / and %)")
     int db[1][1][4] = {{{2, -2, -2, 2}}};
     RowVectorXiR a = RowVectorXiR::fromArray(da), b = RowVectorXiR::fromArray(db);
     int q[1][1][4] = {{{-3, -3, 3, 3}}}, m[1][1][4] = {{{-1, 1, -1, 1}}};
-    requireEqual(q, RowVectorXiR(a / b));
+    requireEqual(q, RowVectorXiR(a.cwiseDiv(b)));
     requireEqual(m, RowVectorXiR(a % b));
     int x[1][1][4] = {{{-7 ^ 2, 7 ^ -2, -7 ^ -2, 7 ^ 2}}};
     requireEqual(x, RowVectorXiR(a ^ b));
@@ -130,7 +130,7 @@ TEST_CASE("casting and comparisons")
     MatrixXbR pos = m > 0;
     bool eb[1][2][3] = {{{true, false, true}, {true, false, false}}};
     requireEqual(eb, pos);
-    MatrixXiR sel = pos.cast<int>() * m;
+    MatrixXiR sel = pos.cast<int>().cwiseMul(m);
     int es[1][2][3] = {{{1, 0, 3}, {4, 0, 0}}};
     requireEqual(es, sel);
 }
