@@ -251,6 +251,28 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
                      "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(name, out[name])
     del ts
+    # ---- dense matrix-vector product, 16384 x 16384 FP32 (1 GiB; two
+    # rotating matrices so launches stream from HBM)
+    Nd = 16384
+    for rm, label in ((False, "dense_gemv_16384_colmajor"), (True, "dense_gemv_16384_rowmajor")):
+        mats = [cm.DeviceArray(Nd, Nd, dtype=np.float32, row_major=rm, ctx=ctx) for _ in range(2)]
+        for m_ in mats:
+            cm.assign(m_, cm.as_expr(0.5), ctx=ctx)
+        xv = cm.DeviceArray.from_numpy(rng.uniform(-1, 1, Nd).astype(np.float32), ctx=ctx)
+        yv = cm.DeviceArray(Nd, dtype=np.float32, ctx=ctx)
+        for k in range(4):
+            cm.dense.matvec(mats[k % 2], xv, out=yv, ctx=ctx)
+        ctx.sync()
+        reps = 20
+        timer.start(stream)
+        for k in range(reps):
+            cm.dense.matvec(mats[k % 2], xv, out=yv, ctx=ctx)
+        ms = timer.stop_ms(stream) / reps
+        by = 4 * Nd * Nd + 8 * Nd
+        out[label] = {"us": round(ms * 1000, 1), "GBps": round(by / (ms * 1e-3) / 1e9, 1),
+                      "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(label, out[label])
+        del mats
     # ---- plain CG on 128^2 (the reference's CPU-runnable configuration)
     from cumat_amd.problems import poisson2d
     JA, IA, V = poisson2d(128)

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(CG_BS) void cg_dense_kernel(int n, const T* __restr
     const int ntiles = div_up_dev(n, DN_TILE);
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int r0 = t * DN_TILE;
-        const T v = dense_tile<T, ROW_MAJOR>(n, n, Ab, pb, r0, red);
+        const T v = dense_tile<T, ROW_MAJOR>(n, n, Ab, pb, r0, 0, n, red);
         const int r = r0 + (int)threadIdx.x;
         if (threadIdx.x < 64 && r < n) {
             tmp[off + r] = v;
@@ -862,12 +862,13 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     const T* ax_pre = nullptr;
     if (dense && use_guess) {
         // t = A x_0, read by the setup kernel
+        // one column range: the same summation order as the iteration kernel
         if (dense->row_major)
             dense_gemv_kernel<T, true><<<dim3(dense_grid(n), B), DN_BS, 0, ctx->stream>>>(
-                n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN);
+                n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN, 1, n, nullptr);
         else
             dense_gemv_kernel<T, false><<<dim3(dense_grid(n), B), DN_BS, 0, ctx->stream>>>(
-                n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN);
+                n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN, 1, n, nullptr);
         CM_LAUNCH_CHECK();
         ax_pre = t;
         launches += 1;

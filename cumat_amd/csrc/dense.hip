@@ -9,14 +9,24 @@ static int launch_dense_gemv(cm_ctx_t ctx, int rows, int cols, int batches, cons
                              int a_row_major, const void* x, int x_batched, void* y, int mode)
 {
     const long long bstride = a_batched ? (long long)rows * cols : 0;
-    dim3 grid(dense_grid(rows), batches);
+    int nsplit = 1, csz = cols;
+    dense_split(rows, cols, nsplit, csz);
+    // per-range partial vectors in the context's stream-ordered scratch
+    void* part = nullptr;
+    if (nsplit > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * (size_t)nsplit * rows * batches, &part));
+    dim3 grid(dense_grid(rows, nsplit), batches);
     if (a_row_major)
         dense_gemv_kernel<T, true><<<grid, DN_BS, 0, ctx->stream>>>(rows, cols, (const T*)A, bstride, (const T*)x,
-                                                                    x_batched, (T*)y, mode);
+                                                                    x_batched, (T*)y, mode, nsplit, csz, (T*)part);
     else
         dense_gemv_kernel<T, false><<<grid, DN_BS, 0, ctx->stream>>>(rows, cols, (const T*)A, bstride, (const T*)x,
-                                                                     x_batched, (T*)y, mode);
+                                                                     x_batched, (T*)y, mode, nsplit, csz, (T*)part);
     CM_LAUNCH_CHECK();
+    if (nsplit > 1) {
+        dense_gemv_combine_kernel<T><<<dim3(div_up(rows, 256), batches), 256, 0, ctx->stream>>>(
+            rows, nsplit, (const T*)part, (T*)y, mode);
+        CM_LAUNCH_CHECK();
+    }
     return CM_OK;
 }
 

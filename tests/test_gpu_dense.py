@@ -36,15 +36,17 @@ def test_matvec_float_shapes(gpu, row_major, shape):
 
 
 @pytest.mark.parametrize("row_major", [False, True])
-def test_matvec_int32_exact_and_modes(gpu, row_major):
+@pytest.mark.parametrize("shape", [(97, 203), (130, 2600)])  # one column range / five ranges + combine
+def test_matvec_int32_exact_and_modes(gpu, row_major, shape):
     cm = gpu
     rng = np.random.default_rng(5)
-    a = rng.integers(-50, 50, (97, 203)).astype(np.int32)
-    x = rng.integers(-50, 50, 203).astype(np.int32)
+    rows, cols = shape
+    a = rng.integers(-50, 50, (rows, cols)).astype(np.int32)
+    x = rng.integers(-50, 50, cols).astype(np.int32)
     A = _dense(cm, a, row_major)
     y = cm.dense.matvec(A, cm.DeviceArray.from_numpy(x))
     np.testing.assert_array_equal(y.numpy(), a.astype(np.int64) @ x)
-    y0 = rng.integers(-9, 9, 97).astype(np.int32)
+    y0 = rng.integers(-9, 9, rows).astype(np.int32)
     out = cm.DeviceArray.from_numpy(y0)
     cm.dense.matvec(A, cm.DeviceArray.from_numpy(x), out=out, mode=cm.capi.ADD)
     np.testing.assert_array_equal(out.numpy(), y0 + a @ x)
