@@ -59,6 +59,11 @@ constexpr int CG_CAP = 4096;   // staged non-zeros per setup row block
 constexpr int CG_EPT = 8;      // staged non-zeros per lane in K1
 constexpr int CG_VPT = 4;      // vector elements per thread per sweep
 constexpr int CG_GMAX = 2048;  // max blocks (= max partials) per kernel
+// max blocks of K2 / K3: each of their blocks first folds all of K1's
+// partials, so fewer, longer blocks pay off (measured at 2048^2, us per
+// iteration: K2/K3 grid 2048 -> 72.8, 1024 -> 69.1, 768 -> 70.2, 512 ->
+// 70.1, 256 -> 73.6; K1 itself keeps 2048 blocks: 1536 -> 71.6, 1024 -> 78.8)
+constexpr int CG_G2MAX = 1024;
 // from this many unknowns (n x batches) on, x is updated in K3 instead of K2
 // (measured on MI355X: +6 % at 1024^2, -14 % at 512^2 where latency rules)
 constexpr long long CG_XK3_MIN = 1 << 20;
@@ -830,7 +835,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     // 512-row blocks with the same grid.
     const int gcap = std::max(8, CG_GMAX / B);
     const int G1 = walk_grid(div_up(div_up(n, 64), CG_BS / 64), gcap);
-    const int G2 = walk_grid(div_up(n, (long long)CG_BS * CG_VPT), gcap);
+    const int G2 = walk_grid(div_up(n, (long long)CG_BS * CG_VPT), std::min(gcap, CG_G2MAX));
     CM_TRY(tmp.get(sizeof(T) * 3 * (size_t)B * std::max(G1, G2), &p1_));
     CM_TRY(tmp.get(sizeof(T) * 2 * (size_t)B * G2, &p2_));
     T* r = (T*)r_;
