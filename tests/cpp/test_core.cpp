@@ -6,6 +6,7 @@
 #include <cuMat/Core>
 
 #include <random>
+#include <sstream>
 
 #include "harness.h"
 
@@ -256,6 +257,22 @@ TEST_CASE("dimension and type errors throw")
     REQUIRE_THROWS_AS(VectorXf(a + b), std::invalid_argument);
     Matrix2i m;
     REQUIRE_THROWS_AS(m.slice(1), std::out_of_range);
+}
+
+TEST_CASE("operator<< prints shape, storage and the batches")
+{
+    int d[2][2][3] = {{{1, 2, 3}, {4, 5, 6}}, {{7, 8, 9}, {10, 11, 12}}};
+    auto m = BMatrixXiR::fromArray(d);
+    std::ostringstream ss;
+    ss << m;
+    REQUIRE_MSG(ss.str() == "Matrix: rows=2 (dynamic), cols=3 (dynamic), batches=2 (dynamic), storage=Row-Major\n"
+                            "[\n 1  2  3\n 4  5  6\n],[\n 7  8  9\n10 11 12\n]",
+                ss.str());
+    std::ostringstream s1;
+    s1 << VectorXf::fromHost(std::vector<float>{1.5f, -2.0f});
+    REQUIRE_MSG(s1.str() == "Matrix: rows=2 (dynamic), cols=1 (compile-time), batches=1 (compile-time), "
+                            "storage=Column-Major\n1.5\n -2",
+                s1.str());
 }
 
 HARNESS_MAIN()

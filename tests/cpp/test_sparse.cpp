@@ -5,6 +5,7 @@
 #include <cuMat/Sparse>
 
 #include <random>
+#include <sstream>
 
 #include "harness.h"
 
@@ -201,6 +202,18 @@ TEST_CASE("malformed sparsity patterns are rejected on the host")
     SMatrix_t A(fixturePattern());
     VectorXf wrong(4);
     REQUIRE_THROWS_AS(VectorXf(A * wrong), std::invalid_argument);
+}
+
+TEST_CASE("operator<< of a CSR matrix prints pattern and values")
+{
+    SMatrix_t A(fixturePattern());
+    A.getData().slice(0) = VectorXf::fromHost(std::vector<float>{1, 4, 2, 3, 5, 7, 8, 9, 6});
+    std::ostringstream ss;
+    ss << A;
+    REQUIRE_MSG(ss.str() == "SparseMatrix: \n rows=4, cols=5, batches=1 (compile-time), storage=CSR\n"
+                            " Outer Indices (row): 0 2 4 7 9\n Inner Indices (column): 0 1 1 2 0 3 4 2 4\n"
+                            " Data:\n   1 4 2 3 5 7 8 9 6\n",
+                ss.str());
 }
 
 HARNESS_MAIN()
