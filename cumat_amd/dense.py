@@ -36,7 +36,8 @@ def inverse_diagonal(A: DeviceArray, ctx=None) -> DeviceArray:
     cuMat/src/UnaryOps.h:177); one column per batch of A."""
     if A.rows != A.cols:
         raise capi.CumatArgError(capi.CM_E_DIM, "inverse_diagonal: matrix must be square")
-    out = DeviceArray(A.rows, 1, A.batches, dtype=A.np_dtype, ctx=A.ctx)
+    ctx = A.ctx if ctx is None else ctx  # ordered with the other work on A
+    out = DeviceArray(A.rows, 1, A.batches, dtype=A.np_dtype, ctx=ctx)
     capi.call("cm_dense_inverse_diagonal", ctx_handle(ctx), A.dtype, A.rows, A.batches, A.ptr,
               int(A.batches > 1), out.ptr)
     return out

@@ -22,10 +22,16 @@ def _scalar(matrix):
 class DiagonalPreconditioner:
     """Jacobi: entries = A.diagonal().cwiseInverseCheck()
     (cuMat/src/ConjugateGradient.h:218-246); A is a SparseMatrix or a dense
-    DeviceArray."""
+    DeviceArray. The entries are computed on the matrix's context (or ctx)
+    and are complete when the constructor returns, so a solve on any other
+    context may read them."""
 
-    def __init__(self, matrix):
-        self.entries = dense.inverse_diagonal(matrix) if dense.is_dense(matrix) else matrix.inverse_diagonal()
+    def __init__(self, matrix, ctx=None):
+        if dense.is_dense(matrix):
+            self.entries = dense.inverse_diagonal(matrix, ctx=ctx)
+        else:
+            self.entries = matrix.inverse_diagonal(ctx=ctx)
+        capi.call("cm_ctx_sync", ctx_handle(self.entries.ctx))
 
 
 class IdentityPreconditioner:

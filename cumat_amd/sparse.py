@@ -94,8 +94,11 @@ class SparseMatrix:
     __matmul__ = matmul = matvec
 
     def inverse_diagonal(self, ctx=None) -> DeviceArray:
+        """Runs on the matrix's context unless ctx says otherwise, so it is
+        ordered with the other work on that matrix."""
         p = self.pattern
-        out = DeviceArray(p.rows, 1, self.batches, dtype=self.values.np_dtype, ctx=p.ctx)
+        ctx = p.ctx if ctx is None else ctx
+        out = DeviceArray(p.rows, 1, self.batches, dtype=self.values.np_dtype, ctx=ctx)
         capi.call("cm_csr_inverse_diagonal", ctx_handle(ctx), self.dtype, p.rows, p.nnz, self.batches,
                   p.JA.ptr, p.IA.ptr, self.values.ptr, int(self.batches > 1), out.ptr)
         return out

@@ -192,3 +192,38 @@ def test_fused_and_three_launch_agree(gpu):
     assert abs(out[1][1] - out[3][1]) <= 1
     np.testing.assert_allclose(out[1][0], out[3][0], rtol=0, atol=1e-8)
 
+
+
+def test_solves_repeat_exactly_across_contexts(gpu):
+    """Matrix on one context, solves on another, the allocator's cache
+    refilled with NaN before every solve: each solve gives the same iteration
+    count and a bit-identical, finite x (the Jacobi entries are computed on
+    the matrix's context and complete before the constructor returns, so a
+    solve on any stream reads them finished)."""
+    cm = gpu
+    from cumat_amd.problems import grid_laplacian, lognormal_sampler
+    ctx1, ctx2 = cm.Context(0), cm.Context(0)
+    P = grid_laplacian((100, 100, 100), lognormal_sampler(np.random.default_rng(41), 0.0, 1.0), edges=False)
+    n = P["n"]
+    A = cm.SparseMatrix(cm.SparsityPattern(n, n, P["JA"], P["IA"], ctx=ctx1),
+                        cm.DeviceArray.from_numpy(P["vals"], ctx=ctx1))
+    b = cm.DeviceArray.from_numpy(np.random.default_rng(42).uniform(-1, 1, n), ctx=ctx2)
+    runs = []
+    for rep in range(3):
+        junk = []
+        for nbytes in [8 * n] * 4 + [16 * n] * 2 + [65536, 16384, 4096] * 2:
+            a = cm.DeviceArray(nbytes // 8, dtype=np.float64, ctx=ctx2)
+            cm.assign(a, cm.as_expr(np.float64("nan")), ctx=ctx2)
+            junk.append(a)
+        ctx2.sync()
+        for a in junk:
+            a.free()
+        x = cm.DeviceArray(n, dtype=np.float64, ctx=ctx2)
+        cg = cm.ConjugateGradient(A, path=1).setTolerance(1e-8).setMaxIterations(5000)
+        cg.solve(b, x, ctx=ctx2)
+        runs.append((cg.iterations(), x.numpy()))
+    assert np.isfinite(runs[0][1]).all()
+    assert runs[0][0] < 5000
+    for it, xs in runs[1:]:
+        assert it == runs[0][0]
+        np.testing.assert_array_equal(xs, runs[0][1])
