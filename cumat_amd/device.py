@@ -155,9 +155,11 @@ class DeviceArray:
             return _unpack(h, self.rows, self.cols, self.batches, self.row_major)
         return h
 
-    def copy_from(self, other: "DeviceArray") -> None:
+    def copy_from(self, other: "DeviceArray", ctx=None) -> None:
+        """Device copy, ordered on ctx (default: this array's context)."""
         assert other.nbytes == self.nbytes
-        capi.call("cm_memcpy_d2d", ctx_handle(self.ctx), self.ptr, other.ptr, self.nbytes)
+        ctx = self.ctx if ctx is None else ctx
+        capi.call("cm_memcpy_d2d", ctx_handle(ctx), self.ptr, other.ptr, self.nbytes)
 
     def fill_zero(self) -> None:
         """Matrix::setZero (cuMat/src/Matrix.h:1232-1238)."""

@@ -98,7 +98,7 @@ class ConjugateGradient:
 
     def solveWithGuess(self, b: DeviceArray, guess: DeviceArray, ctx=None) -> DeviceArray:
         x = DeviceArray(guess.rows, 1, guess.batches, dtype=guess.np_dtype, ctx=guess.ctx)
-        x.copy_from(guess)
+        x.copy_from(guess, ctx=ctx)  # on the solve's stream, so the solve sees the copy
         return self._solve(b, x, True, ctx)
 
     def _solve(self, b, x, use_guess, ctx):
