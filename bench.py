@@ -102,7 +102,7 @@ def main():
 
     # ------------------------------------------------------------ timed CG
     if args.kernels_only:
-        args.steps = args.warmup = 1
+        args.steps = args.warmup = 3  # a few CG iterations, so every kernel is counted mid-solve
     cg.setMaxIterations(args.warmup)
     cg.solve(bd, xd, ctx=ctx)
     ctx.sync()
