@@ -402,7 +402,9 @@ def bench_mixed(cm, capi, ctx):
     t64 = time.perf_counter() - t0
     out["fp64_cg"] = {"seconds": round(t64, 3), "iterations": cg.iterations(), "rel_residual": true_rel(x64)}
     xref = x64.numpy()
-    for storage, itol in (("f32", 1e-4), ("bf16", 1e-3)):
+    # inner tolerances: the fastest of a sweep on this system (FP32 inner CG
+    # stalls near 1e-6, BF16 storage near 1e-3)
+    for storage, itol in (("f32", 1e-5), ("bf16", 1e-3)):
         xd = cm.DeviceArray(n, dtype=np.float64, ctx=ctx)
         st = pcg_ir(dJA, dIA, dV, db, xd, storage=storage, tol=1e-12, inner_tol=itol, max_outer=60, ctx=ctx)
         xh = xd.numpy()
