@@ -368,6 +368,18 @@ TEST_CASE("dense A * x: row-major and column-major against the host, += and -=")
     auto h2 = yc.toHostRowMajor();
     for (int r = 0; r < rows; ++r) REQUIRE(h2[r] == 0);
     REQUIRE_THROWS_AS(VectorXi(Ac * VectorXi::fromHost(std::vector<int>(cols + 1, 1))), std::invalid_argument);
+    // an expression as the left factor is evaluated first: (2 A) x and A^T x
+    VectorXi y2 = (Ac * 2) * x;
+    auto h3 = y2.toHostRowMajor();
+    for (int r = 0; r < rows; ++r) REQUIRE(h3[r] == 2 * hc[r]);
+    VectorXi xr = VectorXi::fromHost(std::vector<int>(rows, 1));
+    VectorXi yt = Ac.transpose() * xr;
+    auto h4 = yt.toHostRowMajor();
+    for (int c = 0; c < cols; ++c) {
+        long long e = 0;
+        for (int r = 0; r < rows; ++r) e += h[r + (size_t)rows * c];
+        REQUIRE(h4[c] == e);
+    }
 }
 
 HARNESS_MAIN()

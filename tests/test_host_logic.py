@@ -134,3 +134,28 @@ def test_batched_poisson_block_structure():
         cols = IA[JA[lo]:JA[hi]]
         assert cols.min() >= lo and cols.max() < hi  # block diagonal
         assert hi - lo == int(P["sides"][s]) ** 2
+
+
+def test_dense_matvec_argument_checks_run_before_the_call():
+    from cumat_amd import dense
+    A = _arr(4, 5, ptr=0x1000)
+    with pytest.raises(capi.CumatArgError):
+        dense.matvec(A, _arr(4, ptr=0x2000))                      # cols != rows of x
+    with pytest.raises(capi.CumatArgError):
+        dense.matvec(A, _arr(5, 2, ptr=0x2000))                   # x is not a column
+    with pytest.raises(TypeError):
+        dense.matvec(A, _arr(5, dtype=np.float64, ptr=0x2000))    # no implicit casting
+    with pytest.raises(capi.CumatArgError):
+        dense.matvec(_arr(4, 5, 2, ptr=0x1000), _arr(5, 1, 3, ptr=0x2000))  # batches 2 vs 3
+    with pytest.raises(capi.CumatArgError):
+        dense.matvec(A, _arr(5, ptr=0x2000), out=_arr(3, ptr=0x3000))      # wrong output shape
+    with pytest.raises(capi.CumatArgError):
+        dense.inverse_diagonal(A)                                  # not square
+
+
+def test_cg_rejects_non_square_and_batched_dense_matrices():
+    from cumat_amd.solvers import ConjugateGradient
+    with pytest.raises(capi.CumatArgError):
+        ConjugateGradient(_arr(4, 5, ptr=0x1000))
+    with pytest.raises(capi.CumatArgError):
+        ConjugateGradient(_arr(4, 4, 2, ptr=0x1000))
