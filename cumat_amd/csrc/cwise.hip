@@ -22,7 +22,10 @@
 namespace cm {
 
 constexpr int CW_BS = 256;
-constexpr int CW_VEC = 4;
+// elements per thread and pass: 8 for 32-bit programs (the interpretation
+// cost is shared by more elements: measured +8 % to +47 % on 2^26-element
+// programs against 4), 4 for 64-bit ones (register banks twice as wide)
+template <bool W> struct CwVec { static constexpr int v = W ? 4 : 8; };
 
 struct CwProgram {
     cm_leaf leaves[CM_CWISE_MAX_LEAVES];
@@ -44,10 +47,10 @@ struct CwProgram {
 // W: 64-bit lanes (any double in the program) or 32-bit lanes.
 template <bool W> struct Reg;
 template <> struct Reg<false> {
-    uint32_t v[CW_VEC];
+    uint32_t v[CwVec<false>::v];
 };
 template <> struct Reg<true> {
-    uint64_t v[CW_VEC];
+    uint64_t v[CwVec<true>::v];
 };
 
 template <bool W> __device__ __forceinline__ float getf(const Reg<W>& r, int k) { return __uint_as_float((uint32_t)r.v[k]); }
@@ -86,7 +89,7 @@ __device__ __forceinline__ void convert(Reg<W>& r, int from, int to)
 {
     if (from == to) return;
 #pragma unroll
-    for (int k = 0; k < CW_VEC; ++k) {
+    for (int k = 0; k < CwVec<W>::v; ++k) {
         double d = 0.0;
         long long i = 0;
         bool isf = false;
@@ -120,11 +123,11 @@ __device__ __forceinline__ void binary(int fn, int dt, Reg<W>& a, const Reg<W>& 
     // structs' addresses would force them into scratch memory)
     Reg<W> x, y;
 #pragma unroll
-    for (int k = 0; k < CW_VEC; ++k) {
+    for (int k = 0; k < CwVec<W>::v; ++k) {
         x.v[k] = rev ? b.v[k] : a.v[k];
         y.v[k] = rev ? a.v[k] : b.v[k];
     }
-#define CW_LOOP(STMT) _Pragma("unroll") for (int k = 0; k < CW_VEC; ++k) { STMT; } break
+#define CW_LOOP(STMT) _Pragma("unroll") for (int k = 0; k < CwVec<W>::v; ++k) { STMT; } break
     if (dt == CM_F32) {
         switch (fn) {
         case CM_BIN_ADD: CW_LOOP(setf(a, k, getf(x, k) + getf(y, k)));
@@ -277,7 +280,7 @@ __device__ __forceinline__ void unary(int fn, int dt, Reg<W>& a)
 template <bool W, int N> struct Bank {
     using E = typename std::conditional<W, uint64_t, uint32_t>::type;
     typedef E V __attribute__((ext_vector_type(N)));
-    V k[CW_VEC];
+    V k[CwVec<W>::v];
 };
 
 template <bool W, int NL>
@@ -285,14 +288,14 @@ __device__ __forceinline__ void fetch_leaf(const Bank<W, NL>& lv, int l, Reg<W>&
 {
     l = __builtin_amdgcn_readfirstlane(l);
 #pragma unroll
-    for (int k = 0; k < CW_VEC; ++k) out.v[k] = lv.k[k][l];
+    for (int k = 0; k < CwVec<W>::v; ++k) out.v[k] = lv.k[k][l];
 }
 
 template <bool W>
 __device__ __forceinline__ void imm(const cm_instr& I, Reg<W>& out)
 {
 #pragma unroll
-    for (int k = 0; k < CW_VEC; ++k) {
+    for (int k = 0; k < CwVec<W>::v; ++k) {
         if (I.dtype == CM_F32) setf(out, k, (float)I.fimm);
         else if (I.dtype == CM_F64) setd(out, k, I.fimm);
         else seti(out, k, (int)I.iimm);
@@ -307,14 +310,14 @@ __device__ __forceinline__ void stack_get(const Stack<W>& st, int s, Reg<W>& out
 {
     s = __builtin_amdgcn_readfirstlane(s);
 #pragma unroll
-    for (int k = 0; k < CW_VEC; ++k) out.v[k] = st.k[k][s];
+    for (int k = 0; k < CwVec<W>::v; ++k) out.v[k] = st.k[k][s];
 }
 template <bool W>
 __device__ __forceinline__ void stack_put(Stack<W>& st, int s, const Reg<W>& v)
 {
     s = __builtin_amdgcn_readfirstlane(s);
 #pragma unroll
-    for (int k = 0; k < CW_VEC; ++k) st.k[k][s] = v.v[k];
+    for (int k = 0; k < CwVec<W>::v; ++k) st.k[k][s] = v.v[k];
 }
 
 template <bool W, int NL>
@@ -363,14 +366,14 @@ __device__ __forceinline__ int run_program(const CwProgram& P, const Bank<W, NL>
             if (I.dtype == CM_F32) {
                 const float c = (float)I.fimm;
 #pragma unroll
-                for (int k = 0; k < CW_VEC; ++k) setf(acc, k, getf(acc, k) + getf(t, k) * c);
+                for (int k = 0; k < CwVec<W>::v; ++k) setf(acc, k, getf(acc, k) + getf(t, k) * c);
             } else if (I.dtype == CM_F64) {
 #pragma unroll
-                for (int k = 0; k < CW_VEC; ++k) setd(acc, k, getd(acc, k) + getd(t, k) * I.fimm);
+                for (int k = 0; k < CwVec<W>::v; ++k) setd(acc, k, getd(acc, k) + getd(t, k) * I.fimm);
             } else {
                 const uint32_t c = (uint32_t)I.iimm;
 #pragma unroll
-                for (int k = 0; k < CW_VEC; ++k) acc.v[k] = (uint32_t)acc.v[k] + (uint32_t)t.v[k] * c;
+                for (int k = 0; k < CwVec<W>::v; ++k) acc.v[k] = (uint32_t)acc.v[k] + (uint32_t)t.v[k] * c;
             }
             break;
         }
@@ -380,7 +383,7 @@ __device__ __forceinline__ int run_program(const CwProgram& P, const Bank<W, NL>
             stack_get<W>(st, I.arg, a);
             I.src == CM_SRC_IMM ? imm<W>(I, b) : fetch_leaf<W, NL>(lv, (int)I.iimm, b);
 #pragma unroll
-            for (int k = 0; k < CW_VEC; ++k) acc.v[k] = ((uint32_t)acc.v[k] != 0) ? a.v[k] : b.v[k];
+            for (int k = 0; k < CwVec<W>::v; ++k) acc.v[k] = ((uint32_t)acc.v[k] != 0) ? a.v[k] : b.v[k];
             break;
         }
         default:
@@ -460,14 +463,14 @@ template <bool W, int NL, bool FLAT>
 __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
 {
     const long long n = P.n;
-    const long long stride = (long long)gridDim.x * CW_BS * CW_VEC;
-    for (long long e0 = ((long long)blockIdx.x * CW_BS + threadIdx.x) * CW_VEC; e0 < n; e0 += stride) {
+    const long long stride = (long long)gridDim.x * CW_BS * CwVec<W>::v;
+    for (long long e0 = ((long long)blockIdx.x * CW_BS + threadIdx.x) * CwVec<W>::v; e0 < n; e0 += stride) {
         Bank<W, NL> lv;
-        const bool full = e0 + CW_VEC <= n;
-        Coords q[CW_VEC];
+        const bool full = e0 + CwVec<W>::v <= n;
+        Coords q[CwVec<W>::v];
         if (!FLAT) {
 #pragma unroll
-            for (int k = 0; k < CW_VEC; ++k) q[k] = decompose(P, min(e0 + k, n - 1));
+            for (int k = 0; k < CwVec<W>::v; ++k) q[k] = decompose(P, min(e0 + k, n - 1));
         }
         // ---- fetch every leaf for the 4 elements before computing
 #pragma unroll
@@ -475,24 +478,28 @@ __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
             const cm_leaf& L = P.leaves[l];
             Reg<W> t;
 #pragma unroll
-            for (int k = 0; k < CW_VEC; ++k) t.v[k] = 0;
+            for (int k = 0; k < CwVec<W>::v; ++k) t.v[k] = 0;
             if (l >= P.nleaves) {
                 // unused leaf slot
             } else if ((P.leaf_scalar >> l) & 1) {
                 load_raw<W>(L, 0, t, 0);
 #pragma unroll
-                for (int k = 1; k < CW_VEC; ++k) t.v[k] = t.v[0];
+                for (int k = 1; k < CwVec<W>::v; ++k) t.v[k] = t.v[0];
             } else if (FLAT || ((P.leaf_direct >> l) & 1)) {
                 // destination-shaped leaf: contiguous, vectorizable
                 const int sz = (L.dtype == CM_F64) ? 8 : (L.dtype == CM_BOOL ? 1 : 4);
                 if (full && sz == 4 && (((uintptr_t)L.ptr & 15) == 0) && (FLAT || P.dst_direct)) {
                     // streamed once: non-temporal (measured +3-7 % on 2^26-element programs)
                     typedef unsigned int u4v __attribute__((ext_vector_type(4)));
-                    const u4v u = __builtin_nontemporal_load(reinterpret_cast<const u4v*>((const uint32_t*)L.ptr + e0));
-                    t.v[0] = u.x; t.v[1] = u.y; t.v[2] = u.z; t.v[3] = u.w;
+#pragma unroll
+                    for (int q = 0; q < CwVec<W>::v / 4; ++q) {
+                        const u4v u = __builtin_nontemporal_load(
+                            reinterpret_cast<const u4v*>((const uint32_t*)L.ptr + e0 + 4 * q));
+                        t.v[4 * q] = u.x; t.v[4 * q + 1] = u.y; t.v[4 * q + 2] = u.z; t.v[4 * q + 3] = u.w;
+                    }
                 } else {
 #pragma unroll
-                    for (int k = 0; k < CW_VEC; ++k) {
+                    for (int k = 0; k < CwVec<W>::v; ++k) {
                         long long e = min(e0 + k, n - 1);
                         long long off = (FLAT || P.dst_direct) ? e
                                          : (q[k].r * P.dst_sr + q[k].c * P.dst_sc + q[k].b * P.dst_sb);
@@ -501,7 +508,7 @@ __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
                 }
             } else {
 #pragma unroll
-                for (int k = 0; k < CW_VEC; ++k) {
+                for (int k = 0; k < CwVec<W>::v; ++k) {
                     bool z;
                     long long off = leaf_offset(L, q[k], z);
                     load_raw<W>(L, z ? 0 : off, t, k);
@@ -509,19 +516,41 @@ __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
                 }
             }
 #pragma unroll
-            for (int k = 0; k < CW_VEC; ++k) lv.k[k][l] = t.v[k];
+            for (int k = 0; k < CwVec<W>::v; ++k) lv.k[k][l] = t.v[k];
         }
         Reg<W> acc;
         run_program<W, NL>(P, lv, acc);
         // ---- store
-        if (FLAT && full && P.dst_dtype == CM_F32 && P.mode == CM_ASSIGN && (((uintptr_t)P.dst & 15) == 0)) {
+        if (FLAT && full && (P.dst_dtype == CM_F32 || P.dst_dtype == CM_I32) && (((uintptr_t)P.dst & 15) == 0)) {
+            // 16-byte stores; a compound mode reads the old values with 16-byte loads
             typedef unsigned int u4v __attribute__((ext_vector_type(4)));
-            u4v u;
-            u.x = (uint32_t)acc.v[0]; u.y = (uint32_t)acc.v[1]; u.z = (uint32_t)acc.v[2]; u.w = (uint32_t)acc.v[3];
-            __builtin_nontemporal_store(u, reinterpret_cast<u4v*>((float*)P.dst + e0));
+            const bool f32 = P.dst_dtype == CM_F32;
+            if (P.mode == CM_ASSIGN) {
+#pragma unroll
+                for (int q = 0; q < CwVec<W>::v / 4; ++q) {
+                    u4v u;
+                    u.x = (uint32_t)acc.v[4 * q]; u.y = (uint32_t)acc.v[4 * q + 1];
+                    u.z = (uint32_t)acc.v[4 * q + 2]; u.w = (uint32_t)acc.v[4 * q + 3];
+                    __builtin_nontemporal_store(u, reinterpret_cast<u4v*>((uint32_t*)P.dst + e0 + 4 * q));
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < CwVec<W>::v / 4; ++q) {
+                    u4v* d = reinterpret_cast<u4v*>((uint32_t*)P.dst + e0 + 4 * q);
+                    const u4v old = *d;
+                    u4v u;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t v = (uint32_t)acc.v[4 * q + j];
+                        u[j] = f32 ? __float_as_uint(assign_mode<float>(P.mode, __uint_as_float(old[j]), __uint_as_float(v)))
+                                   : (uint32_t)assign_mode<int>(P.mode, (int)old[j], (int)v);
+                    }
+                    __builtin_nontemporal_store(u, d);
+                }
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < CW_VEC; ++k) {
+            for (int k = 0; k < CwVec<W>::v; ++k) {
                 if (e0 + k < n) {
                     long long off = (FLAT || P.dst_direct) ? (e0 + k)
                                      : (q[k].r * P.dst_sr + q[k].c * P.dst_sc + q[k].b * P.dst_sb);
@@ -550,7 +579,7 @@ __global__ void cwise_csr_kernel(const CwProgram P, const int* __restrict__ JA, 
         for (int l = 0; l < NL; ++l) {
             Reg<W> t;
 #pragma unroll
-            for (int j = 0; j < CW_VEC; ++j) t.v[j] = 0;
+            for (int j = 0; j < CwVec<W>::v; ++j) t.v[j] = 0;
             if (l < P.nleaves) {
                 bool z;
                 long long off = leaf_offset(P.leaves[l], q, z);
@@ -561,7 +590,7 @@ __global__ void cwise_csr_kernel(const CwProgram P, const int* __restrict__ JA, 
                 if (z) t.v[0] = 0;
             }
 #pragma unroll
-            for (int j = 0; j < CW_VEC; ++j) lv.k[j][l] = t.v[j];
+            for (int j = 0; j < CwVec<W>::v; ++j) lv.k[j][l] = t.v[j];
         }
         Reg<W> acc;
         run_program<W, NL>(P, lv, acc);
@@ -609,7 +638,7 @@ static int build_program(const cm_leaf* leaves, int nleaves, const cm_instr* cod
 template <bool W, int NL>
 static int launch_dense(cm_ctx_t ctx, const CwProgram& P, bool flat)
 {
-    const long long chunks = (P.n + CW_VEC - 1) / CW_VEC;
+    const long long chunks = (P.n + CwVec<W>::v - 1) / CwVec<W>::v;
     long long blocks = (chunks + CW_BS - 1) / CW_BS;
     const long long cap = (long long)(ctx->cu_count > 0 ? ctx->cu_count : 256) * 16;
     if (blocks > cap) blocks = cap;  // grid-stride beyond 16 blocks per CU

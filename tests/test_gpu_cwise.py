@@ -82,6 +82,38 @@ def test_int32_binary_bitexact(gpu, orc, fn):
     assert np.array_equal(got.astype(exp.dtype), exp)
 
 
+@pytest.mark.parametrize("dtype,modes", [(np.int32, ("ADD", "SUB", "MUL", "DIV", "MOD", "AND", "OR")),
+                                          (np.float32, ("ADD", "SUB", "MUL", "DIV")),
+                                          (np.float64, ("ADD", "SUB", "MUL", "DIV"))])
+def test_compound_modes_large_bitexact(gpu, orc, dtype, modes):
+    """Every compound assignment mode (CwiseAssignmentHandler,
+    cuMat/src/CwiseOp.h:15-62) on 100003 elements -- full 16-byte groups and
+    a ragged tail -- bit-exact against the oracle, for a leaf source and for
+    an expression source."""
+    cm = gpu
+    rng = np.random.default_rng(77)
+    n = 100003
+    for mode_name in modes:
+        mode = getattr(cm.capi, mode_name)
+        if dtype == np.int32:
+            d0 = rng.integers(-2**20, 2**20, n).astype(np.int32)
+            s0 = rng.integers(-1000, 1000, n).astype(np.int32)
+            s0[s0 == 0] = 5
+        else:
+            d0 = rng.standard_normal(n).astype(dtype)
+            s0 = rng.standard_normal(n).astype(dtype)
+        dst = cm.DeviceArray.from_numpy(d0)
+        src = cm.DeviceArray.from_numpy(s0)
+        cm.assign(dst, src.expr(), mode=mode)
+        exp = orc.assign(mode, d0, s0)
+        assert np.array_equal(dst.numpy(), exp), mode_name
+        # expression source: (s + s) evaluated in the same pass
+        dst2 = cm.DeviceArray.from_numpy(d0)
+        cm.assign(dst2, src.expr() + src.expr(), mode=mode)
+        exp2 = orc.assign(mode, d0, (s0 + s0).astype(dtype))
+        assert np.array_equal(dst2.numpy(), exp2), mode_name + " (expression)"
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("fn", ["add", "sub", "mul", "div", "min", "max", "lt", "eq"])
 def test_float_binary_bitexact(gpu, orc, dtype, fn):
