@@ -288,6 +288,96 @@ __global__ __launch_bounds__(64) void col_combine_v4_kernel(const T* __restrict_
     if (q4 < Q4) *(reinterpret_cast<T4*>(out + p * Q) + q4) = r;
 }
 
+// ------------------------------------------ 16-byte block-per-chunk path
+// Sources of four consecutive elements (aligned, 4-byte types).
+template <typename T> struct Src4Plain {
+    const T* a;
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    __device__ __forceinline__ T4 operator()(long long i4) const
+    {
+        return __builtin_nontemporal_load(reinterpret_cast<const T4*>(a) + i4);
+    }
+};
+template <typename T> struct Src4Dot {
+    const T* a;
+    const T* b;
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    __device__ __forceinline__ T4 operator()(long long i4) const
+    {
+        return __builtin_nontemporal_load(reinterpret_cast<const T4*>(a) + i4) *
+               __builtin_nontemporal_load(reinterpret_cast<const T4*>(b) + i4);
+    }
+};
+
+// seg_block_kernel with 16-byte loads: segment length L4 groups of four;
+// each lane folds its groups per component, the four components are folded
+// (x op y) op (z op w), then the block reduction as before.
+template <typename Op, typename T, typename Src4>
+__global__ __launch_bounds__(RBS) void seg_block_v4_kernel(Src4 src, long long L4, int nchunk, long long chunk4,
+                                                           T* partials, T* out)
+{
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    __shared__ T lds[RBS / 64];
+    auto app = [](T4 a, T4 b) {
+        T4 r;
+        r.x = Op::apply(a.x, b.x);
+        r.y = Op::apply(a.y, b.y);
+        r.z = Op::apply(a.z, b.z);
+        r.w = Op::apply(a.w, b.w);
+        return r;
+    };
+    const long long s = blockIdx.x / nchunk;
+    const int c = blockIdx.x % nchunk;
+    const long long beg = c * chunk4;
+    const long long end = min(L4, beg + chunk4);
+    const long long base = s * L4;
+    const T id = Identity<Op, T>::v();
+    T4 acc = {id, id, id, id};
+    long long j = beg + threadIdx.x;
+    for (; j + 3 * RBS < end; j += 4 * RBS) {
+        const T4 v0 = src(base + j), v1 = src(base + j + RBS), v2 = src(base + j + 2 * RBS),
+                 v3 = src(base + j + 3 * RBS);
+        acc = app(acc, app(app(v0, v1), app(v2, v3)));
+    }
+    for (; j < end; j += RBS) acc = app(acc, src(base + j));
+    T a = Op::apply(Op::apply(acc.x, acc.y), Op::apply(acc.z, acc.w));
+    a = block_reduce<Op, RBS>(a, lds);
+    if (nchunk == 1) {
+        if (threadIdx.x == 0) out[s] = a;
+        return;
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = a;
+}
+
+// Long contiguous segments (L % 4 == 0, 16-byte aligned) through the 16-byte
+// block kernel; same chunking as run_segments' block path.
+template <typename Op, typename T, typename Src4>
+static int run_segments_v4(cm_ctx_t ctx, Src4 src, long long L, long long nseg, T* out)
+{
+    const long long cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
+    const long long L4 = L / 4;
+    long long want_blocks = cus * 8;
+    long long nchunk = (want_blocks + nseg - 1) / nseg;
+    long long maxchunks = (L4 + 1023) / 1024;  // >= 4096 elements per chunk
+    if (nchunk > maxchunks) nchunk = maxchunks;
+    if (nchunk < 1) nchunk = 1;
+    if (nchunk > 4096) nchunk = 4096;
+    long long chunk4 = (L4 + nchunk - 1) / nchunk;
+    nchunk = (L4 + chunk4 - 1) / chunk4;
+    void* part = nullptr;
+    if (nchunk > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * nseg * nchunk, &part));
+    const long long blocks = nseg * nchunk;
+    CM_REQUIRE(blocks < (1ll << 31), CM_E_DIM, "reduction too large");
+    seg_block_v4_kernel<Op, T, Src4><<<(unsigned)blocks, RBS, 0, ctx->stream>>>(src, L4, (int)nchunk, chunk4,
+                                                                                (T*)part, out);
+    CM_LAUNCH_CHECK();
+    if (nchunk > 1) {
+        seg_combine_kernel<Op, T, T><<<(unsigned)nseg, RBS, 0, ctx->stream>>>((const T*)part, (int)nchunk, out);
+        CM_LAUNCH_CHECK();
+    }
+    return CM_OK;
+}
+
 // ---------------------------------------------------------- host side
 struct Shape {
     long long I, M, O;  // inner, middle, outer(batch) sizes in memory order
@@ -325,6 +415,9 @@ static int run_segments(cm_ctx_t ctx, Src src, long long L, long long nseg, S* o
         return CM_OK;
     }
     // block per chunk: aim for >= 8 blocks per CU overall, chunks >= 4096
+    if constexpr (std::is_same<S, T>::value && sizeof(T) == 4) {
+        if (plain_vec4 && L % 4 == 0) return run_segments_v4<Op, T>(ctx, Src4Plain<T>{(const T*)plain_ptr}, L, nseg, (T*)out);
+    }
     long long want_blocks = cus * 8;
     long long nchunk = (want_blocks + nseg - 1) / nseg;
     long long maxchunks = (L + 4095) / 4096;
@@ -434,7 +527,7 @@ static int reduce_shape(cm_ctx_t ctx, const S* in, Shape sh, S* out)
         if (!rr[0]) {  // nothing reduced: copy (through a trivial segment reduce of length 1)
             return run_segments<Op, T, S>(ctx, SrcPlain<S, T>{in}, 1, rs[0], out, false, in);
         }
-        return run_segments<Op, T, S>(ctx, SrcPlain<S, T>{in}, rs[0], 1, out, false, in);
+        return run_segments<Op, T, S>(ctx, SrcPlain<S, T>{in}, rs[0], 1, out, vec_ok && rs[0] % 4 == 0, in);
     }
     if (n == 2) {
         if (rr[0]) {  // R K: contiguous segments
@@ -533,6 +626,13 @@ static int dot_entry(cm_ctx_t ctx, long long n, int batches, const T* a, int ab,
         // batch counts of the solver (<= a handful)
         const T* pa = a + (ab ? (long long)k * n : 0);
         const T* pb = b + (bb ? (long long)k * n : 0);
+        if constexpr (sizeof(T) == 4) {
+            // 16-byte loads for long aligned vectors (2^26 FP32: see run_segments_v4)
+            if (n % 4 == 0 && n >= (1 << 16) && (((uintptr_t)pa | (uintptr_t)pb) & 15) == 0) {
+                CM_TRY((run_segments_v4<OpSum, T>(ctx, Src4Dot<T>{pa, pb}, n, 1, out + k)));
+                continue;
+            }
+        }
         CM_TRY((run_segments<OpSum, T, T>(ctx, SrcDot<T, T>{pa, pb}, n, 1, out + k, false, nullptr)));
     }
     return CM_OK;
