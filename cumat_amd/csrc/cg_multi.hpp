@@ -520,61 +520,82 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
     if (threadIdx.x == 0) part1[(long long)bb * gridDim.x + blockIdx.x] = s;
 }
 
+// Vector width of K2 / K3: VW elements per load (VW = 16 / sizeof(T) when n
+// is a multiple of it and every vector is 16-byte aligned, else 1).
+template <typename T, int VW> struct CgVec { typedef T type __attribute__((ext_vector_type(VW))); };
+template <typename T> struct CgVec<T, 1> { typedef T type; };
+template <typename T, int VW>
+__device__ __forceinline__ T cg_hsum(const typename CgVec<T, VW>::type& v)
+{
+    if constexpr (VW == 1) return v;
+    else if constexpr (VW == 2) return v.x + v.y;
+    else return (v.x + v.y) + (v.z + v.w);
+}
+
 // ---------------------------------------------------------------- K2
 // The first sweep's operands are loaded before the partials of K1 are
 // reduced, so the streaming loads overlap that reduction's latency.
 // UPDATE_X = false (three-launch form): x += alpha p is left to K3, which
 // reads p anyway, so this pass reads r, Ap, M^-1 and writes r only.
-template <typename T, bool UPDATE_X>
+template <typename T, bool UPDATE_X, int VW = 1>
 __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int G1, const T* __restrict__ part1,
                                                           const T* __restrict__ invd, int istride,
                                                           const T* __restrict__ p, const T* __restrict__ tmp,
                                                           T* __restrict__ x, T* __restrict__ r,
                                                           T* __restrict__ part2, const CgState<T>* st)
 {
+    typedef typename CgVec<T, VW>::type V;
     const int bb = blockIdx.y;
     if (batch_inactive(st, bb)) return;
     const int it = it_arg >= 0 ? it_arg : st->it;  // < 0: the counter kept by K1
     __shared__ T red[2 * CG_BS / 64];
     const long long off = (long long)bb * n;
-    const T* ib = invd + (long long)bb * istride;
+    // vectors of VW elements (n is a multiple of VW)
+    const long long nv = n / VW;
+    const V* xb = reinterpret_cast<const V*>(x + off);
+    V* xo = reinterpret_cast<V*>(x + off);
+    const V* pb = reinterpret_cast<const V*>(p + off);
+    const V* tb = reinterpret_cast<const V*>(tmp + off);
+    V* rb = reinterpret_cast<V*>(r + off);
+    const V* ib = reinterpret_cast<const V*>(invd + (long long)bb * istride);
     const long long stride = (long long)gridDim.x * CG_BS;
     long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
-    T xv[CG_VPT], pv[CG_VPT], rv[CG_VPT], tv[CG_VPT], dv[CG_VPT];
+    V xv[CG_VPT], pv[CG_VPT], rv[CG_VPT], tv[CG_VPT], dv[CG_VPT];
     auto load = [&](long long base) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = base + j * stride;
-            const bool ok = i < n;
+            const bool ok = i < nv;
             if (UPDATE_X) {
-                xv[j] = ok ? x[off + i] : T(0);
-                pv[j] = ok ? p[off + i] : T(0);
+                xv[j] = ok ? xb[i] : V(0);
+                pv[j] = ok ? pb[i] : V(0);
             }
-            rv[j] = ok ? r[off + i] : T(0);
-            tv[j] = ok ? tmp[off + i] : T(0);
-            dv[j] = ok ? ib[i] : T(0);
+            rv[j] = ok ? rb[i] : V(0);
+            tv[j] = ok ? tb[i] : V(0);
+            dv[j] = ok ? ib[i] : V(0);
         }
     };
     const T rz = st->absNew[it & 1][bb];
     load(i0);
     const T pap = reduce_partials(part1 + (long long)bb * G1, G1, 1, red);
     const T alpha = rz / pap;
-    T srr = 0, srz = 0;
-    while (i0 < n) {
+    V vrr = V(0), vrz = V(0);
+    while (i0 < nv) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = i0 + j * stride;
-            if (i < n) {
-                const T rn = rv[j] - alpha * tv[j];
-                if (UPDATE_X) x[off + i] = xv[j] + alpha * pv[j];
-                r[off + i] = rn;
-                srr += rn * rn;
-                srz += rn * (dv[j] * rn);
+            if (i < nv) {
+                const V rn = rv[j] - alpha * tv[j];
+                if (UPDATE_X) xo[i] = xv[j] + alpha * pv[j];
+                rb[i] = rn;
+                vrr += rn * rn;
+                vrz += rn * (dv[j] * rn);
             }
         }
         i0 += stride * CG_VPT;
-        if (i0 < n) load(i0);
+        if (i0 < nv) load(i0);
     }
+    T srr = cg_hsum<T, VW>(vrr), srz = cg_hsum<T, VW>(vrz);
     block_sum2<CG_BS>(srr, srz, red);
     if (threadIdx.x == 0) {
         long long base = 2ll * ((long long)bb * gridDim.x + blockIdx.x);
@@ -588,7 +609,7 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int
 // XK3: K3 also performs x += alpha p (K2 then skips x and p: one vector pass
 // less per iteration; pays off for large systems, where the pass costs more
 // than the extra partial reduction K3 needs for alpha).
-template <typename T, bool XK3>
+template <typename T, bool XK3, int VW = 1>
 __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, int B, int G2, const T* __restrict__ part2,
                                                            const T* __restrict__ invd, int istride,
                                                            const T* __restrict__ r, T* __restrict__ p,
@@ -615,20 +636,25 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
     }
     const T absOld = st->absNew[it & 1][bb];
     const int max_iter = st->max_iter;
+    typedef typename CgVec<T, VW>::type V;
     const long long off = (long long)bb * n;
-    const T* ib = invd + (long long)bb * istride;
+    const long long nv = n / VW;  // vectors of VW elements (n is a multiple of VW)
+    V* pb = reinterpret_cast<V*>(p + off);
+    const V* rb = reinterpret_cast<const V*>(r + off);
+    V* xb = reinterpret_cast<V*>(x + off);
+    const V* ib = reinterpret_cast<const V*>(invd + (long long)bb * istride);
     const long long stride = (long long)gridDim.x * CG_BS;
     long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
-    T pv[CG_VPT], rv[CG_VPT], dv[CG_VPT], xv[CG_VPT];
+    V pv[CG_VPT], rv[CG_VPT], dv[CG_VPT], xv[CG_VPT];
     auto load = [&](long long base) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = base + j * stride;
-            const bool ok = i < n;
-            pv[j] = ok ? p[off + i] : T(0);
-            rv[j] = ok ? r[off + i] : T(0);
-            dv[j] = ok ? ib[i] : T(0);
-            if (XK3) xv[j] = ok ? x[off + i] : T(0);
+            const bool ok = i < nv;
+            pv[j] = ok ? pb[i] : V(0);
+            rv[j] = ok ? rb[i] : V(0);
+            dv[j] = ok ? ib[i] : V(0);
+            if (XK3) xv[j] = ok ? xb[i] : V(0);
         }
     };
     load(i0);
@@ -703,17 +729,17 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
         return;
     }
     const T beta = rz_s[bb] / absOld;
-    while (i0 < n) {
+    while (i0 < nv) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = i0 + j * stride;
-            if (i < n) {
-                if (XK3) x[off + i] = xv[j] + alpha * pv[j];
-                p[off + i] = dv[j] * rv[j] + beta * pv[j];
+            if (i < nv) {
+                if (XK3) xb[i] = xv[j] + alpha * pv[j];
+                pb[i] = dv[j] * rv[j] + beta * pv[j];
             }
         }
         i0 += stride * CG_VPT;
-        if (i0 < n) load(i0);
+        if (i0 < nv) load(i0);
     }
 }
 
@@ -786,7 +812,16 @@ static void launch_cg_iteration(cm_ctx_t ctx, int n, int B, int G1, int G2, cons
                                                                           st);
     else
         cg_spmv_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, p, t, part1, st, diag);
-    if ((long long)n * B >= CG_XK3_MIN) {
+    // 16-byte loads and stores when every vector allows them
+    constexpr int VW = 16 / (int)sizeof(T);
+    const bool wide = n % VW == 0 && ((((uintptr_t)invd | (uintptr_t)p | (uintptr_t)t | (uintptr_t)x |
+                                        (uintptr_t)r) & 15) == 0);
+    if ((long long)n * B >= CG_XK3_MIN && wide) {
+        cg_update_kernel<T, false, VW><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p,
+                                                                               t, x, r, part2, st);
+        cg_pupdate_kernel<T, true, VW><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride,
+                                                                               r, p, st, G1, part1, x);
+    } else if ((long long)n * B >= CG_XK3_MIN) {
         cg_update_kernel<T, false><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x,
                                                                            r, part2, st);
         cg_pupdate_kernel<T, true><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r, p,
