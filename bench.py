@@ -587,12 +587,47 @@ def cpu_baseline(JA, IA, V, b, n, nnz):
         t0 = time.perf_counter()
         it = L.eref_pcg_handle_f32(h, b.ctypes.data, x.ctypes.data, 0.0, sample, threads, C.byref(err))
         t = time.perf_counter() - t0
-        L.eref_csr_destroy_f32(h)
         v = it / t
         log(f"CPU Eigen CG 2048^2: {v:.2f} it/s on {threads} threads")
+        rows = {}
+        # the other measured rows on the same Eigen build, bounded samples
+        y = np.zeros(n, np.float32)
+        L.eref_spmv_f32(h, b.ctypes.data, y.ctypes.data, threads)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            L.eref_spmv_f32(h, b.ctypes.data, y.ctypes.data, threads)
+        dt = (time.perf_counter() - t0) / 10
+        rows["spmv_2048^2"] = {"ms": round(dt * 1e3, 3), "GBps": round((8 * nnz + 12 * n) / dt / 1e9, 1),
+                               "threads": threads}
+        L.eref_csr_destroy_f32(h)
+        N = 1 << 26
+        rng = np.random.default_rng(5)
+        vecs = [rng.random(N, dtype=np.float32) for _ in range(8)]
+        coef = np.linspace(-1, 1, 8).astype(np.float32)
+        out = np.empty(N, np.float32)
+        ptrs = (C.c_void_p * 8)(*[v.ctypes.data for v in vecs])
+        t0 = time.perf_counter()
+        L.eref_lincomb8_f32(N, ptrs, coef.ctypes.data, out.ctypes.data)
+        dt = time.perf_counter() - t0
+        rows["lincomb8_2^26"] = {"ms": round(dt * 1e3, 2), "GBps": round(9 * 4 * N / dt / 1e9, 1), "threads": 1}
+        t0 = time.perf_counter()
+        L.eref_dot_f32(N, vecs[0].ctypes.data, vecs[1].ctypes.data)
+        dt = time.perf_counter() - t0
+        rows["dot_2^26"] = {"ms": round(dt * 1e3, 2), "GBps": round(8 * N / dt / 1e9, 1), "threads": 1}
+        R_, C_, B_ = 1024, 1024, 64
+        for name, axis, osz in (("row_sum", 1, C_ * B_), ("col_sum", 2, R_ * B_), ("batch_sum", 4, R_ * C_)):
+            o = np.empty(osz, np.float32)
+            t0 = time.perf_counter()
+            L.eref_axis_sum_f32(axis, R_, C_, B_, vecs[2].ctypes.data, o.ctypes.data)
+            dt = time.perf_counter() - t0
+            rows[name + "_1024x1024x64"] = {"ms": round(dt * 1e3, 2), "GBps": round(4 * N / dt / 1e9, 1),
+                                            "threads": 1}
+        del vecs
+        log("CPU Eigen rows:", rows)
         return {"value": round(v, 3), "unit": "CG iterations/s", "cores": threads, "kind": "reference",
                 "sample": f"{it} Jacobi-PCG iterations of Eigen 3.3.90 (reference third-party) on the same "
-                          f"2048^2 FP32 system, Eigen::setNbThreads({threads})"}
+                          f"2048^2 FP32 system, Eigen::setNbThreads({threads})",
+                "other_rows": rows}
     except Exception as ex:  # the baseline is reported, never required
         log("cpu baseline failed:", ex)
         return None
