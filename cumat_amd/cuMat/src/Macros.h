@@ -15,6 +15,13 @@
 
 #include <cumat_amd.h>
 
+// Eigen interop (Matrix::toEigen/fromEigen, CSR from Eigen::SparseMatrix):
+// off unless the user defines CUMAT_EIGEN_SUPPORT=1 before the first
+// include (cuMat/src/Macros.h:65-71)
+#ifndef CUMAT_EIGEN_SUPPORT
+#define CUMAT_EIGEN_SUPPORT 0
+#endif
+
 #define CUMAT_NAMESPACE_BEGIN namespace cuMat {
 #define CUMAT_NAMESPACE_END }
 #define CUMAT_AMD_STR_(x) #x

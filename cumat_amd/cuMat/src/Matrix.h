@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "CwiseOp.h"
+#include "EigenInterop.h"
 
 CUMAT_NAMESPACE_BEGIN
 
@@ -449,6 +450,27 @@ public:
         m.copyFromHost(v.data());
         return m;
     }
+
+#if CUMAT_EIGEN_SUPPORT == 1
+    // ---- Eigen interop (cuMat/src/Matrix.h:860-930): synchronous copies
+    // through host memory; one batch only, Eigen has no batches
+    typedef typename eigen::EigenTypeOf<_Scalar, _Rows, _Columns, _Flags>::type EigenMatrix_t;
+    EigenMatrix_t toEigen() const
+    {
+        static_assert(_Batches == 1 || _Batches == Dynamic, "Eigen matrices have one batch");
+        CUMAT_ASSERT_ARGUMENT(batches_ == 1);
+        EigenMatrix_t m(rows_, cols_);
+        copyToHost(m.data());
+        return m;
+    }
+    static Matrix fromEigen(const EigenMatrix_t& m)
+    {
+        static_assert(_Batches == 1 || _Batches == Dynamic, "Eigen matrices have one batch");
+        Matrix out(m.rows(), m.cols(), 1);
+        out.copyFromHost(m.data());
+        return out;
+    }
+#endif
 };
 
 template <typename _Derived>

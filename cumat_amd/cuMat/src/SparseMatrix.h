@@ -16,6 +16,10 @@
 
 #include "ProductOp.h"
 
+#if CUMAT_EIGEN_SUPPORT == 1
+#include <Eigen/SparseCore>
+#endif
+
 CUMAT_NAMESPACE_BEGIN
 
 template <int _SparseFlags> struct SparsityPattern;
@@ -65,6 +69,35 @@ struct SparsityPattern<SparseFlags::CSR> {
         c.JA = JA.deepClone();
         return c;
     }
+
+#if CUMAT_EIGEN_SUPPORT == 1
+    // The pattern of an Eigen sparse matrix: the rows of a row-major Eigen
+    // matrix are CSR rows (outer index = JA, inner index = IA), as the
+    // reference's CSR benchmark builds it by hand
+    // (benchmarks/csrmv/Implementation_cuMat_CSR.cu:47-57). A column-major
+    // or uncompressed matrix is converted to a compressed row-major copy
+    // first.
+    template <typename _EigenScalar, int _EigenOptions, typename _EigenIndex>
+    static SparsityPattern fromEigen(const ::Eigen::SparseMatrix<_EigenScalar, _EigenOptions, _EigenIndex>& m)
+    {
+        if (!(_EigenOptions & ::Eigen::RowMajor) || !m.isCompressed() ||
+            !std::is_same<_EigenIndex, StorageIndex>::value) {
+            ::Eigen::SparseMatrix<_EigenScalar, ::Eigen::RowMajor, StorageIndex> c(m);
+            c.makeCompressed();
+            return fromEigen(c);
+        }
+        SparsityPattern p;
+        p.rows = m.rows();
+        p.cols = m.cols();
+        p.nnz = m.nonZeros();
+        p.JA = IndexVector(p.rows + 1);
+        p.JA.copyFromHost(reinterpret_cast<const StorageIndex*>(m.outerIndexPtr()));
+        p.IA = IndexVector(p.nnz);
+        p.IA.copyFromHost(reinterpret_cast<const StorageIndex*>(m.innerIndexPtr()));
+        p.assertValid();
+        return p;
+    }
+#endif
 };
 
 template <>
@@ -139,6 +172,36 @@ public:
         m.A_ = A_.deepClone();
         return m;
     }
+
+#if CUMAT_EIGEN_SUPPORT == 1
+    // CSR matrix of one batch from / to an Eigen sparse matrix (compressed,
+    // row-major, int indices on the way out); synchronous copies through
+    // host memory
+    typedef ::Eigen::SparseMatrix<_Scalar, ::Eigen::RowMajor, int> EigenSparseMatrix_t;
+    template <int _EigenOptions, typename _EigenIndex>
+    static SparseMatrix fromEigen(const ::Eigen::SparseMatrix<_Scalar, _EigenOptions, _EigenIndex>& m)
+    {
+        static_assert(_SparseFlags == SparseFlags::CSR, "Eigen interop is CSR only");
+        static_assert(_Batches == 1 || _Batches == Dynamic, "Eigen matrices have one batch");
+        EigenSparseMatrix_t c(m);  // row-major, compressed copy: values in CSR order
+        c.makeCompressed();
+        SparseMatrix out(Pattern::fromEigen(c), 1);
+        out.A_.copyFromHost(c.valuePtr());
+        return out;
+    }
+    EigenSparseMatrix_t toEigen() const
+    {
+        static_assert(_SparseFlags == SparseFlags::CSR, "Eigen interop is CSR only");
+        CUMAT_ASSERT_ARGUMENT(batches_ == 1);
+        const Index n = nnz();
+        EigenSparseMatrix_t m(rows(), cols());
+        m.resizeNonZeros(n);
+        sparsity_.JA.copyToHost(m.outerIndexPtr());
+        sparsity_.IA.copyToHost(m.innerIndexPtr());
+        A_.copyToHost(m.valuePtr());
+        return m;
+    }
+#endif
 
     // sparse * dense column vector (batched: matrix and/or vector batches)
     template <typename V>
