@@ -90,8 +90,20 @@ __global__ __launch_bounds__(RBS) void seg_wave_vec4_kernel(const T* __restrict_
     long long s = (long long)blockIdx.x * (RBS / 64) + (threadIdx.x >> 6);
     if (s >= nseg) return;
     const V* p = reinterpret_cast<const V*>(a) + s * L4;
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    const T4* p4 = reinterpret_cast<const T4*>(p);
     T acc = Identity<Op, T>::v();
     long long j = lane;
+    // four 16-byte loads in flight per lane (a 4 KB segment in one round),
+    // folded in the same order as the two-load loop below
+    for (; j + 192 < L4; j += 256) {
+        const T4 u0 = __builtin_nontemporal_load(p4 + j), u1 = __builtin_nontemporal_load(p4 + j + 64),
+                 u2 = __builtin_nontemporal_load(p4 + j + 128), u3 = __builtin_nontemporal_load(p4 + j + 192);
+        acc = Op::apply(acc, Op::apply(Op::apply(u0.x, u0.y), Op::apply(u0.z, u0.w)));
+        acc = Op::apply(acc, Op::apply(Op::apply(u1.x, u1.y), Op::apply(u1.z, u1.w)));
+        acc = Op::apply(acc, Op::apply(Op::apply(u2.x, u2.y), Op::apply(u2.z, u2.w)));
+        acc = Op::apply(acc, Op::apply(Op::apply(u3.x, u3.y), Op::apply(u3.z, u3.w)));
+    }
     for (; j + 64 < L4; j += 128) {
         V u = p[j], w = p[j + 64];
         acc = Op::apply(acc, Op::apply(Op::apply(u.x, u.y), Op::apply(u.z, u.w)));
@@ -263,6 +275,62 @@ __global__ __launch_bounds__(RBS) void col_kernel_v4(const T* __restrict__ in, l
         return;
     }
     if (w == 0) reinterpret_cast<T4*>(partials)[(long long)blockIdx.x * 64 + threadIdx.x] = acc;
+}
+
+// Short folds over many columns (the batch sum of 64 rows of 2^20 values):
+// each lane owns U groups of four consecutive q (RBS*4 apart) and folds all
+// K rows itself, KU rows' 16-byte loads in flight per group, so a block
+// reads runs of 4 KB of each row and no LDS combine is needed. Each output
+// folds k in ascending order (a sequential fold). Requirements as
+// col_kernel_v4.
+template <typename Op, typename T, int KU, int U>
+__global__ __launch_bounds__(RBS) void col_stream_v4_kernel(const T* __restrict__ in, long long Q, long long K,
+                                                            long long st, long long PS, T* out)
+{
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    const long long Q4 = Q / 4;
+    const long long qtiles = (Q4 + RBS * U - 1) / (RBS * U);
+    const long long p = blockIdx.x / qtiles;
+    const long long q40 = (blockIdx.x % qtiles) * (RBS * U) + threadIdx.x;
+    auto app = [](T4 a, T4 b) {
+        T4 r;
+        r.x = Op::apply(a.x, b.x);
+        r.y = Op::apply(a.y, b.y);
+        r.z = Op::apply(a.z, b.z);
+        r.w = Op::apply(a.w, b.w);
+        return r;
+    };
+    const T id = Identity<Op, T>::v();
+    T4 acc[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        acc[u] = T4{id, id, id, id};
+        ok[u] = q40 + u * RBS < Q4;
+    }
+    const T4* col = reinterpret_cast<const T4*>(in + p * PS) + q40;
+    const long long st4 = st / 4;
+    const T4 zero = {T(0), T(0), T(0), T(0)};
+    long long k = 0;
+    for (; k + KU <= K; k += KU) {
+        T4 v[KU][U];
+#pragma unroll
+        for (int j = 0; j < KU; ++j)
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                v[j][u] = ok[u] ? __builtin_nontemporal_load(col + (k + j) * st4 + u * RBS) : zero;
+#pragma unroll
+        for (int j = 0; j < KU; ++j)
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[u] = app(acc[u], v[j][u]);
+    }
+    for (; k < K; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) acc[u] = app(acc[u], __builtin_nontemporal_load(col + k * st4 + u * RBS));
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (ok[u]) *(reinterpret_cast<T4*>(out + p * Q) + q40 + u * RBS) = acc[u];
 }
 
 // Folds the ksplit partial tiles of col_kernel_v4 in split order.
@@ -450,12 +518,25 @@ static int run_columns(cm_ctx_t ctx, const S* in, long long P, long long Q, long
     if constexpr (std::is_same<S, T>::value && sizeof(T) == 4) {
         const bool aligned = Q % 4 == 0 && st % 4 == 0 && PS % 4 == 0 && (((uintptr_t)in & 15) == 0) &&
                              (((uintptr_t)out & 15) == 0);
-        // long folds only: column sum of 1024 x 1024 x 64 51.7 -> 44.9 us
-        // against 4-byte loads; for a short fold over very many columns (a
-        // batch sum of 64 batches over 2^20 elements) the 4-byte kernel's
-        // four times as many blocks win (52.5 against 54.7 us; a variant
-        // with one wavefront per 256-column tile folding all rows: 55.0 us)
+        // long folds: column sum of 1024 x 1024 x 64 51.7 -> 44.9 us against
+        // 4-byte loads
         const bool v4 = aligned && K >= 256;
+        // short folds (K < 256) over many columns: the streaming kernel with
+        // two rows x two column groups of 16-byte loads in flight per lane.
+        // Batch sum of 1024 x 1024 x 64 (same box): 40.0 us against 52.1 us
+        // for col_kernel; rows x groups 2x4 40.4, 4x4 42.9, 8x1 54.2, 1x4
+        // 54.2, 1x8 61.0 us: loads spread over fewer rows (4 MiB apart here)
+        // at a time stream faster, but one row at a time leaves too little
+        // in flight
+        constexpr int SKU = 2, SU = 2;
+        const long long stream_blocks = P * ((Q / 4 + RBS * SU - 1) / (RBS * SU));
+        if (aligned && !v4 && stream_blocks >= cus) {
+            CM_REQUIRE(stream_blocks < (1ll << 31), CM_E_DIM, "reduction too large");
+            col_stream_v4_kernel<Op, T, SKU, SU><<<(unsigned)stream_blocks, RBS, 0, ctx->stream>>>(
+                (const T*)in, Q, K, st, PS, (T*)out);
+            CM_LAUNCH_CHECK();
+            return CM_OK;
+        }
         if (v4) {
             const long long tiles = P * ((Q / 4 + 63) / 64);
             long long ksplit = 1;

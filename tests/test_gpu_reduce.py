@@ -107,6 +107,33 @@ def test_bool_all_any_random(gpu, orc):
                 assert np.array_equal(got, exp), (shape, op, axis)
 
 
+@pytest.mark.parametrize("shape,row_major,axis", [
+    ((512, 512, 7), False, 4),     # batch fold, 256 tiles of 1024 columns
+    ((1000, 300, 9), False, 4),    # ragged last tile, fold not a multiple of 8
+    ((1024, 256, 64), False, 4),   # 64 batches
+    ((9, 2 ** 18, 2), True, 1),    # row fold of a row-major matrix, two batches
+    ((40, 5000, 2), True, 1),      # too few columns for the streaming kernel
+])
+def test_short_fold_many_columns(gpu, orc, shape, row_major, axis):
+    """Short strided folds over many columns (the batch-sum shape): int32
+    exact against the oracle for sum / min / max / prod / and / or; FP32 sum
+    norm-wise within 1e-5 of the FP64 sums."""
+    cm = gpu
+    R, Cc, B = shape
+    rng = np.random.default_rng(R * 31 + Cc + B)
+    flat = rng.integers(-1000, 1001, R * Cc * B).astype(np.int32)
+    a = cm.DeviceArray.from_numpy(flat, rows=R, cols=Cc, batches=B, row_major=row_major)
+    for op in ("sum", "min", "max", "prod", "bitwise_and", "bitwise_or"):
+        got = cm.reduce(a, op, axis, out_row_major=row_major).numpy()
+        exp = orc.reduce(flat, OPS[op], axis, R, Cc, B, row_major, row_major)
+        assert np.array_equal(got, exp), op
+    f = rng.standard_normal(R * Cc * B).astype(np.float32)
+    af = cm.DeviceArray.from_numpy(f, rows=R, cols=Cc, batches=B, row_major=row_major)
+    got = cm.reduce(af, "sum", axis, out_row_major=row_major).numpy()
+    exp = orc.reduce(f.astype(np.float64), OPS["sum"], axis, R, Cc, B, row_major, row_major)
+    _check(got, exp, np.float32, "sum")
+
+
 def test_full_size_1024x1024x64(gpu):
     """The benchmark tensor at full size through size-independent properties:
     int32 sums are exact, so every axis order must give the same totals, and
