@@ -220,17 +220,18 @@ __global__ __launch_bounds__(64) void col_combine_kernel(const T* __restrict__ p
 
 // col_kernel with 16-byte loads (4-byte types, Q, st and PS multiples of 4,
 // 16-byte aligned input): lane l owns q = 4*(qtile*64 + l) + c, c < 4, so
-// a wavefront's load covers 1 KB of a row instead of 256 B. Each component
-// accumulates over k in exactly the order of col_kernel, so the results are
-// bit-identical to it.
-template <typename Op, typename T>
-__global__ __launch_bounds__(RBS) void col_kernel_v4(const T* __restrict__ in, long long P, long long Q,
-                                                     long long K, long long st, long long PS, int ksplit,
-                                                     T* partials, T* out)
+// a wavefront's load covers 1 KB of a row instead of 256 B. BS threads per
+// block (BS/64 wavefronts split k). With BS = 256 each component accumulates
+// over k in exactly the order of col_kernel, so the results are
+// bit-identical to it; BS = 1024 folds in another fixed order.
+template <typename Op, typename T, int BS = RBS>
+__global__ __launch_bounds__(BS) void col_kernel_v4(const T* __restrict__ in, long long P, long long Q,
+                                                    long long K, long long st, long long PS, int ksplit,
+                                                    T* partials, T* out)
 {
     typedef T T4 __attribute__((ext_vector_type(4)));
-    __shared__ T4 lds[RBS];
-    constexpr int NW = RBS / 64;
+    __shared__ T4 lds[BS];
+    constexpr int NW = BS / 64;
     const long long Q4 = Q / 4;
     const long long qtiles = (Q4 + 63) / 64;
     const long long tile = blockIdx.x / ksplit;
@@ -539,6 +540,16 @@ static int run_columns(cm_ctx_t ctx, const S* in, long long P, long long Q, long
         }
         if (v4) {
             const long long tiles = P * ((Q / 4 + 63) / 64);
+            // at least one column tile per CU: one 1024-thread block per tile
+            // folds all K rows (16 wavefronts split them, combined in LDS),
+            // so no partials and no second launch
+            if (tiles >= cus) {
+                CM_REQUIRE(tiles < (1ll << 31), CM_E_DIM, "reduction too large");
+                col_kernel_v4<Op, T, 1024><<<(unsigned)tiles, 1024, 0, ctx->stream>>>(
+                    (const T*)in, P, Q, K, st, PS, 1, nullptr, (T*)out);
+                CM_LAUNCH_CHECK();
+                return CM_OK;
+            }
             long long ksplit = 1;
             if (tiles < cus * 4) {
                 ksplit = (cus * 8 + tiles - 1) / tiles;

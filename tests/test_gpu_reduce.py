@@ -113,9 +113,12 @@ def test_bool_all_any_random(gpu, orc):
     ((1024, 256, 64), False, 4),   # 64 batches
     ((9, 2 ** 18, 2), True, 1),    # row fold of a row-major matrix, two batches
     ((40, 5000, 2), True, 1),      # too few columns for the streaming kernel
+    ((1000, 300, 64), False, 2),   # long fold, one 1024-thread block per column tile, ragged tile
+    ((400, 1000, 64), True, 1),    # the same for a row-major matrix
 ])
 def test_short_fold_many_columns(gpu, orc, shape, row_major, axis):
-    """Short strided folds over many columns (the batch-sum shape): int32
+    """Strided folds over many columns (the batch-sum shape, and long folds
+    with a column tile per CU): int32
     exact against the oracle for sum / min / max / prod / and / or; FP32 sum
     norm-wise within 1e-5 of the FP64 sums."""
     cm = gpu
