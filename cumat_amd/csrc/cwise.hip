@@ -459,18 +459,34 @@ __device__ __forceinline__ long long leaf_offset(const cm_leaf& L, const Coords&
 
 // FLAT: every leaf is direct or scalar and the destination is direct, so no
 // index decomposition is needed anywhere.
+// Element of slot k of a thread whose first element is e0. 4-byte programs:
+// a thread's 8 elements are two groups of 4 that lie 256 elements apart, so
+// that one 16-byte load of the wavefront covers 1 KB contiguously; 8-byte
+// programs keep 4 consecutive elements.
+constexpr bool CW_SPREAD = true;
+template <bool W>
+__device__ __forceinline__ long long cw_elem(long long e0, int k)
+{
+    return (CW_SPREAD && !W) ? e0 + (k >> 2) * 256 + (k & 3) : e0 + k;
+}
+template <bool W>
+__device__ __forceinline__ long long cw_first(long long tid)
+{
+    return (CW_SPREAD && !W) ? (tid >> 6) * (64 * CwVec<W>::v) + (tid & 63) * 4 : tid * CwVec<W>::v;
+}
+
 template <bool W, int NL, bool FLAT>
 __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
 {
     const long long n = P.n;
     const long long stride = (long long)gridDim.x * CW_BS * CwVec<W>::v;
-    for (long long e0 = ((long long)blockIdx.x * CW_BS + threadIdx.x) * CwVec<W>::v; e0 < n; e0 += stride) {
+    for (long long e0 = cw_first<W>((long long)blockIdx.x * CW_BS + threadIdx.x); e0 < n; e0 += stride) {
         Bank<W, NL> lv;
-        const bool full = e0 + CwVec<W>::v <= n;
+        const bool full = cw_elem<W>(e0, CwVec<W>::v - 1) < n;
         Coords q[CwVec<W>::v];
         if (!FLAT) {
 #pragma unroll
-            for (int k = 0; k < CwVec<W>::v; ++k) q[k] = decompose(P, min(e0 + k, n - 1));
+            for (int k = 0; k < CwVec<W>::v; ++k) q[k] = decompose(P, min(cw_elem<W>(e0, k), n - 1));
         }
         // ---- fetch every leaf for the 4 elements before computing
 #pragma unroll
@@ -494,13 +510,13 @@ __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
 #pragma unroll
                     for (int q = 0; q < CwVec<W>::v / 4; ++q) {
                         const u4v u = __builtin_nontemporal_load(
-                            reinterpret_cast<const u4v*>((const uint32_t*)L.ptr + e0 + 4 * q));
+                            reinterpret_cast<const u4v*>((const uint32_t*)L.ptr + cw_elem<W>(e0, 4 * q)));
                         t.v[4 * q] = u.x; t.v[4 * q + 1] = u.y; t.v[4 * q + 2] = u.z; t.v[4 * q + 3] = u.w;
                     }
                 } else {
 #pragma unroll
                     for (int k = 0; k < CwVec<W>::v; ++k) {
-                        long long e = min(e0 + k, n - 1);
+                        long long e = min(cw_elem<W>(e0, k), n - 1);
                         long long off = (FLAT || P.dst_direct) ? e
                                          : (q[k].r * P.dst_sr + q[k].c * P.dst_sc + q[k].b * P.dst_sb);
                         load_raw<W>(L, off, t, k);
@@ -531,12 +547,12 @@ __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
                     u4v u;
                     u.x = (uint32_t)acc.v[4 * q]; u.y = (uint32_t)acc.v[4 * q + 1];
                     u.z = (uint32_t)acc.v[4 * q + 2]; u.w = (uint32_t)acc.v[4 * q + 3];
-                    __builtin_nontemporal_store(u, reinterpret_cast<u4v*>((uint32_t*)P.dst + e0 + 4 * q));
+                    __builtin_nontemporal_store(u, reinterpret_cast<u4v*>((uint32_t*)P.dst + cw_elem<W>(e0, 4 * q)));
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < CwVec<W>::v / 4; ++q) {
-                    u4v* d = reinterpret_cast<u4v*>((uint32_t*)P.dst + e0 + 4 * q);
+                    u4v* d = reinterpret_cast<u4v*>((uint32_t*)P.dst + cw_elem<W>(e0, 4 * q));
                     const u4v old = *d;
                     u4v u;
 #pragma unroll
@@ -551,8 +567,9 @@ __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
         } else {
 #pragma unroll
             for (int k = 0; k < CwVec<W>::v; ++k) {
-                if (e0 + k < n) {
-                    long long off = (FLAT || P.dst_direct) ? (e0 + k)
+                const long long e = cw_elem<W>(e0, k);
+                if (e < n) {
+                    long long off = (FLAT || P.dst_direct) ? e
                                      : (q[k].r * P.dst_sr + q[k].c * P.dst_sc + q[k].b * P.dst_sb);
                     store_elem<W>(P, off, acc, k);
                 }
