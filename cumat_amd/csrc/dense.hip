@@ -9,13 +9,21 @@ static int launch_dense_gemv(cm_ctx_t ctx, int rows, int cols, int batches, cons
                              int a_row_major, const void* x, int x_batched, void* y, int mode)
 {
     const long long bstride = a_batched ? (long long)rows * cols : 0;
+    // column-major with rows a multiple of the vector width and a 16-byte
+    // aligned base: 16-byte loads (16384^2 FP32: see dense_gemv_v_kernel)
+    constexpr int VW = DenseVec<T>::w;
+    const bool vec = !a_row_major && rows % VW == 0 && (((uintptr_t)A & 15) == 0);
+    const int tile = vec ? 64 * VW : DN_TILE;
     int nsplit = 1, csz = cols;
-    dense_split(rows, cols, nsplit, csz);
+    dense_split(rows, cols, nsplit, csz, tile);
     // per-range partial vectors in the context's stream-ordered scratch
     void* part = nullptr;
     if (nsplit > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * (size_t)nsplit * rows * batches, &part));
-    dim3 grid(dense_grid(rows, nsplit), batches);
-    if (a_row_major)
+    dim3 grid(dense_grid(rows, nsplit, tile), batches);
+    if (vec)
+        dense_gemv_v_kernel<T><<<grid, DN_BS, 0, ctx->stream>>>(rows, cols, (const T*)A, bstride, (const T*)x,
+                                                                x_batched, (T*)y, mode, nsplit, csz, (T*)part);
+    else if (a_row_major)
         dense_gemv_kernel<T, true><<<grid, DN_BS, 0, ctx->stream>>>(rows, cols, (const T*)A, bstride, (const T*)x,
                                                                     x_batched, (T*)y, mode, nsplit, csz, (T*)part);
     else

@@ -54,6 +54,38 @@ def test_matvec_int32_exact_and_modes(gpu, row_major, shape):
     np.testing.assert_array_equal(out.numpy(), y0)
 
 
+@pytest.mark.parametrize("shape", [(4, 9), (260, 3000), (1024, 700), (2, 513)])
+def test_matvec_16byte_column_major(gpu, shape):
+    """Column-major A with rows a multiple of the vector width (4 for 4-byte
+    types, 2 for FP64) takes the 16-byte path: int32 exact (also += / -= and
+    a batched A), FP32 / FP64 within the product tolerances; ragged last row
+    tile, one and several column ranges."""
+    cm = gpu
+    rng = np.random.default_rng(shape[0] + 7 * shape[1])
+    rows, cols = shape
+    B = 2
+    a = rng.integers(-30, 30, (B, rows, cols)).astype(np.int32)
+    x = rng.integers(-30, 30, (B, cols, 1)).astype(np.int32)
+    A = cm.DeviceArray.from_numpy(a)
+    X = cm.DeviceArray.from_numpy(x)
+    y = cm.dense.matvec(A, X).numpy().reshape(B, rows)
+    exp = np.einsum("brc,bc->br", a.astype(np.int64), x[:, :, 0])
+    np.testing.assert_array_equal(y, exp)
+    y0 = rng.integers(-9, 9, (B, rows, 1)).astype(np.int32)
+    out = cm.DeviceArray.from_numpy(y0)
+    cm.dense.matvec(A, X, out=out, mode=cm.capi.ADD)
+    np.testing.assert_array_equal(out.numpy().reshape(B, rows), y0[:, :, 0] + exp)
+    cm.dense.matvec(A, X, out=out, mode=cm.capi.SUB)
+    np.testing.assert_array_equal(out.numpy().reshape(B, rows), y0[:, :, 0])
+    for dt, tol in ((np.float64, 1e-12), (np.float32, 1e-5)):
+        af = rng.uniform(-1, 1, (rows, cols)).astype(dt)
+        xf = rng.uniform(-1, 1, cols).astype(dt)
+        yf = cm.dense.matvec(cm.DeviceArray.from_numpy(af), cm.DeviceArray.from_numpy(xf)).numpy()
+        ref = af.astype(np.float64) @ xf.astype(np.float64)
+        scale = np.abs(af).astype(np.float64) @ np.abs(xf).astype(np.float64) + 1e-300
+        assert np.all(np.abs(yf - ref) <= tol * scale), (dt, shape)
+
+
 def test_matvec_batched_and_broadcast(gpu):
     """Batched A with batched x, batch-1 A broadcast over batched x, batched A
     over a batch-1 x (cuMat's batched products)."""
