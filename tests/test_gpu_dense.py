@@ -195,6 +195,27 @@ def test_cg_dense_solve_batched(gpu):
     np.testing.assert_allclose(x, xt[:, :, 0], rtol=1e-4)
 
 
+@pytest.mark.parametrize("n", [256, 301])
+def test_cg_dense_float32_both_layouts(gpu, n):
+    """FP32 dense CG with row-major A (16-byte row loads when n % 4 == 0)
+    and column-major A: both converge to tolerance, to the FP64 solution
+    within 1e-4, and within a few iterations of each other."""
+    cm = gpu
+    rng = np.random.default_rng(n)
+    m = rng.standard_normal((n, n))
+    a = (m @ m.T / n + 0.5 * np.eye(n)).astype(np.float32)
+    b = rng.uniform(-1, 1, n).astype(np.float32)
+    x64 = np.linalg.solve(a.astype(np.float64), b.astype(np.float64))
+    its = []
+    for rm in (True, False):
+        cg = cm.ConjugateGradient(_dense(cm, a, rm)).setTolerance(1e-5).setMaxIterations(2000)
+        x = cg.solve(cm.DeviceArray.from_numpy(b)).numpy()
+        assert cg.error() <= 1e-5
+        assert np.linalg.norm(x - x64) / np.linalg.norm(x64) <= 1e-4
+        its.append(cg.iterations())
+    assert abs(its[0] - its[1]) <= 4, its
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_cg_dense_matches_oracle_on_csr_form(gpu, orc, dt):
     """Dense SPD matrix: the dense CG against the oracle's CG on the same
