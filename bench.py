@@ -293,6 +293,58 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
                       "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(label, out[label])
         del mats
+    # ---- ELLPACK SpMV and component-wise evaluation into CSR values on the
+    # 2048^2 Poisson matrix (two rotating copies, more than the last-level
+    # cache together)
+    from cumat_amd.problems import poisson2d
+    JA, IA, V = poisson2d(2048)
+    n = 2048 * 2048
+    W = int(np.max(np.diff(JA)))
+    lens = np.diff(JA)
+    rows = np.repeat(np.arange(n), lens)
+    pos = np.arange(IA.size) - np.repeat(JA[:-1], lens)
+    ind = np.full((n, W), -1, np.int32)
+    val = np.zeros((n, W), np.float32)
+    ind[rows, pos] = IA
+    val[rows, pos] = V
+    ells = [cm.ELLMatrix(n, n, ind, val, ctx=ctx) for _ in range(2)]
+    del ind, val
+    xs = [cm.DeviceArray.from_numpy(rng.uniform(-1, 1, n).astype(np.float32), ctx=ctx) for _ in range(2)]
+    ys = [cm.DeviceArray(n, dtype=np.float32, ctx=ctx) for _ in range(2)]
+    for k in range(4):
+        ells[k % 2].matvec(xs[k % 2], out=ys[k % 2], ctx=ctx)
+    ctx.sync()
+    reps = 20
+    timer.start(stream)
+    for k in range(reps):
+        ells[k % 2].matvec(xs[k % 2], out=ys[k % 2], ctx=ctx)
+    ms = timer.stop_ms(stream) / reps
+    by = 8 * n * W + 8 * n
+    out["ell_spmv_2048^2"] = {"us": round(ms * 1000, 1), "GBps": round(by / (ms * 1e-3) / 1e9, 1),
+                              "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "width": W}
+    log("ell_spmv_2048^2", out["ell_spmv_2048^2"])
+    del ells
+    sms = [cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx) for _ in range(2)]
+    rv = cm.DeviceArray.from_numpy(rng.uniform(-1, 1, n).astype(np.float32), rows=1, cols=n, batches=1, ctx=ctx)
+    nnz = IA.size
+    csr_cw = {}
+    # bytes: values read + written; the column-dependent program also reads
+    # IA and gathers the row vector (counted once)
+    for label, mk, by in (("A = A*2+1", lambda S: S.direct() * 2.0 + 1.0, 8 * nnz),
+                          ("A = A*r(col)", lambda S: S.direct() * rv.expr(), 12 * nnz + 4 * n)):
+        exprs = [mk(S) for S in sms]
+        for k in range(2):
+            sms[k].assign(exprs[k], ctx=ctx)
+        ctx.sync()
+        timer.start(stream)
+        for k in range(reps):
+            sms[k % 2].assign(exprs[k % 2], ctx=ctx)
+        ms = timer.stop_ms(stream) / reps
+        csr_cw[label] = {"us": round(ms * 1000, 1), "GBps": round(by / (ms * 1e-3) / 1e9, 1),
+                         "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out["csr_cwise_2048^2"] = csr_cw
+    log("csr_cwise_2048^2", csr_cw)
+    del sms, xs, ys
     # ---- plain CG on 128^2 (the reference's CPU-runnable configuration)
     from cumat_amd.problems import poisson2d
     JA, IA, V = poisson2d(128)

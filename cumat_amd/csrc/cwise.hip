@@ -579,18 +579,14 @@ __global__ __launch_bounds__(CW_BS) void cwise_kernel(const CwProgram P)
 }
 
 // ------------------------------------------------ CSR destination
-// One thread per (row, batch); the program is evaluated at (row, IA[k], b)
-// for each stored entry, one entry at a time (lanes 1..3 of the registers
-// idle). dst layout: vals[k + nnz*b].
+// The program evaluated at stored entry k of batch q.b, coordinates
+// q = (row, column, batch), one entry at a time (lanes 1..3 of the
+// registers idle). dst layout: vals[k + nnz*b].
 template <bool W, int NL>
-__global__ void cwise_csr_kernel(const CwProgram P, const int* __restrict__ JA, const int* __restrict__ IA,
-                                 long long nnz)
+__device__ __forceinline__ void csr_entry_eval(const CwProgram& P, const Coords& q, long long k, long long nnz)
 {
-    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long b = blockIdx.y;
-    if (row >= P.rows) return;
-    for (int k = JA[row]; k < JA[row + 1]; ++k) {
-        Coords q{row, (long long)IA[k], b};
+    const long long b = q.b;
+    {
         Bank<W, NL> lv;
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
@@ -600,9 +596,10 @@ __global__ void cwise_csr_kernel(const CwProgram P, const int* __restrict__ JA, 
             if (l < P.nleaves) {
                 bool z;
                 long long off = leaf_offset(P.leaves[l], q, z);
-                // a leaf marked sr == -1 is the sparse value array itself
-                // (SparseMatrix::direct())
-                if (P.leaves[l].sr == -1) off = k + nnz * b, z = false;
+                // a leaf marked sr == -1 is a value array of this pattern
+                // (SparseMatrix::direct()): entry k of batch b, batch stride
+                // nnz, or 0 when one batch is broadcast over several
+                if (P.leaves[l].sr == -1) off = k + P.leaves[l].sb * b, z = false;
                 load_raw<W>(P.leaves[l], z ? 0 : off, t, 0);
                 if (z) t.v[0] = 0;
             }
@@ -613,6 +610,32 @@ __global__ void cwise_csr_kernel(const CwProgram P, const int* __restrict__ JA, 
         run_program<W, NL>(P, lv, acc);
         store_elem<W>(P, k + nnz * b, acc, 0);
     }
+}
+
+// Thread per row, walking the row's stored entries (the loop of
+// cuMat/src/SparseEvaluation.h:14-34): any program, including leaves that
+// depend on the row.
+template <bool W, int NL>
+__global__ void cwise_csr_kernel(const CwProgram P, const int* __restrict__ JA, const int* __restrict__ IA,
+                                 long long nnz)
+{
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long b = blockIdx.y;
+    if (row >= P.rows) return;
+    for (int k = JA[row]; k < JA[row + 1]; ++k) csr_entry_eval<W, NL>(P, Coords{row, (long long)IA[k], b}, k, nnz);
+}
+
+// Thread per stored entry (coalesced over the value array) for programs
+// whose leaves need no row coordinate: every leaf is the value array itself
+// or a plain strided leaf with row stride 0; the column is read from IA only
+// when some leaf depends on it.
+template <bool W, int NL>
+__global__ void cwise_csr_entry_kernel(const CwProgram P, const int* __restrict__ IA, long long nnz, int need_col)
+{
+    const long long b = blockIdx.y;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += stride)
+        csr_entry_eval<W, NL>(P, Coords{0, need_col ? (long long)IA[k] : 0, b}, k, nnz);
 }
 
 // --------------------------------------------------------- host side
@@ -680,6 +703,24 @@ static int launch_dense_w(cm_ctx_t ctx, const CwProgram& P, bool flat)
 template <bool W>
 static int launch_csr_w(cm_ctx_t ctx, const CwProgram& P, const int* JA, const int* IA, long long nnz)
 {
+    bool row_free = true, need_col = false;
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        if (L.sr == -1) continue;  // the sparse value array
+        if (L.kind != CM_LEAF_STRIDED || L.sr != 0) row_free = false;
+        if (L.sc != 0) need_col = true;
+    }
+    if (row_free) {
+        const long long cap = (long long)(ctx->cu_count > 0 ? ctx->cu_count : 256) * 64;
+        const long long blocks = std::max<long long>(1, std::min<long long>(div_up(nnz, 256), cap));
+        dim3 g((unsigned)blocks, (unsigned)P.batches);
+        if (P.nleaves <= 4)
+            cwise_csr_entry_kernel<W, 4><<<g, 256, 0, ctx->stream>>>(P, IA, nnz, need_col ? 1 : 0);
+        else
+            cwise_csr_entry_kernel<W, CM_CWISE_MAX_LEAVES><<<g, 256, 0, ctx->stream>>>(P, IA, nnz, need_col ? 1 : 0);
+        CM_LAUNCH_CHECK();
+        return CM_OK;
+    }
     dim3 grid(div_up(P.rows, 128), (unsigned)P.batches);
     if (P.nleaves <= 4) cwise_csr_kernel<W, 4><<<grid, 128, 0, ctx->stream>>>(P, JA, IA, nnz);
     else cwise_csr_kernel<W, CM_CWISE_MAX_LEAVES><<<grid, 128, 0, ctx->stream>>>(P, JA, IA, nnz);
@@ -792,6 +833,12 @@ static bool match_lincomb(const CwProgram& P, LincombArgs& A, int& k)
 
 }  // namespace cm
 
+namespace cm {
+static int cwise_dense(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_instr* code, int ncode, void* dst,
+                       int dst_dtype, int rows, int cols, int batches, int64_t dst_sr, int64_t dst_sc, int64_t dst_sb,
+                       int mode);
+}  // namespace cm
+
 extern "C" {
 
 int cm_cwise_eval(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_instr* code, int ncode, void* dst,
@@ -806,6 +853,20 @@ int cm_cwise_eval(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_ins
                "cm_cwise_eval: dst dtype %d", dst_dtype);
     cm::count(CM_CNT_EVAL_CWISE);
     cm::count(CM_CNT_EVAL_ANY);
+    return cm::cwise_dense(ctx, leaves, nleaves, code, ncode, dst, dst_dtype, rows, cols, batches, dst_sr, dst_sc,
+                           dst_sb, mode);
+    CM_API_END
+}
+
+}  // extern "C"
+
+namespace cm {
+
+// the dense evaluation behind cm_cwise_eval (arguments checked there)
+static int cwise_dense(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_instr* code, int ncode, void* dst,
+                       int dst_dtype, int rows, int cols, int batches, int64_t dst_sr, int64_t dst_sc, int64_t dst_sb,
+                       int mode)
+{
     const long long n = (long long)rows * cols * batches;
     if (n == 0) return CM_OK;
     CM_REQUIRE(dst, CM_E_ARG, "cm_cwise_eval: dst is NULL");
@@ -846,8 +907,11 @@ int cm_cwise_eval(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_ins
     }
     if (wide) return cm::launch_dense_w<true>(ctx, P, flat);
     return cm::launch_dense_w<false>(ctx, P, flat);
-    CM_API_END
 }
+
+}  // namespace cm
+
+extern "C" {
 
 int cm_lincomb_f32(cm_ctx_t ctx, int k, const float* const* vecs, const float* coeffs, float* out, int64_t n)
 {
@@ -880,6 +944,32 @@ int cm_csr_cwise_eval(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm
     cm::count(CM_CNT_EVAL_CWISE_SPARSE);
     cm::count(CM_CNT_EVAL_ANY);
     if (rows == 0 || nnz == 0) return CM_OK;
+    // a program whose leaves are the value arrays and scalars only needs no
+    // coordinate at all: it is a dense program over the nnz x 1 x batches
+    // value array (value leaves become leaves with row stride 1 and their
+    // own batch stride, the same offsets as below), run by the streaming
+    // dense kernels
+    {
+        bool values_only = nleaves >= 0 && nleaves <= CM_CWISE_MAX_LEAVES && leaves != nullptr;
+        for (int l = 0; values_only && l < nleaves; ++l) {
+            const cm_leaf& L = leaves[l];
+            const bool value_leaf = L.sr == -1;
+            const bool scalar = L.kind == CM_LEAF_STRIDED && L.sr == 0 && L.sc == 0 && L.sb == 0;
+            values_only = value_leaf || scalar;
+        }
+        if (values_only && val_dtype != CM_BF16 && cm::dtype_size(val_dtype) > 0) {
+            cm_leaf dl[CM_CWISE_MAX_LEAVES];
+            for (int l = 0; l < nleaves; ++l) {
+                dl[l] = leaves[l];
+                if (dl[l].sr == -1) {
+                    dl[l].sr = 1;
+                    dl[l].sc = 0;
+                }
+            }
+            return cm::cwise_dense(ctx, dl, nleaves, code, ncode, vals, val_dtype, nnz, 1, batches, 1, nnz, nnz,
+                                   mode);
+        }
+    }
     cm::CwProgram P;
     bool wide = false;
     CM_TRY(cm::build_program(leaves, nleaves, code, ncode, P, wide));

@@ -61,8 +61,28 @@ __global__ void ell_spmv_kernel(int rows, int cols, int npr, const int* __restri
     const T* v = vals + (vals_batched ? (long long)b * rows * npr : 0);
     const T* xb = x + (x_batched ? (long long)b * cols : 0);
     T acc = T(0);
-    for (int k = 0; k < npr; ++k) {
-        int c = idx[r + (long long)rows * k];   // column-major: coalesced across rows
+    // column-major: the k-th entries of consecutive rows are contiguous.
+    // Four entries' index and value loads are issued together (streamed
+    // once: non-temporal), then gathered and added in k order, skipping
+    // padding (index < 0)
+    int k = 0;
+    for (; k + 4 <= npr; k += 4) {
+        int c[4];
+        T a[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j] = __builtin_nontemporal_load(idx + r + (long long)rows * (k + j));
+            a[j] = __builtin_nontemporal_load(v + r + (long long)rows * (k + j));
+        }
+        T g[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) g[j] = c[j] >= 0 ? xb[c[j]] : T(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (c[j] >= 0) acc += a[j] * g[j];
+    }
+    for (; k < npr; ++k) {
+        const int c = idx[r + (long long)rows * k];
         if (c < 0) continue;
         acc += v[r + (long long)rows * k] * xb[c];
     }

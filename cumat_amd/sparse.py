@@ -142,6 +142,10 @@ class ELLMatrix:
         self.npr = ind.shape[1]
         if ind.shape[0] != rows or self.npr > cols:
             raise capi.CumatArgError(capi.CM_E_DIM, "ELLPACK pattern has the wrong size")
+        # the kernel gathers x at every index >= 0: reject indices outside
+        # [-1, cols) here rather than let one reach the device
+        if ind.size and (int(ind.min()) < -1 or int(ind.max()) >= self.cols):
+            raise capi.CumatArgError(capi.CM_E_ARG, "ELLPACK index outside [-1, cols)")
         self.indices = DeviceArray.from_numpy(np.asfortranarray(ind).reshape(-1, order="F"), ctx=ctx)
         v = np.asarray(values_rowsxk)
         self.values = DeviceArray.from_numpy(np.asfortranarray(v).reshape(-1, order="F"), ctx=ctx)

@@ -201,3 +201,71 @@ def test_spmv_dimension_errors(gpu):
     A, g = _csr4x5(cm)
     with pytest.raises(cm.CumatArgError):
         A.matvec(cm.DeviceArray.from_numpy(np.ones(4, np.float32)))
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.float32])
+def test_csr_cwise_eval_both_kernels(gpu, dtype):
+    """Component-wise programs evaluated at the stored entries of a 2D
+    Poisson pattern (cuMat/src/SparseEvaluation.h:14-34): programs whose
+    leaves need no row coordinate (values and scalars; a row vector that
+    depends on the column only) run one thread per entry, a column vector
+    (depends on the row) runs one thread per row. int32 exact against numpy,
+    FP32 to 1 ulp-level tolerance; plus a compound mode."""
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    g = 300
+    JA, IA, V = poisson2d(g)
+    n = g * g
+    rng = np.random.default_rng(41)
+    vals = (rng.integers(-50, 50, IA.size) if dtype == np.int32 else rng.uniform(-1, 1, IA.size)).astype(dtype)
+    A = cm.SparseMatrix.from_csr(n, n, JA, IA, vals)
+    rows = np.repeat(np.arange(n), np.diff(JA))
+    rv = (rng.integers(-9, 9, n) if dtype == np.int32 else rng.uniform(-1, 1, n)).astype(dtype)
+    cv = (rng.integers(-9, 9, n) if dtype == np.int32 else rng.uniform(-1, 1, n)).astype(dtype)
+    R = cm.DeviceArray.from_numpy(rv, rows=1, cols=n, batches=1)
+    Cv = cm.DeviceArray.from_numpy(cv, rows=n, cols=1, batches=1)
+    two, one = (2, 1) if dtype == np.int32 else (2.0, 1.0)
+
+    def check(got, exp):
+        if dtype == np.int32:
+            np.testing.assert_array_equal(got, exp)
+        else:
+            np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-6)
+
+    A.assign(A.direct() * two + one)                  # values and scalars only
+    v1 = (vals * dtype(two) + dtype(one)).astype(dtype)
+    check(A.values.numpy(), v1)
+    A.assign(A.direct() * R.expr())                   # depends on the column
+    v2 = (v1 * rv[IA]).astype(dtype)
+    check(A.values.numpy(), v2)
+    A.assign(A.direct() * Cv.expr())                  # depends on the row
+    v3 = (v2 * cv[rows]).astype(dtype)
+    check(A.values.numpy(), v3)
+    A.assign(R.expr() - A.direct(), mode=cm.capi.ADD)  # compound, column leaf
+    v4 = (v3 + (rv[IA] - v3)).astype(dtype)
+    check(A.values.numpy(), v4)
+
+
+def test_csr_cwise_eval_broadcast_single_batch_values(gpu):
+    """A one-batch sparse matrix used in a program assigned to a two-batch
+    matrix of the same pattern is broadcast over the batches (its value leaf
+    has batch stride 0): values-only program (dense route) and a
+    row-dependent program (row kernel), int32 exact."""
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    g = 40
+    JA, IA, V = poisson2d(g)
+    n = g * g
+    rng = np.random.default_rng(43)
+    v1 = rng.integers(-50, 50, IA.size).astype(np.int32)
+    pat = cm.SparsityPattern(n, n, JA, IA)
+    S1 = cm.SparseMatrix(pat, v1, batches=1)
+    S2 = cm.SparseMatrix(pat, np.zeros(2 * IA.size, np.int32), batches=2)
+    S2.assign(S1.direct() * 3)
+    np.testing.assert_array_equal(S2.values.numpy().reshape(2, -1), np.stack([v1 * 3, v1 * 3]))
+    cv = rng.integers(-9, 9, n).astype(np.int32)
+    Cv = cm.DeviceArray.from_numpy(cv, rows=n, cols=1, batches=1)
+    S2.assign(S1.direct() * Cv.expr(), mode=cm.capi.ADD)
+    rows = np.repeat(np.arange(n), np.diff(JA))
+    exp = v1 * 3 + v1 * cv[rows]
+    np.testing.assert_array_equal(S2.values.numpy().reshape(2, -1), np.stack([exp, exp]))

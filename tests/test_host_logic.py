@@ -159,3 +159,12 @@ def test_cg_rejects_non_square_and_batched_dense_matrices():
         ConjugateGradient(_arr(4, 5, ptr=0x1000))
     with pytest.raises(capi.CumatArgError):
         ConjugateGradient(_arr(4, 4, 2, ptr=0x1000))
+
+
+def test_ell_pattern_indices_checked_before_upload():
+    """An ELLPACK index outside [-1, cols) is rejected on the host (the
+    kernel gathers x at every index >= 0)."""
+    from cumat_amd.sparse import ELLMatrix
+    for bad in ([[0, 2], [1, -1]], [[0, -2], [1, 0]]):
+        with pytest.raises(capi.CumatArgError):
+            ELLMatrix(2, 2, bad, [[1.0, 1.0], [1.0, 1.0]])
