@@ -76,7 +76,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
-              "described in DESIGN.md and not run.", flush=True)
+              "a design, not run.", flush=True)
         sys.exit(2)
 
     import cumat_amd as cm
