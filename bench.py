@@ -11,9 +11,13 @@ successive launches cannot be served from the 256 MB last-level cache.
 Algorithmic bytes per launch = 8*nnz + 12*n.
 
 Also reported (bounded, after the timed region): the linear combination of 8
-vectors of 2^26 floats, dot of 2^26, row/column/batch sums of 1024x1024x64,
-the 128^2 plain CG solve (iters/s, time to tol 1e-6), and the CPU baseline:
-the reference's vendored Eigen (oracle/_ref) on the host cores.
+vectors of 2^26 floats, dot of 2^26, general cwise programs over 2^26,
+row/column/batch sums of 1024x1024x64, the 16384^2 dense product (both
+layouts), ELLPACK SpMV and cwise evaluation into CSR values on the 2048^2
+matrix, the 128^2 plain CG solve (iters/s, time to tol 1e-6) and small
+systems, the batched, mixed-precision and adjoint solvers, the reference's
+published benchmark workloads, and the CPU baseline: the reference's
+vendored Eigen (oracle/_ref) on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras]
 """
