@@ -700,6 +700,17 @@ static int launch_dense_w(cm_ctx_t ctx, const CwProgram& P, bool flat)
     return launch_dense<W, CM_CWISE_MAX_LEAVES>(ctx, P, flat);
 }
 
+// dst[k] = src[IA[k] * sc]: a leaf that depends on the column, laid out
+// along the stored entries (1-, 4- or 8-byte elements)
+template <typename E>
+__global__ void csr_gather_kernel(const E* __restrict__ src, long long sc, const int* __restrict__ IA, long long nnz,
+                                  E* __restrict__ dst)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += stride)
+        dst[k] = src[(long long)IA[k] * sc];
+}
+
 template <bool W>
 static int launch_csr_w(cm_ctx_t ctx, const CwProgram& P, const int* JA, const int* IA, long long nnz)
 {
@@ -944,26 +955,57 @@ int cm_csr_cwise_eval(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm
     cm::count(CM_CNT_EVAL_CWISE_SPARSE);
     cm::count(CM_CNT_EVAL_ANY);
     if (rows == 0 || nnz == 0) return CM_OK;
-    // a program whose leaves are the value arrays and scalars only needs no
-    // coordinate at all: it is a dense program over the nnz x 1 x batches
-    // value array (value leaves become leaves with row stride 1 and their
-    // own batch stride, the same offsets as below), run by the streaming
-    // dense kernels
-    {
-        bool values_only = nleaves >= 0 && nleaves <= CM_CWISE_MAX_LEAVES && leaves != nullptr;
-        for (int l = 0; values_only && l < nleaves; ++l) {
+    // a program whose leaves are the value arrays, scalars and leaves that
+    // depend on the column only (row stride 0; one batch or batch stride 0)
+    // needs no row coordinate: each column leaf is first gathered along IA
+    // into an nnz-long temporary, then the program is a dense program over
+    // the nnz x 1 x batches value array (value leaves become leaves with
+    // row stride 1 and their own batch stride, the same offsets as below),
+    // run by the streaming dense kernels
+    if (nleaves >= 0 && nleaves <= CM_CWISE_MAX_LEAVES && leaves != nullptr && val_dtype != CM_BF16 &&
+        cm::dtype_size(val_dtype) > 0) {
+        bool ok = true;
+        size_t gbytes = 0;
+        for (int l = 0; ok && l < nleaves; ++l) {
             const cm_leaf& L = leaves[l];
-            const bool value_leaf = L.sr == -1;
-            const bool scalar = L.kind == CM_LEAF_STRIDED && L.sr == 0 && L.sc == 0 && L.sb == 0;
-            values_only = value_leaf || scalar;
+            if (L.sr == -1) continue;  // a value array
+            const int es = cm::dtype_size(L.dtype);
+            ok = L.kind == CM_LEAF_STRIDED && L.sr == 0 && L.ptr != nullptr && (es == 1 || es == 4 || es == 8);
+            if (ok && L.sc != 0) {
+                ok = L.sb == 0 || batches == 1;
+                gbytes += ((size_t)es * nnz + 255) / 256 * 256;
+            }
         }
-        if (values_only && val_dtype != CM_BF16 && cm::dtype_size(val_dtype) > 0) {
+        if (ok) {
+            void* tmp = nullptr;
+            if (gbytes) CM_TRY(cm::scratch(ctx, 0, gbytes, &tmp));
             cm_leaf dl[CM_CWISE_MAX_LEAVES];
+            size_t off = 0;
+            const long long cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
+            const unsigned gblocks = (unsigned)std::max<long long>(1, std::min<long long>(cm::div_up(nnz, 256), cus * 16));
             for (int l = 0; l < nleaves; ++l) {
                 dl[l] = leaves[l];
                 if (dl[l].sr == -1) {
                     dl[l].sr = 1;
                     dl[l].sc = 0;
+                } else if (dl[l].sc != 0) {
+                    const int es = cm::dtype_size(dl[l].dtype);
+                    void* g = (char*)tmp + off;
+                    if (es == 4)
+                        cm::csr_gather_kernel<uint32_t><<<gblocks, 256, 0, ctx->stream>>>(
+                            (const uint32_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint32_t*)g);
+                    else if (es == 8)
+                        cm::csr_gather_kernel<uint64_t><<<gblocks, 256, 0, ctx->stream>>>(
+                            (const uint64_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint64_t*)g);
+                    else
+                        cm::csr_gather_kernel<uint8_t><<<gblocks, 256, 0, ctx->stream>>>(
+                            (const uint8_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint8_t*)g);
+                    CM_LAUNCH_CHECK();
+                    dl[l].ptr = g;
+                    dl[l].sr = 1;
+                    dl[l].sc = 0;
+                    dl[l].sb = 0;
+                    off += ((size_t)es * nnz + 255) / 256 * 256;
                 }
             }
             return cm::cwise_dense(ctx, dl, nleaves, code, ncode, vals, val_dtype, nnz, 1, batches, 1, nnz, nnz,

@@ -269,3 +269,29 @@ def test_csr_cwise_eval_broadcast_single_batch_values(gpu):
     rows = np.repeat(np.arange(n), np.diff(JA))
     exp = v1 * 3 + v1 * cv[rows]
     np.testing.assert_array_equal(S2.values.numpy().reshape(2, -1), np.stack([exp, exp]))
+
+
+def test_csr_cwise_eval_batched_column_leaf(gpu):
+    """A column-dependent leaf with its own batch stride (a batched row
+    vector) over a two-batch CSR matrix: evaluated per stored entry; and a
+    one-batch row vector over the same matrix (gathered along IA, then a
+    dense program over the values). int32 exact."""
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    g = 50
+    JA, IA, V = poisson2d(g)
+    n = g * g
+    rng = np.random.default_rng(47)
+    v = rng.integers(-50, 50, (2, IA.size)).astype(np.int32)
+    pat = cm.SparsityPattern(n, n, JA, IA)
+    S = cm.SparseMatrix(pat, v.reshape(-1), batches=2)
+    r2 = rng.integers(-9, 9, (2, 1, n)).astype(np.int32)
+    R2 = cm.DeviceArray.from_numpy(r2)  # 1 x n x 2
+    S.assign(S.direct() * R2.expr())
+    exp = v * r2[:, 0, :][:, IA]
+    np.testing.assert_array_equal(S.values.numpy().reshape(2, -1), exp)
+    r1 = rng.integers(-9, 9, n).astype(np.int32)
+    R1 = cm.DeviceArray.from_numpy(r1, rows=1, cols=n, batches=1)
+    S.assign(S.direct() - R1.expr(), mode=cm.capi.ADD)
+    exp = exp + (exp - r1[IA][None, :])
+    np.testing.assert_array_equal(S.values.numpy().reshape(2, -1), exp)
