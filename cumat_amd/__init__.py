@@ -18,7 +18,7 @@ from ._capi import CumatArgError, CumatError  # noqa: F401
 from .device import DeviceArray, Context, default_context, synchronize  # noqa: F401
 from .expr import (Expr, as_expr, assign, cast, select, where_scalar,  # noqa: F401
                    lower, evaluate)
-from .sparse import SparseMatrix, SparsityPattern, ELLMatrix  # noqa: F401
+from .sparse import SparseMatrix, SparsityPattern, ELLMatrix, SparseExpressionView, sparse_view  # noqa: F401
 from .reductions import reduce, dot, squared_norm, norm  # noqa: F401
 from .solvers import ConjugateGradient, DiagonalPreconditioner, pcg_batched, pcg_ir  # noqa: F401
 from . import dense  # noqa: F401
@@ -26,7 +26,7 @@ from . import dense  # noqa: F401
 __all__ = [
     "capi", "DeviceArray", "Context", "default_context", "synchronize", "Expr", "as_expr",
     "assign", "cast", "select", "lower", "evaluate", "SparseMatrix", "SparsityPattern",
-    "ELLMatrix", "reduce", "dot", "squared_norm", "norm", "ConjugateGradient",
+    "ELLMatrix", "SparseExpressionView", "sparse_view", "reduce", "dot", "squared_norm", "norm", "ConjugateGradient",
     "DiagonalPreconditioner", "pcg_batched", "pcg_ir", "dense", "counters", "reset_counters",
     "device_info",
 ]
@@ -35,7 +35,7 @@ __all__ = [
 def counters() -> dict:
     """Profiling counters (cuMat::Profiling, cuMat/src/Profiling.h:23-109)."""
     names = ["DeviceMemAlloc", "DeviceMemFree", "EvalAny", "EvalCwise", "EvalCwiseSparse",
-             "EvalReduction", "EvalMatmulSparse", "KernelLaunch", "EvalMatmul"]
+             "EvalReduction", "EvalMatmulSparse", "KernelLaunch", "EvalMatmul", "HostSync"]
     out = {}
     for i, n in enumerate(names):
         v = C.c_longlong(0)

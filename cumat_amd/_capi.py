@@ -97,6 +97,10 @@ _PROTOS = {
                     _I32),
     "cm_csr_cwise_eval": ([_VP, C.POINTER(Leaf), _I32, C.POINTER(Instr), _I32, _I32, _I32, _I32,
                            _I32, _VP, _VP, _VP, _I32, _I32], _I32),
+    "cm_csr_expr_spmv": ([_VP, C.POINTER(Leaf), _I32, C.POINTER(Instr), _I32, _I32, _I32, _I32, _I32,
+                          _I32, _VP, _VP, _VP, _I32, _VP, _I32], _I32),
+    "cm_ell_expr_spmv": ([_VP, C.POINTER(Leaf), _I32, C.POINTER(Instr), _I32, _I32, _I32, _I32, _I32,
+                          _I32, _VP, _VP, _I32, _VP, _I32], _I32),
     "cm_csr_inverse_diagonal": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP], _I32),
     "cm_dense_gemv": ([_VP, _I32, _I32, _I32, _I32, _VP, _I32, _I32, _VP, _I32, _VP, _I32], _I32),
     "cm_dense_inverse_diagonal": ([_VP, _I32, _I32, _I32, _VP, _I32, _VP], _I32),

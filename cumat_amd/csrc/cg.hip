@@ -54,7 +54,10 @@ struct ResidentArgs {
     int max_iter;
     int* iters_out;       // per system
     T* err_out;           // per system
-    unsigned* queue;      // work counter (zeroed before launch)
+    unsigned* queue;      // queue[0]: work counter, queue[1]: invalid-pattern
+                          // flag (both zeroed before launch)
+    int nnz_total;        // entries of IA / vals (every JA value must lie in
+                          // [0, nnz_total])
 };
 
 template <typename T, int BS, int RPT>
@@ -64,10 +67,14 @@ __global__ __launch_bounds__(BS) void cg_resident_kernel(const ResidentArgs<T> A
     T* pl = reinterpret_cast<T*>(smem_raw);
     __shared__ T red[2 * BS / 64];
     __shared__ int s_sys;
+    __shared__ int s_bad;
     const int tid = threadIdx.x;
     for (;;) {
         // ---- fetch the next system (dynamic queue balances uneven sizes)
-        if (tid == 0) s_sys = (int)atomicAdd(A.queue, 1u);
+        if (tid == 0) {
+            s_sys = (int)atomicAdd(A.queue, 1u);
+            s_bad = 0;
+        }
         __syncthreads();
         const int q = s_sys;
         __syncthreads();
@@ -75,134 +82,169 @@ __global__ __launch_bounds__(BS) void cg_resident_kernel(const ResidentArgs<T> A
         const int sys = A.order ? A.order[q] : q;
         const int row0 = A.row_off ? A.row_off[sys] : 0;
         const int n = A.row_off ? (A.row_off[sys + 1] - row0) : A.n_single;
-        const int k0 = A.JA[row0];
-        T xr[RPT], rr[RPT], id[RPT];
-        // ---- setup: invdiag, x, r = b - A x
+        // ---- the pattern must keep every row inside its own system: p lives
+        // in LDS indexed by IA - row0, so a column outside [row0, row0 + n)
+        // or a row pointer outside [0, nnz_total] would leave the tile. Such a
+        // system is not solved: iterations -1, error NaN, queue[1] set.
+        bool bad = false;
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
-            int i = tid + j * BS;
-            xr[j] = T(0);
-            id[j] = T(1);
+            const int i = tid + j * BS;
             if (i < n) {
-                if (A.invd) id[j] = A.invd[row0 + i];
-                else {
-                    T d = T(0);
-                    for (int k = A.JA[row0 + i]; k < A.JA[row0 + i + 1]; ++k)
-                        if (A.IA[k] - row0 == i) { d = A.vals[k]; break; }
-                    id[j] = (d == T(0)) ? T(1) : T(1) / d;
-                }
-                if (A.use_guess) xr[j] = A.x[row0 + i];
-                pl[i] = xr[j];
-            }
-        }
-        __syncthreads();
-        T bb = 0, r2 = 0;
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            int i = tid + j * BS;
-            rr[j] = T(0);
-            if (i < n) {
-                T ax = T(0);
-                if (A.use_guess) {
-                    int s = A.JA[row0 + i], e = A.JA[row0 + i + 1];
-                    if (s < e) {
-                        ax = A.vals[s] * pl[A.IA[s] - row0];
-                        for (int k = s + 1; k < e; ++k) ax += A.vals[k] * pl[A.IA[k] - row0];
+                const int s = A.JA[row0 + i], e = A.JA[row0 + i + 1];
+                if (s < 0 || e < s || e > A.nnz_total) {
+                    bad = true;
+                } else {
+                    for (int k = s; k < e; ++k) {
+                        const int c = A.IA[k] - row0;
+                        bad |= (c < 0) | (c >= n);
                     }
                 }
-                T bv = A.b[row0 + i];
-                rr[j] = bv - ax;
-                bb += bv * bv;
-                r2 += rr[j] * rr[j];
             }
         }
-        block_sum2<BS>(bb, r2, red);
-        const T thr = A.tol * A.tol * bb;
-        int iters = 0;
-        T err = T(0);
-        bool done = false;
-        if (bb < A.tol * A.tol) {  // zero right-hand side
-#pragma unroll
-            for (int j = 0; j < RPT; ++j) xr[j] = T(0);
-            done = true;
-        } else if (r2 < thr) {  // the guess is already converged
-            err = sqrt(r2 / bb);
-            done = true;
+        // one flag for the workgroup, read into a scalar register so that the
+        // branches on it are uniform
+        if (bad) s_bad = 1;
+        __syncthreads();
+        // Every path ends at the barrier at the end of the loop body: the
+        // loop has no early "continue". (With one, the compiled loop sent the
+        // lanes other than thread 0 straight back to the barrier at its top
+        // while thread 0 waited to fetch the next system, so a workgroup that
+        // met an invalid system spun on it for ever.)
+        const bool invalid = __builtin_amdgcn_readfirstlane(s_bad) != 0;
+        if (invalid && tid == 0) {
+            if (A.iters_out) A.iters_out[sys] = -1;
+            if (A.err_out) A.err_out[sys] = T(NAN);
+            atomicOr(A.queue + 1, 1u);
         }
-        if (!done) {
-            // p = M^-1 r -> LDS, absNew = r.p
-            T absNew = 0, dummy = 0;
+        if (!invalid) {
+            T xr[RPT], rr[RPT], id[RPT];
+            // ---- setup: invdiag, x, r = b - A x
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
                 int i = tid + j * BS;
+                xr[j] = T(0);
+                id[j] = T(1);
                 if (i < n) {
-                    T pv = id[j] * rr[j];
-                    pl[i] = pv;
-                    absNew += rr[j] * pv;
+                    if (A.invd) id[j] = A.invd[row0 + i];
+                    else {
+                        T d = T(0);
+                        for (int k = A.JA[row0 + i]; k < A.JA[row0 + i + 1]; ++k)
+                            if (A.IA[k] - row0 == i) { d = A.vals[k]; break; }
+                        id[j] = (d == T(0)) ? T(1) : T(1) / d;
+                    }
+                    if (A.use_guess) xr[j] = A.x[row0 + i];
+                    pl[i] = xr[j];
                 }
             }
-            block_sum2<BS>(absNew, dummy, red);  // also orders the LDS writes
-            T rsq = r2;
-            int it = 0;
-            while (it < A.max_iter) {
-                // tmp = A p ; p.tmp
-                T tmp[RPT];
-                T pap = 0;
+            __syncthreads();
+            T bb = 0, r2 = 0;
 #pragma unroll
-                for (int j = 0; j < RPT; ++j) {
-                    int i = tid + j * BS;
-                    tmp[j] = T(0);
-                    if (i < n) {
+            for (int j = 0; j < RPT; ++j) {
+                int i = tid + j * BS;
+                rr[j] = T(0);
+                if (i < n) {
+                    T ax = T(0);
+                    if (A.use_guess) {
                         int s = A.JA[row0 + i], e = A.JA[row0 + i + 1];
-                        T acc = T(0);
                         if (s < e) {
-                            acc = A.vals[s] * pl[A.IA[s] - row0];
-                            for (int k = s + 1; k < e; ++k) acc += A.vals[k] * pl[A.IA[k] - row0];
+                            ax = A.vals[s] * pl[A.IA[s] - row0];
+                            for (int k = s + 1; k < e; ++k) ax += A.vals[k] * pl[A.IA[k] - row0];
                         }
-                        tmp[j] = acc;
-                        pap += pl[i] * acc;
                     }
+                    T bv = A.b[row0 + i];
+                    rr[j] = bv - ax;
+                    bb += bv * bv;
+                    r2 += rr[j] * rr[j];
                 }
-                T z0 = 0;
-                block_sum2<BS>(pap, z0, red);
-                const T alpha = absNew / pap;
-                T sr = 0, sz = 0;
+            }
+            block_sum2<BS>(bb, r2, red);
+            const T thr = A.tol * A.tol * bb;
+            int iters = 0;
+            T err = T(0);
+            bool done = false;
+            if (bb < A.tol * A.tol) {  // zero right-hand side
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) xr[j] = T(0);
+                done = true;
+            } else if (r2 < thr) {  // the guess is already converged
+                err = sqrt(r2 / bb);
+                done = true;
+            }
+            if (!done) {
+                // p = M^-1 r -> LDS, absNew = r.p
+                T absNew = 0, dummy = 0;
 #pragma unroll
                 for (int j = 0; j < RPT; ++j) {
                     int i = tid + j * BS;
                     if (i < n) {
-                        xr[j] += alpha * pl[i];
-                        rr[j] -= alpha * tmp[j];
-                        sr += rr[j] * rr[j];
-                        sz += rr[j] * (id[j] * rr[j]);
+                        T pv = id[j] * rr[j];
+                        pl[i] = pv;
+                        absNew += rr[j] * pv;
                     }
                 }
-                block_sum2<BS>(sr, sz, red);
-                rsq = sr;
-                if (sr < thr) break;
-                const T beta = sz / absNew;
-                absNew = sz;
+                block_sum2<BS>(absNew, dummy, red);  // also orders the LDS writes
+                T rsq = r2;
+                int it = 0;
+                while (it < A.max_iter) {
+                    // tmp = A p ; p.tmp
+                    T tmp[RPT];
+                    T pap = 0;
 #pragma unroll
-                for (int j = 0; j < RPT; ++j) {
-                    int i = tid + j * BS;
-                    if (i < n) pl[i] = id[j] * rr[j] + beta * pl[i];
+                    for (int j = 0; j < RPT; ++j) {
+                        int i = tid + j * BS;
+                        tmp[j] = T(0);
+                        if (i < n) {
+                            int s = A.JA[row0 + i], e = A.JA[row0 + i + 1];
+                            T acc = T(0);
+                            if (s < e) {
+                                acc = A.vals[s] * pl[A.IA[s] - row0];
+                                for (int k = s + 1; k < e; ++k) acc += A.vals[k] * pl[A.IA[k] - row0];
+                            }
+                            tmp[j] = acc;
+                            pap += pl[i] * acc;
+                        }
+                    }
+                    T z0 = 0;
+                    block_sum2<BS>(pap, z0, red);
+                    const T alpha = absNew / pap;
+                    T sr = 0, sz = 0;
+#pragma unroll
+                    for (int j = 0; j < RPT; ++j) {
+                        int i = tid + j * BS;
+                        if (i < n) {
+                            xr[j] += alpha * pl[i];
+                            rr[j] -= alpha * tmp[j];
+                            sr += rr[j] * rr[j];
+                            sz += rr[j] * (id[j] * rr[j]);
+                        }
+                    }
+                    block_sum2<BS>(sr, sz, red);
+                    rsq = sr;
+                    if (sr < thr) break;
+                    const T beta = sz / absNew;
+                    absNew = sz;
+#pragma unroll
+                    for (int j = 0; j < RPT; ++j) {
+                        int i = tid + j * BS;
+                        if (i < n) pl[i] = id[j] * rr[j] + beta * pl[i];
+                    }
+                    __syncthreads();
+                    ++it;
                 }
-                __syncthreads();
-                ++it;
+                iters = it;
+                err = sqrt(rsq / bb);
             }
-            iters = it;
-            err = sqrt(rsq / bb);
-        }
 #pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            int i = tid + j * BS;
-            if (i < n) A.x[row0 + i] = xr[j];
+            for (int j = 0; j < RPT; ++j) {
+                int i = tid + j * BS;
+                if (i < n) A.x[row0 + i] = xr[j];
+            }
+            if (tid == 0) {
+                if (A.iters_out) A.iters_out[sys] = iters;
+                if (A.err_out) A.err_out[sys] = err;
+            }
         }
-        if (tid == 0) {
-            if (A.iters_out) A.iters_out[sys] = iters;
-            if (A.err_out) A.err_out[sys] = err;
-        }
-        (void)k0;
         __syncthreads();
     }
 }
@@ -257,8 +299,8 @@ static int launch_resident(cm_ctx_t ctx, const ResidentArgs<T>& A, int nmax, int
 }
 
 template <typename T>
-static int pcg_resident(cm_ctx_t ctx, int n, const int* JA, const int* IA, const T* vals, const T* invd, const T* b,
-                        T* x, int use_guess, double tol, int max_iter, int* iters_out, double* err_out,
+static int pcg_resident(cm_ctx_t ctx, int n, int nnz, const int* JA, const int* IA, const T* vals, const T* invd,
+                        const T* b, T* x, int use_guess, double tol, int max_iter, int* iters_out, double* err_out,
                         cm_cg_stats* stats)
 {
     Temps tmp(ctx);
@@ -284,13 +326,17 @@ static int pcg_resident(cm_ctx_t ctx, int n, const int* JA, const int* IA, const
     A.iters_out = (int*)io;
     A.err_out = (T*)eo;
     A.queue = (unsigned*)q;
+    A.nnz_total = nnz;
     auto t0 = std::chrono::steady_clock::now();
     CM_TRY(launch_resident<T>(ctx, A, n, 1024, 1));
     int hi = 0;
     T he = 0;
+    unsigned bad = 0;
     CM_HIP(hipMemcpyAsync(&hi, io, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     CM_HIP(hipMemcpyAsync(&he, eo, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_HIP(hipMemcpyAsync(&bad, (unsigned*)q + 1, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    CM_TRY(::cm::host_sync(ctx));
+    CM_REQUIRE(!bad, CM_E_ARG, "resident CG: a row pointer or column index lies outside the %d x %d matrix", n, n);
     auto t1 = std::chrono::steady_clock::now();
     if (iters_out) *iters_out = hi;
     if (err_out) *err_out = (double)he;
@@ -306,15 +352,20 @@ static int pcg_resident(cm_ctx_t ctx, int n, const int* JA, const int* IA, const
 struct BatchedPlan {
     int nmax = 0;
     std::vector<int> order;  // systems by decreasing nnz (longest first)
+    uint64_t offsets_hash = 0;  // FNV-1a of row_off: identifies the solve order
 };
 
-static int make_plan(cm_ctx_t ctx, int nsys, const int* row_off_dev, const int* JA_dev, BatchedPlan& plan)
+static int make_plan(cm_ctx_t ctx, int nsys, const int* row_off_dev, int total_rows, BatchedPlan& plan)
 {
-    (void)JA_dev;
     std::vector<int> ro(nsys + 1);
     CM_HIP(hipMemcpyAsync(ro.data(), row_off_dev, sizeof(int) * (nsys + 1), hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_TRY(::cm::host_sync(ctx));
+    CM_REQUIRE(ro[0] >= 0 && ro[nsys] <= total_rows, CM_E_ARG,
+               "batched CG: row offsets [%d, %d] outside the %d rows", ro[0], ro[nsys], total_rows);
     plan.nmax = 0;
+    uint64_t h = 1469598103934665603ull;
+    for (int v : ro) h = (h ^ (uint32_t)v) * 1099511628211ull;
+    plan.offsets_hash = h;
     std::vector<std::pair<long long, int>> cost(nsys);
     for (int s = 0; s < nsys; ++s) {
         int n = ro[s + 1] - ro[s];
@@ -330,25 +381,59 @@ static int make_plan(cm_ctx_t ctx, int nsys, const int* row_off_dev, const int* 
     return CM_OK;
 }
 
+// A captured solve fixes the launch shape (block size, register tile and LDS
+// size, both from the largest system), the solve order and every pointer, so
+// all of them are part of the key.
 struct GraphKey {
     cm_ctx_t ctx;
     const void* ptrs[9];
-    int nsys, bs, max_iter;
+    int nsys, bs, max_iter, nmax, total_rows, total_nnz;
     float tol;
+    uint64_t offsets_hash;
     bool operator<(const GraphKey& o) const { return memcmp(this, &o, sizeof(GraphKey)) < 0; }
 };
 struct GraphEntry {
     hipGraphExec_t exec = nullptr;
     void* order = nullptr;
     void* queue = nullptr;
+    unsigned long long last_use = 0;
 };
+// Bounded caches: at most this many captured solves (least recently used
+// first out) and autotuned shapes.
+constexpr size_t GRAPH_CACHE_MAX = 8;
+constexpr size_t TUNE_CACHE_MAX = 64;
 static std::mutex g_graph_mutex;
 static std::map<GraphKey, GraphEntry> g_graphs;
+static unsigned long long g_graph_clock = 0;
 static std::map<std::tuple<int, int, int, int>, int> g_tuned_bs;
 
-static int batched_launch(cm_ctx_t ctx, int nsys, const int* row_off, const int* JA, const int* IA, const float* vals,
-                          const float* b, float* x, float tol, int max_iter, int* iters, float* errors, int bs,
-                          const BatchedPlan& plan, void* order_dev, void* queue_dev)
+// Destroys a cached graph and returns its blocks (caller holds the mutex; the
+// stream is idle after the synchronous solve that last used it).
+static void graph_release(GraphEntry& ge)
+{
+    if (ge.exec) (void)hipGraphExecDestroy(ge.exec);
+    if (ge.order) (void)dfree(nullptr, ge.order);
+    if (ge.queue) (void)dfree(nullptr, ge.queue);
+    ge = GraphEntry();
+}
+
+// Removes every cached graph of one context (called before it is destroyed).
+void graph_cache_drop(cm_ctx_t ctx)
+{
+    std::lock_guard<std::mutex> lk(g_graph_mutex);
+    for (auto it = g_graphs.begin(); it != g_graphs.end();) {
+        if (it->first.ctx == ctx) {
+            graph_release(it->second);
+            it = g_graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+static int batched_launch(cm_ctx_t ctx, int nsys, const int* row_off, int total_nnz, const int* JA, const int* IA,
+                          const float* vals, const float* b, float* x, float tol, int max_iter, int* iters,
+                          float* errors, int bs, const BatchedPlan& plan, void* order_dev, void* queue_dev)
 {
     CM_TRY(fill_zero(ctx, queue_dev, 256));
     ResidentArgs<float> A{};
@@ -367,6 +452,7 @@ static int batched_launch(cm_ctx_t ctx, int nsys, const int* row_off, const int*
     A.iters_out = iters;
     A.err_out = errors;
     A.queue = (unsigned*)queue_dev;
+    A.nnz_total = total_nnz;
     return launch_resident<float>(ctx, A, plan.nmax, bs, 0);
 }
 
@@ -486,13 +572,13 @@ static int pcg_ir(cm_ctx_t ctx, int n, int nnz, const int* JA, const int* IA, co
     CM_TRY(cm_dot(ctx, CM_F64, n, 1, b, 1, b, 1, bn2));
     double hbn2 = 0, hrn2 = 0;
     CM_HIP(hipMemcpyAsync(&hbn2, bn2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_TRY(::cm::host_sync(ctx));
     int outer = 0, inner_total = 0;
     double rel = (hbn2 > 0) ? 1.0 : 0.0;
     while (hbn2 > 0) {
         CM_TRY(cm_dot(ctx, CM_F64, n, 1, r, 1, r, 1, rn2));
         CM_HIP(hipMemcpyAsync(&hrn2, rn2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        CM_HIP(hipStreamSynchronize(ctx->stream));
+        CM_TRY(::cm::host_sync(ctx));
         rel = std::sqrt(hrn2 / hbn2);
         if (rel <= tol || outer >= max_outer || hrn2 == 0.0) break;
         ir_scale_kernel<<<g, 256, 0, ctx->stream>>>(n, (const double*)r, rn2, (float*)r32);
@@ -539,60 +625,100 @@ int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off, int total_r
     if (nsys == 0) return CM_OK;
     CM_REQUIRE(row_off && JA && IA && vals && b && x, CM_E_ARG, "cm_pcg_batched_solve: NULL pointer");
     cm::BatchedPlan plan;
-    CM_TRY(cm::make_plan(ctx, nsys, row_off, JA, plan));
+    CM_TRY(cm::make_plan(ctx, nsys, row_off, total_rows, plan));
     CM_REQUIRE(plan.nmax <= 16384, CM_E_UNSUPPORTED, "batched CG: system of %d rows (max 16384)", plan.nmax);
     if (block_size == 0) {
         std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
         auto it = cm::g_tuned_bs.find({nsys, total_rows, total_nnz, plan.nmax});
         block_size = (it != cm::g_tuned_bs.end()) ? it->second : cm::default_bs(plan.nmax);
     }
+    unsigned bad = 0;
     if (!use_graph) {
         cm::Temps tmp(ctx);
         void *ord, *q;
         CM_TRY(tmp.get(sizeof(int) * nsys, &ord));
         CM_TRY(tmp.get(256, &q));
         CM_HIP(hipMemcpyAsync(ord, plan.order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
-        CM_TRY(cm::batched_launch(ctx, nsys, row_off, JA, IA, vals, b, x, tol, max_iter, iterations, errors,
-                                  block_size, plan, ord, q));
-        CM_HIP(hipStreamSynchronize(ctx->stream));
-        return CM_OK;
+        CM_TRY(cm::batched_launch(ctx, nsys, row_off, total_nnz, JA, IA, vals, b, x, tol, max_iter, iterations,
+                                  errors, block_size, plan, ord, q));
+        CM_HIP(hipMemcpyAsync(&bad, (unsigned*)q + 1, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        CM_TRY(::cm::host_sync(ctx));
+    } else {
+        // graph path: the (queue reset + persistent solve) pair is captured
+        // once per argument set and replayed
+        cm::GraphKey key;
+        memset(&key, 0, sizeof(key));
+        key.ctx = ctx;
+        const void* ps[9] = {row_off, JA, IA, vals, b, x, iterations, errors, nullptr};
+        memcpy(key.ptrs, ps, sizeof(ps));
+        key.nsys = nsys;
+        key.bs = block_size;
+        key.max_iter = max_iter;
+        key.nmax = plan.nmax;
+        key.total_rows = total_rows;
+        key.total_nnz = total_nnz;
+        key.tol = tol;
+        key.offsets_hash = plan.offsets_hash;
+        std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
+        auto it = cm::g_graphs.find(key);
+        if (it == cm::g_graphs.end()) {
+            if (cm::g_graphs.size() >= cm::GRAPH_CACHE_MAX) {
+                auto lru = cm::g_graphs.begin();
+                for (auto j = cm::g_graphs.begin(); j != cm::g_graphs.end(); ++j)
+                    if (j->second.last_use < lru->second.last_use) lru = j;
+                // the evicted graph may belong to another context's stream
+                CM_HIP(hipStreamSynchronize(lru->first.ctx->stream));
+                cm::graph_release(lru->second);
+                cm::g_graphs.erase(lru);
+            }
+            cm::GraphEntry ge;
+            CM_TRY(cm::dalloc(ctx, sizeof(int) * nsys, &ge.order));
+            int s0 = cm::dalloc(ctx, 256, &ge.queue);
+            if (s0 != CM_OK) {
+                cm::graph_release(ge);
+                return s0;
+            }
+            hipGraph_t g = nullptr;
+            int s = CM_OK;
+            if (hipMemcpyAsync(ge.order, plan.order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice,
+                               ctx->stream) != hipSuccess)
+                s = cm::fail(CM_E_HIP, "batched CG: upload of the solve order failed");
+            // the kernel attribute must be set outside capture
+            if (s == CM_OK)
+                s = cm::batched_launch(ctx, 0, row_off, total_nnz, JA, IA, vals, b, x, tol, max_iter, iterations,
+                                       errors, block_size, plan, ge.order, ge.queue);
+            if (s == CM_OK && ::cm::host_sync(ctx) != CM_OK)
+                s = cm::fail(CM_E_HIP, "batched CG: stream synchronization failed");
+            if (s == CM_OK) {
+                if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                    s = cm::fail(CM_E_HIP, "batched CG: stream capture failed to start");
+                } else {
+                    s = cm::batched_launch(ctx, nsys, row_off, total_nnz, JA, IA, vals, b, x, tol, max_iter,
+                                           iterations, errors, block_size, plan, ge.order, ge.queue);
+                    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+                    if (s == CM_OK && e != hipSuccess)
+                        s = cm::fail(CM_E_HIP, "batched CG: capture failed: %s", hipGetErrorString(e));
+                }
+            }
+            if (s == CM_OK && hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0) != hipSuccess)
+                s = cm::fail(CM_E_HIP, "batched CG: graph instantiation failed");
+            if (g) (void)hipGraphDestroy(g);
+            if (s != CM_OK) {
+                cm::graph_release(ge);
+                return s;
+            }
+            it = cm::g_graphs.emplace(key, ge).first;
+        }
+        it->second.last_use = ++cm::g_graph_clock;
+        CM_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
+        cm::count(CM_CNT_KERNEL_LAUNCH, 2);
+        CM_HIP(hipMemcpyAsync(&bad, (unsigned*)it->second.queue + 1, sizeof(unsigned), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        CM_TRY(::cm::host_sync(ctx));
     }
-    // graph path: the (queue reset + persistent solve) pair is captured once
-    // per argument set and replayed
-    cm::GraphKey key;
-    memset(&key, 0, sizeof(key));
-    key.ctx = ctx;
-    const void* ps[9] = {row_off, JA, IA, vals, b, x, iterations, errors, nullptr};
-    memcpy(key.ptrs, ps, sizeof(ps));
-    key.nsys = nsys;
-    key.bs = block_size;
-    key.max_iter = max_iter;
-    key.tol = tol;
-    std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
-    auto it = cm::g_graphs.find(key);
-    if (it == cm::g_graphs.end()) {
-        cm::GraphEntry ge;
-        CM_TRY(cm::dalloc(ctx, sizeof(int) * nsys, &ge.order));
-        CM_TRY(cm::dalloc(ctx, 256, &ge.queue));
-        CM_HIP(hipMemcpyAsync(ge.order, plan.order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
-        // the kernel attribute must be set outside capture
-        CM_TRY(cm::batched_launch(ctx, 0, row_off, JA, IA, vals, b, x, tol, max_iter, iterations, errors, block_size,
-                                  plan, ge.order, ge.queue));
-        CM_HIP(hipStreamSynchronize(ctx->stream));
-        hipGraph_t g;
-        CM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int s = cm::batched_launch(ctx, nsys, row_off, JA, IA, vals, b, x, tol, max_iter, iterations, errors,
-                                   block_size, plan, ge.order, ge.queue);
-        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        CM_TRY(s);
-        CM_HIP(e);
-        CM_HIP(hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0));
-        CM_HIP(hipGraphDestroy(g));
-        it = cm::g_graphs.emplace(key, ge).first;
-    }
-    CM_HIP(hipGraphLaunch(it->second.exec, ctx->stream));
-    cm::count(CM_CNT_KERNEL_LAUNCH, 2);
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_REQUIRE(!bad, CM_E_ARG,
+               "batched CG: a system's row pointers or column indices leave its own rows (those systems report "
+               "iterations -1 and error NaN)");
     return CM_OK;
     CM_API_END
 }
@@ -604,8 +730,9 @@ int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off, int tota
     CM_API_BEGIN
     CM_TRY(cm::resolve(ctx, &ctx));
     CM_REQUIRE(nsys > 0, CM_E_DIM, "autotune: no systems");
+    CM_REQUIRE(row_off && JA && IA && vals && b, CM_E_ARG, "autotune: NULL pointer");
     cm::BatchedPlan plan;
-    CM_TRY(cm::make_plan(ctx, nsys, row_off, JA, plan));
+    CM_TRY(cm::make_plan(ctx, nsys, row_off, total_rows, plan));
     cm::Temps tmp(ctx);
     void *xs, *it, *er, *ord, *q;
     CM_TRY(tmp.get(sizeof(float) * (size_t)total_rows, &xs));
@@ -620,27 +747,37 @@ int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off, int tota
     int best = 0;
     float bestms = 1e30f;
     const int cands[3] = {256, 512, 1024};
+    int status = CM_OK;
     for (int bs : cands) {
         if ((plan.nmax + bs - 1) / bs > (bs <= 512 ? 32 : 16)) continue;  // register tile too large
         float ms_min = 1e30f;
-        for (int rep = 0; rep < 2; ++rep) {
-            CM_HIP(hipEventRecord(e0, ctx->stream));
-            int s = cm::batched_launch(ctx, nsys, row_off, JA, IA, vals, b, (float*)xs, tol, max_iter, (int*)it,
-                                       (float*)er, bs, plan, ord, q);
-            if (s != CM_OK) break;
-            CM_HIP(hipEventRecord(e1, ctx->stream));
-            CM_HIP(hipEventSynchronize(e1));
+        for (int rep = 0; rep < 2 && status == CM_OK; ++rep) {
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) { status = cm::fail(CM_E_HIP, "autotune: event"); break; }
+            status = cm::batched_launch(ctx, nsys, row_off, total_nnz, JA, IA, vals, b, (float*)xs, tol, max_iter,
+                                        (int*)it, (float*)er, bs, plan, ord, q);
+            if (status != CM_OK) break;
             float ms = 0;
-            CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                status = cm::fail(CM_E_HIP, "autotune: event timing failed");
+                break;
+            }
             ms_min = std::min(ms_min, ms);
         }
+        if (status != CM_OK) break;
         if (ms_min < bestms) { bestms = ms_min; best = bs; }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    CM_TRY(status);
+    unsigned bad = 0;
+    CM_HIP(hipMemcpyAsync(&bad, (unsigned*)q + 1, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    CM_TRY(::cm::host_sync(ctx));
+    CM_REQUIRE(!bad, CM_E_ARG, "autotune: a system's row pointers or column indices leave its own rows");
     CM_REQUIRE(best > 0, CM_E_UNSUPPORTED, "autotune: no launch shape fits systems of %d rows", plan.nmax);
     {
         std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
+        if (cm::g_tuned_bs.size() >= cm::TUNE_CACHE_MAX) cm::g_tuned_bs.clear();
         cm::g_tuned_bs[{nsys, total_rows, total_nnz, plan.nmax}] = best;
     }
     if (best_block_size) *best_block_size = best;
@@ -672,10 +809,10 @@ int cm_pcg_solve(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int
     if (path == CM_CG_RESIDENT) {
         CM_REQUIRE(resident_ok, CM_E_UNSUPPORTED, "resident CG needs one rhs and n <= 16384");
         if (dtype == CM_F32)
-            return cm::pcg_resident<float>(ctx, n, JA, IA, (const float*)vals, (const float*)invdiag, (const float*)b,
+            return cm::pcg_resident<float>(ctx, n, nnz, JA, IA, (const float*)vals, (const float*)invdiag, (const float*)b,
                                            (float*)x, use_guess, tol, max_iter, iterations_out, error_out, stats);
         if (dtype == CM_F64)
-            return cm::pcg_resident<double>(ctx, n, JA, IA, (const double*)vals, (const double*)invdiag,
+            return cm::pcg_resident<double>(ctx, n, nnz, JA, IA, (const double*)vals, (const double*)invdiag,
                                             (const double*)b, (double*)x, use_guess, tol, max_iter, iterations_out,
                                             error_out, stats);
         return cm::fail(CM_E_UNSUPPORTED, "cm_pcg_solve: dtype %d", dtype);

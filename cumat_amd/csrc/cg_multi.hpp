@@ -951,6 +951,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         // lagged status check: read the previous chunk's word without draining
         // the queue, then request the current one
         if (pending) {
+            count(CM_CNT_HOST_SYNC);
             CM_HIP(hipEventSynchronize(ev));
             if (hstat[0] != CG_RUNNING) break;
         }
@@ -972,7 +973,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     launches += 1;
     CgState<T> hs;
     CM_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_TRY(::cm::host_sync(ctx));
     CM_HIP(hipEventDestroy(ev));
     auto t1 = std::chrono::steady_clock::now();
     if (opt.per_batch) {

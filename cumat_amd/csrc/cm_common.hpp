@@ -94,10 +94,16 @@ int scratch(cm_ctx_t ctx, int slot, size_t bytes, void** out);
 int pinned(cm_ctx_t ctx, void** out);
 // Caching device allocator (the same pool as cm_malloc / cm_free).
 int dalloc(cm_ctx_t ctx, size_t bytes, void** out);
+// Stream-ordered on ctx's stream; ctx == nullptr: no queued work uses p.
 int dfree(cm_ctx_t ctx, void* p);
+// Drops the captured batched-CG graphs of a context (before it is destroyed).
+void graph_cache_drop(cm_ctx_t ctx);
+
+// Host wait for the context's stream (counted as CM_CNT_HOST_SYNC).
+int host_sync(cm_ctx_t ctx);
 
 // RAII holder for temporaries of one entry point; the blocks return to the
-// cache when the holder dies (after a stream sync, as the work may be queued).
+// cache when the holder dies, stream-ordered after the work queued with them.
 struct Temps {
     cm_ctx_t ctx;
     std::vector<void*> ptrs;
@@ -110,9 +116,7 @@ struct Temps {
     }
     ~Temps()
     {
-        if (ptrs.empty()) return;
-        (void)hipStreamSynchronize(ctx->stream);
-        for (void* p : ptrs) dfree(ctx, p);
+        for (void* p : ptrs) (void)dfree(ctx, p);
     }
 };
 

@@ -420,8 +420,27 @@ __global__ __launch_bounds__(RBS) void seg_block_v4_kernel(Src4 src, long long L
 
 // Long contiguous segments (L % 4 == 0, 16-byte aligned) through the 16-byte
 // block kernel; same chunking as run_segments' block path.
+// Where the per-chunk partials of a split reduction live: the context's
+// scratch (grown by the caching allocator) unless a fixed buffer is given.
+struct PartBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+static int partials(cm_ctx_t ctx, const PartBuf& pb, size_t bytes, void** out)
+{
+    if (pb.ptr) {
+        CM_REQUIRE(bytes <= pb.bytes, CM_E_DIM,
+                   "reduction partials of %zu bytes exceed the fixed %zu-byte device buffer", bytes, pb.bytes);
+        *out = pb.ptr;
+        return CM_OK;
+    }
+    return scratch(ctx, 1, bytes, out);
+}
+
 template <typename Op, typename T, typename Src4>
-static int run_segments_v4(cm_ctx_t ctx, Src4 src, long long L, long long nseg, T* out)
+static int run_segments_v4(cm_ctx_t ctx, Src4 src, long long L, long long nseg, T* out,
+                           const PartBuf& pb = PartBuf())
 {
     const long long cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
     const long long L4 = L / 4;
@@ -434,7 +453,7 @@ static int run_segments_v4(cm_ctx_t ctx, Src4 src, long long L, long long nseg, 
     long long chunk4 = (L4 + nchunk - 1) / nchunk;
     nchunk = (L4 + chunk4 - 1) / chunk4;
     void* part = nullptr;
-    if (nchunk > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * nseg * nchunk, &part));
+    if (nchunk > 1) CM_TRY(partials(ctx, pb, sizeof(T) * nseg * nchunk, &part));
     const long long blocks = nseg * nchunk;
     CM_REQUIRE(blocks < (1ll << 31), CM_E_DIM, "reduction too large");
     seg_block_v4_kernel<Op, T, Src4><<<(unsigned)blocks, RBS, 0, ctx->stream>>>(src, L4, (int)nchunk, chunk4,
@@ -455,7 +474,7 @@ struct Shape {
 
 template <typename Op, typename T, typename S, typename Src>
 static int run_segments(cm_ctx_t ctx, Src src, long long L, long long nseg, S* out, bool plain_vec4,
-                        const void* plain_ptr)
+                        const void* plain_ptr, const PartBuf& pb = PartBuf())
 {
     if (nseg == 0) return CM_OK;
     if (L == 0) {
@@ -485,7 +504,8 @@ static int run_segments(cm_ctx_t ctx, Src src, long long L, long long nseg, S* o
     }
     // block per chunk: aim for >= 8 blocks per CU overall, chunks >= 4096
     if constexpr (std::is_same<S, T>::value && sizeof(T) == 4) {
-        if (plain_vec4 && L % 4 == 0) return run_segments_v4<Op, T>(ctx, Src4Plain<T>{(const T*)plain_ptr}, L, nseg, (T*)out);
+        if (plain_vec4 && L % 4 == 0)
+            return run_segments_v4<Op, T>(ctx, Src4Plain<T>{(const T*)plain_ptr}, L, nseg, (T*)out, pb);
     }
     long long want_blocks = cus * 8;
     long long nchunk = (want_blocks + nseg - 1) / nseg;
@@ -496,7 +516,7 @@ static int run_segments(cm_ctx_t ctx, Src src, long long L, long long nseg, S* o
     long long chunk = (L + nchunk - 1) / nchunk;
     nchunk = (L + chunk - 1) / chunk;
     void* part = nullptr;
-    if (nchunk > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * nseg * nchunk, &part));
+    if (nchunk > 1) CM_TRY(partials(ctx, pb, sizeof(T) * nseg * nchunk, &part));
     long long blocks = nseg * nchunk;
     CM_REQUIRE(blocks < (1ll << 31), CM_E_DIM, "reduction too large");
     seg_block_kernel<Op, T, S, Src><<<(unsigned)blocks, RBS, 0, ctx->stream>>>(src, L, L, (int)nchunk, chunk,
@@ -711,7 +731,8 @@ static int reduce_entry(cm_ctx_t ctx, int dtype, int op, int axis, int rows, int
 
 // dot: per batch sum a*b (a/b optionally broadcast)
 template <typename T>
-static int dot_entry(cm_ctx_t ctx, long long n, int batches, const T* a, int ab, const T* b, int bb, T* out)
+static int dot_entry(cm_ctx_t ctx, long long n, int batches, const T* a, int ab, const T* b, int bb, T* out,
+                     const PartBuf& part = PartBuf())
 {
     for (int k = 0; k < batches; ++k) {
         // batch-by-batch keeps broadcasting trivial; launches stay few for the
@@ -721,11 +742,11 @@ static int dot_entry(cm_ctx_t ctx, long long n, int batches, const T* a, int ab,
         if constexpr (sizeof(T) == 4) {
             // 16-byte loads for long aligned vectors (2^26 FP32: see run_segments_v4)
             if (n % 4 == 0 && n >= (1 << 16) && (((uintptr_t)pa | (uintptr_t)pb) & 15) == 0) {
-                CM_TRY((run_segments_v4<OpSum, T>(ctx, Src4Dot<T>{pa, pb}, n, 1, out + k)));
+                CM_TRY((run_segments_v4<OpSum, T>(ctx, Src4Dot<T>{pa, pb}, n, 1, out + k, part)));
                 continue;
             }
         }
-        CM_TRY((run_segments<OpSum, T, T>(ctx, SrcDot<T, T>{pa, pb}, n, 1, out + k, false, nullptr)));
+        CM_TRY((run_segments<OpSum, T, T>(ctx, SrcDot<T, T>{pa, pb}, n, 1, out + k, false, nullptr, part)));
     }
     return CM_OK;
 }

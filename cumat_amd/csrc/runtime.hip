@@ -32,16 +32,27 @@ void count(int counter, long long v)
 
 // ---- caching allocator: freed blocks are kept per (device, size class) and
 // reused, like cuMat's use of cub::CachingDeviceAllocator
-// (cuMat/src/Context.h:223-229,271-286). Thread-safe.
+// (cuMat/src/Context.h:223-229,271-286,311-325). Frees are stream-ordered:
+// a freed block carries the stream that freed it and an event recorded there
+// after the work queued so far. The same stream reuses it at once (its later
+// work runs after that work); another stream either takes a block whose event
+// has completed or makes its stream wait for the event on the device. The
+// host never waits in cm_free. Thread-safe.
 namespace {
 struct Block {
     void* ptr;
     size_t bytes;
     int device;
 };
+struct FreeBlock {
+    void* ptr;
+    hipStream_t stream;   // stream of the last free (nullptr: no pending work)
+    hipEvent_t event;     // recorded on 'stream' at the free (nullptr: none)
+};
 std::mutex g_alloc_mutex;
-std::multimap<std::pair<int, size_t>, void*> g_free_blocks;
+std::multimap<std::pair<int, size_t>, FreeBlock> g_free_blocks;
 std::map<void*, Block> g_live_blocks;
+std::vector<hipEvent_t> g_event_pool;  // idle events (disable-timing)
 
 size_t size_class(size_t bytes)
 {
@@ -52,29 +63,78 @@ size_t size_class(size_t bytes)
     // above 1 MiB: round up to 1 MiB granules
     return (bytes + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
 }
+
+// caller holds g_alloc_mutex
+void event_release(hipEvent_t e)
+{
+    if (e) g_event_pool.push_back(e);
+}
+hipEvent_t event_acquire()
+{
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+}
 }  // namespace
 
-static int device_alloc(int device, size_t bytes, void** out)
+// Picks a cached block of class 'cls' for 'stream' (caller holds the mutex).
+// Returns nullptr when none is usable without a host wait.
+static void* take_cached(int device, size_t cls, hipStream_t stream)
+{
+    auto range = g_free_blocks.equal_range({device, cls});
+    if (range.first == range.second) return nullptr;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(stream, &cap);
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    auto pick = range.second;
+    // 1. freed on this stream (or nothing pending): stream order suffices
+    for (auto it = range.first; it != range.second; ++it)
+        if (!it->second.event || it->second.stream == stream) { pick = it; break; }
+    // 2. its event has completed
+    if (pick == range.second)
+        for (auto it = range.first; it != range.second; ++it)
+            if (hipEventQuery(it->second.event) == hipSuccess) { pick = it; break; }
+    // 3. make this stream wait for the free on the device (not while the
+    //    stream is captured: the wait would tie the graph to outside work)
+    if (pick == range.second && !capturing) {
+        for (auto it = range.first; it != range.second; ++it)
+            if (hipStreamWaitEvent(stream, it->second.event, 0) == hipSuccess) { pick = it; break; }
+    }
+    (void)hipGetLastError();
+    if (pick == range.second) return nullptr;
+    void* p = pick->second.ptr;
+    event_release(pick->second.event);
+    g_free_blocks.erase(pick);
+    return p;
+}
+
+static int device_alloc(int device, size_t bytes, hipStream_t stream, void** out)
 {
     size_t cls = size_class(bytes);
     {
         std::lock_guard<std::mutex> lk(g_alloc_mutex);
-        auto it = g_free_blocks.find({device, cls});
-        if (it != g_free_blocks.end()) {
-            *out = it->second;
-            g_live_blocks[*out] = Block{*out, cls, device};
-            g_free_blocks.erase(it);
+        void* p = take_cached(device, cls, stream);
+        if (p) {
+            *out = p;
+            g_live_blocks[p] = Block{p, cls, device};
             return CM_OK;
         }
     }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, cls);
     if (e != hipSuccess) {
-        // release the cache of this device and retry once
+        // release the cache of this device and retry once (hipFree waits for
+        // the device, so pending work on the cached blocks has finished)
         std::lock_guard<std::mutex> lk(g_alloc_mutex);
         for (auto it = g_free_blocks.begin(); it != g_free_blocks.end();) {
             if (it->first.first == device) {
-                hipFree(it->second);
+                (void)hipFree(it->second.ptr);
+                event_release(it->second.event);
                 it = g_free_blocks.erase(it);
             } else {
                 ++it;
@@ -91,15 +151,39 @@ static int device_alloc(int device, size_t bytes, void** out)
     return CM_OK;
 }
 
-static int device_free(void* p)
+// Returns a block to the cache. ordered: after the work queued on 'stream'
+// (which may be the null stream); otherwise no work may use it any more.
+static int device_free(void* p, hipStream_t stream, bool ordered)
 {
     if (!p) return CM_OK;
     std::lock_guard<std::mutex> lk(g_alloc_mutex);
     auto it = g_live_blocks.find(p);
     if (it == g_live_blocks.end()) return fail(CM_E_ARG, "cm_free: pointer %p was not allocated by cm_malloc", p);
-    g_free_blocks.insert({{it->second.device, it->second.bytes}, p});
+    FreeBlock fb{p, nullptr, nullptr};
+    if (ordered) {
+        hipEvent_t ev = event_acquire();
+        if (ev && hipEventRecord(ev, stream) == hipSuccess) {
+            fb.stream = stream;
+            fb.event = ev;
+        } else {
+            // no event: wait here rather than hand out a block still in use
+            event_release(ev);
+            (void)hipGetLastError();
+            count(CM_CNT_HOST_SYNC);
+            if (hipStreamSynchronize(stream) != hipSuccess)
+                return fail(CM_E_HIP, "cm_free: stream synchronization failed");
+        }
+    }
+    g_free_blocks.insert({{it->second.device, it->second.bytes}, fb});
     g_live_blocks.erase(it);
     count(CM_CNT_DEVICE_MEM_FREE);
+    return CM_OK;
+}
+
+int host_sync(cm_ctx_t ctx)
+{
+    count(CM_CNT_HOST_SYNC);
+    CM_HIP(hipStreamSynchronize(ctx->stream));
     return CM_OK;
 }
 
@@ -123,8 +207,9 @@ static void release_context(cm_context* c)
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    graph_cache_drop(c);
     for (auto& s : c->scratch)
-        if (s.ptr) device_free(s.ptr);
+        if (s.ptr) device_free(s.ptr, nullptr, false);
     if (c->pinned) hipHostFree(c->pinned);
     if (c->owns_stream && c->stream) hipStreamDestroy(c->stream);
 }
@@ -165,14 +250,14 @@ int scratch(cm_ctx_t ctx, int slot, size_t bytes, void** out)
     Scratch& s = ctx->scratch[slot];
     if (s.bytes < bytes) {
         if (s.ptr) {
-            // the old block may still be in use by queued work
-            CM_HIP(hipStreamSynchronize(ctx->stream));
-            CM_TRY(device_free(s.ptr));
+            // the old block may still be read by queued work: a stream-ordered
+            // free keeps it from other streams until that work is done
+            CM_TRY(device_free(s.ptr, ctx->stream, true));
             s.ptr = nullptr;
             s.bytes = 0;
         }
         void* p = nullptr;
-        CM_TRY(device_alloc(ctx->device, bytes, &p));
+        CM_TRY(device_alloc(ctx->device, bytes, ctx->stream, &p));
         // zero once so that "last block" counters start armed (a fill kernel,
         // not a memset node, so that the call is safe under stream capture)
         CM_TRY(fill_zero(ctx, p, bytes));
@@ -183,8 +268,8 @@ int scratch(cm_ctx_t ctx, int slot, size_t bytes, void** out)
     return CM_OK;
 }
 
-int dalloc(cm_ctx_t ctx, size_t bytes, void** out) { return device_alloc(ctx->device, bytes, out); }
-int dfree(cm_ctx_t, void* p) { return device_free(p); }
+int dalloc(cm_ctx_t ctx, size_t bytes, void** out) { return device_alloc(ctx->device, bytes, ctx->stream, out); }
+int dfree(cm_ctx_t ctx, void* p) { return device_free(p, ctx ? ctx->stream : nullptr, ctx != nullptr); }
 
 int pinned(cm_ctx_t ctx, void** out)
 {
@@ -294,7 +379,7 @@ int cm_ctx_set_stream(cm_ctx_t ctx, void* stream)
 {
     CM_API_BEGIN
     CM_TRY(cm::resolve(ctx, &ctx));
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_TRY(::cm::host_sync(ctx));
     if (ctx->owns_stream && ctx->stream) CM_HIP(hipStreamDestroy(ctx->stream));
     ctx->stream = (hipStream_t)stream;
     ctx->owns_stream = false;
@@ -306,7 +391,7 @@ int cm_ctx_sync(cm_ctx_t ctx)
 {
     CM_API_BEGIN
     CM_TRY(cm::resolve(ctx, &ctx));
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_TRY(cm::host_sync(ctx));
     return CM_OK;
     CM_API_END
 }
@@ -317,7 +402,7 @@ int cm_malloc(cm_ctx_t ctx, size_t bytes, void** ptr)
     CM_REQUIRE(ptr, CM_E_ARG, "ptr is NULL");
     CM_TRY(cm::resolve(ctx, &ctx));
     if (bytes == 0) { *ptr = nullptr; return CM_OK; }
-    return cm::device_alloc(ctx->device, bytes, ptr);
+    return cm::device_alloc(ctx->device, bytes, ctx->stream, ptr);
     CM_API_END
 }
 
@@ -326,9 +411,8 @@ int cm_free(cm_ctx_t ctx, void* ptr)
     CM_API_BEGIN
     if (!ptr) return CM_OK;
     CM_TRY(cm::resolve(ctx, &ctx));
-    // the block may still be read by queued work on this stream
-    CM_HIP(hipStreamSynchronize(ctx->stream));
-    return cm::device_free(ptr);
+    // stream-ordered: queued work on this stream may still read the block
+    return cm::device_free(ptr, ctx->stream, true);
     CM_API_END
 }
 
@@ -338,8 +422,12 @@ int cm_memcpy_h2d(cm_ctx_t ctx, void* dst, const void* src, size_t bytes)
     CM_TRY(cm::resolve(ctx, &ctx));
     if (!bytes) return CM_OK;
     CM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    // pageable source: make the copy complete before the caller may reuse it
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    // a pageable source may be reused by the caller on return, so that copy
+    // is waited for; pinned host memory stays stream-ordered
+    hipPointerAttribute_t attr;
+    const bool pinned_src = hipPointerGetAttributes(&attr, src) == hipSuccess && attr.type == hipMemoryTypeHost;
+    (void)hipGetLastError();
+    if (!pinned_src) CM_TRY(cm::host_sync(ctx));
     return CM_OK;
     CM_API_END
 }
@@ -350,7 +438,7 @@ int cm_memcpy_d2h(cm_ctx_t ctx, void* dst, const void* src, size_t bytes)
     CM_TRY(cm::resolve(ctx, &ctx));
     if (!bytes) return CM_OK;
     CM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipStreamSynchronize(ctx->stream));
+    CM_TRY(cm::host_sync(ctx));
     return CM_OK;
     CM_API_END
 }

@@ -70,7 +70,9 @@ public:
         EvalCwiseSparse = CM_CNT_EVAL_CWISE_SPARSE,
         EvalReduction = CM_CNT_EVAL_REDUCTION,
         EvalMatmulSparse = CM_CNT_EVAL_MATMUL_SPARSE,
-        KernelLaunch = CM_CNT_KERNEL_LAUNCH
+        KernelLaunch = CM_CNT_KERNEL_LAUNCH,
+        // no counterpart in cuMat: host waits issued by the library
+        HostSync = CM_CNT_HOST_SYNC
     };
     static Profiling& instance()
     {

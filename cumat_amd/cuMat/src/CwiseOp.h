@@ -106,6 +106,8 @@ struct is_axpy_term<BinaryOp<L, R, F>> : std::integral_constant<bool, BinaryOp<L
 }  // namespace internal
 template <typename _Child, typename _Target> class CastingOp;
 template <typename _Scalar> class HostScalar;
+template <int _SparseFlags> struct SparsityPattern;
+template <typename _Child, int _SparseFlags> class SparseExpressionOp;
 
 template <typename _Derived>
 class MatrixBase {
@@ -205,6 +207,12 @@ public:
     template <typename _Other> auto dot(const MatrixBase<_Other>& rhs) const;
     auto squaredNorm() const;
     auto norm() const;
+
+    // the expression read only at the stored entries of 'pattern', as the
+    // matrix of a sparse product (sparseView,
+    // cuMat/src/SparseExpressionOpPlugin.inl:54-57); defined in SparseMatrix.h
+    template <int _SparseFlags>
+    SparseExpressionOp<_Derived, _SparseFlags> sparseView(const SparsityPattern<_SparseFlags>& pattern) const;
 };
 
 // ------------------------------------------------------------------ scalar

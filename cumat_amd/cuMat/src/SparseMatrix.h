@@ -254,17 +254,17 @@ public:
 
     // the stored values as an expression leaf evaluated at each stored entry
     // (SparseMatrix::direct(), cuMat/src/SparseMatrix.h:230-233); valid in
-    // assignments to a sparse matrix of the same pattern
+    // assignments to a sparse matrix of the same pattern and in sparse views
+    // of that pattern (sparseView(...) * x)
     StridedView<_Scalar> direct() const
     {
-        static_assert(_SparseFlags == SparseFlags::CSR, "direct() is CSR only");
         internal::LeafDesc d{};
         d.ptr = A_.data();
         d.dtype = internal::DType<_Scalar>::value;
         d.kind = CM_LEAF_STRIDED;
         d.sr = -1;  // marks the value array of the destination pattern
         d.sc = -1;
-        d.sb = nnz();
+        d.sb = nnz();  // entries per batch (CSR: nnz; ELLPACK: rows x nnzPerRow)
         d.rows = rows();
         d.cols = cols();
         d.batches = batches_;
@@ -305,6 +305,95 @@ public:
         return out;
     }
 };
+
+// A sparse matrix in an expression reads its stored values; on its own
+// pattern this equals cuMat's coeff() search (cuMat/src/SparseMatrix.h:
+// 135-142), the only pattern a sparse view of it may be evaluated on:
+// (2 * A).sparseView<CSR>(A.getSparsityPattern()) * b.
+template <typename S, typename _Scalar, int _Batches, int _SparseFlags, typename = internal::if_arith<S>>
+auto operator*(const S& s, const SparseMatrix<_Scalar, _Batches, _SparseFlags>& A)
+{
+    return (_Scalar)s * A.direct();
+}
+template <typename S, typename _Scalar, int _Batches, int _SparseFlags, typename = internal::if_arith<S>>
+auto operator*(const SparseMatrix<_Scalar, _Batches, _SparseFlags>& A, const S& s)
+{
+    return A.direct() * (_Scalar)s;
+}
+
+// expr.sparseView<CSR|ELLPACK>(pattern): the expression read only at the
+// stored entries of 'pattern' (SparseExpressionOp,
+// cuMat/src/SparseExpressionOp.h:32-72). Used as the left factor of a
+// product with a dense vector, evaluated by ONE cm_csr_expr_spmv /
+// cm_ell_expr_spmv launch that evaluates the expression inside the value
+// loads (no value array is written).
+template <typename _Child, int _SparseFlags>
+class SparseExpressionOp {
+    _Child e_;
+    SparsityPattern<_SparseFlags> pattern_;
+
+public:
+    typedef typename _Child::Scalar Scalar;
+    enum { SFlags = _SparseFlags };
+    SparseExpressionOp(const _Child& e, const SparsityPattern<_SparseFlags>& p) : e_(e), pattern_(p)
+    {
+        CUMAT_ASSERT_DIMENSION(e.rows() == p.rows || e.rows() == 1);
+        CUMAT_ASSERT_DIMENSION(e.cols() == p.cols || e.cols() == 1);
+    }
+    Index rows() const { return pattern_.rows; }
+    Index cols() const { return pattern_.cols; }
+    Index batches() const { return e_.batches(); }
+    const _Child& expression() const { return e_; }
+    const SparsityPattern<_SparseFlags>& getSparsityPattern() const { return pattern_; }
+
+    template <typename V>
+    ProductOp<SparseExpressionOp, V> operator*(const MatrixBase<V>& v) const
+    {
+        return ProductOp<SparseExpressionOp, V>(*this, v.derived());
+    }
+};
+
+template <typename _Derived>
+template <int _SparseFlags>
+SparseExpressionOp<_Derived, _SparseFlags> MatrixBase<_Derived>::sparseView(
+    const SparsityPattern<_SparseFlags>& pattern) const
+{
+    return SparseExpressionOp<_Derived, _SparseFlags>(derived(), pattern);
+}
+
+// y (mode)= view * x
+template <typename C, int F, typename X>
+void productEvalTo(const SparseExpressionOp<C, F>& A, const X& x, typename C::Scalar* y, Index batches,
+                   AssignmentMode mode)
+{
+    typedef typename C::Scalar S;
+    const auto& P = A.getSparsityPattern();
+    internal::ProgramBuilder b;
+    A.expression().emit(b, 0);
+    const Index R = P.rows, Cc = P.cols;
+    std::vector<cm_leaf> leaves(b.leaves.size() ? b.leaves.size() : 1);
+    for (size_t k = 0; k < b.leaves.size(); ++k) {
+        const internal::LeafDesc& l = b.leaves[k];
+        leaves[k].ptr = l.ptr;
+        leaves[k].dtype = l.dtype;
+        leaves[k].kind = l.kind;
+        leaves[k].sr = (l.rows == 1 && R != 1) ? 0 : l.sr;
+        leaves[k].sc = (l.cols == 1 && Cc != 1) ? 0 : l.sc;
+        leaves[k].sb = (l.batches == 1 && batches != 1) ? 0 : l.sb;
+    }
+    const int dt = internal::DType<S>::value;
+    if constexpr (F == SparseFlags::CSR) {
+        CUMAT_AMD_CALL(cm_csr_expr_spmv(Context::current().handle(), leaves.data(), (int)b.leaves.size(),
+                                        b.code.data(), (int)b.code.size(), dt, (int)R, (int)Cc, (int)P.nnz,
+                                        (int)batches, P.JA.data(), P.IA.data(), x.data(), x.batches() > 1 ? 1 : 0,
+                                        y, (int)mode));
+    } else {
+        CUMAT_AMD_CALL(cm_ell_expr_spmv(Context::current().handle(), leaves.data(), (int)b.leaves.size(),
+                                        b.code.data(), (int)b.code.size(), dt, (int)R, (int)Cc, (int)P.nnzPerRow,
+                                        (int)batches, P.indices.data(), x.data(), x.batches() > 1 ? 1 : 0, y,
+                                        (int)mode));
+    }
+}
 
 namespace internal {
 template <typename T>

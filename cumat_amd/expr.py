@@ -346,6 +346,9 @@ def assign(dst: DeviceArray, e, mode: int = capi.ASSIGN, ctx=None) -> DeviceArra
     if e.dtype != dst.dtype:
         raise TypeError(f"no implicit casting: expression dtype {e.dtype} into {dst.dtype}")
     leaves, nl, code, nc = lower(e, shape)
+    if any(leaves[i].sr == -1 for i in range(nl)):
+        raise TypeError("a sparse matrix's stored values are read only at its own entries: assign into "
+                        "the sparse matrix or use it through a sparse view, not in a dense assignment")
     sr, sc, sb = dst.strides()
     capi.call("cm_cwise_eval", ctx_handle(ctx if ctx is not None else dst.ctx), leaves, nl, code,
               nc, dst.ptr, dst.dtype, dst.rows, dst.cols, dst.batches, sr, sc, sb, mode)

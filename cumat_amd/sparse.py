@@ -9,7 +9,7 @@ import numpy as np
 
 from . import _capi as capi
 from .device import DeviceArray, ctx_handle, np_to_dtype
-from .expr import Expr, as_expr, lower
+from .expr import Expr, LeafNode, as_expr, lower
 
 
 class SparsityPattern:
@@ -106,9 +106,31 @@ class SparseMatrix:
     def direct(self):
         """Expression leaf over the stored values, evaluated at each non-zero
         (SparseMatrix::direct(), cuMat/src/SparseMatrix.h:230-233)."""
-        from .expr import LeafNode
-        return LeafNode(self.values, strides=(-1, -1, self.pattern.nnz),
-                        shape=(self.rows, self.cols, self.batches))
+        return SparseValueLeaf(self, self.pattern.nnz)
+
+    # A sparse matrix used in an expression reads its stored values; cuMat's
+    # coeff() search (cuMat/src/SparseMatrix.h:135-142) gives the same values
+    # wherever the view's pattern is the matrix's own, the only pattern the
+    # fused product accepts.
+    def _leaf(self):
+        return self.direct()
+
+    def __mul__(self, o):
+        if isinstance(o, DeviceArray):
+            return self.matvec(o)
+        return self._leaf() * o
+
+    def __rmul__(self, o): return o * self._leaf()
+    def __add__(self, o): return self._leaf() + _sparse_operand(o)
+    def __radd__(self, o): return _sparse_operand(o) + self._leaf()
+    def __sub__(self, o): return self._leaf() - _sparse_operand(o)
+    def __rsub__(self, o): return _sparse_operand(o) - self._leaf()
+    def __neg__(self): return -self._leaf()
+
+    def sparse_view(self, pattern=None):
+        """The matrix as a lazily evaluated sparse operand (its own pattern by
+        default)."""
+        return SparseExpressionView(self._leaf(), self.pattern if pattern is None else pattern)
 
     def assign(self, e, mode: int = capi.ASSIGN, ctx=None):
         """Evaluate a cwise expression only at the stored entries
@@ -133,6 +155,119 @@ class SparseMatrix:
         return out
 
 
+class SparseValueLeaf(LeafNode):
+    """Leaf over the stored values of a CSR or ELLPACK matrix, read at each
+    stored entry of the product's own pattern (direct access). Lowered as a
+    leaf with row stride -1 and batch stride = stored entries per batch."""
+
+    def __init__(self, matrix, per_batch: int):
+        super().__init__(matrix.values, strides=(-1, -1, per_batch),
+                         shape=(matrix.rows, matrix.cols, matrix.batches))
+        self.matrix = matrix
+
+
+def _sparse_operand(o):
+    return o._leaf() if isinstance(o, (SparseMatrix, ELLMatrix)) else o
+
+
+def _value_leaves(e):
+    """Every sparse value leaf of an expression tree."""
+    out = []
+    stack = [e]
+    while stack:
+        n = stack.pop()
+        if isinstance(n, SparseValueLeaf):
+            out.append(n)
+        for k in ("a", "b", "cond"):
+            c = getattr(n, k, None)
+            if isinstance(c, Expr):
+                stack.append(c)
+    return out
+
+
+class SparseExpressionView:
+    """expr.sparseView<CSR|ELLPACK>(pattern): a component-wise expression
+    read only at the stored entries of 'pattern' (a SparsityPattern for CSR,
+    an ELLMatrix for ELLPACK), used as the matrix of a product that
+    evaluates the expression inside its value loads -- one launch, no value
+    array written (SparseExpressionOp, cuMat/src/SparseExpressionOp.h:32-72;
+    ProductAssignment over it, cuMat/src/SparseProductEvaluation.h:144-187,
+    261-324)."""
+
+    def __init__(self, e, pattern):
+        e = as_expr(e)
+        if e.dtype not in (capi.F32, capi.F64, capi.I32):
+            raise TypeError("sparse-expression products take float32, float64 or int32 expressions")
+        self.expr = e
+        self.pattern = pattern
+        self.ell = isinstance(pattern, ELLMatrix)
+        if not self.ell and not isinstance(pattern, SparsityPattern):
+            raise TypeError("sparseView needs a SparsityPattern (CSR) or an ELLMatrix (ELLPACK pattern)")
+        R, Cc = pattern.rows, pattern.cols
+        if e.shape[0] not in (1, R) or e.shape[1] not in (1, Cc):
+            raise capi.CumatArgError(capi.CM_E_DIM, f"expression of shape {e.shape} viewed as {R}x{Cc}")
+        for leaf in _value_leaves(e):
+            own = leaf.matrix.pattern if not self.ell else leaf.matrix
+            if not self._same_pattern(own):
+                raise capi.CumatArgError(capi.CM_E_ARG, "a sparse matrix in the expression has another pattern "
+                                         "than the view (only the view's own pattern is read directly)")
+
+    def _same_pattern(self, other):
+        p = self.pattern
+        if other is p:
+            return True
+        if self.ell:
+            return (isinstance(other, ELLMatrix) and other.rows == p.rows and other.npr == p.npr
+                    and other.indices.ptr == p.indices.ptr)
+        return (isinstance(other, SparsityPattern) and other.rows == p.rows and other.nnz == p.nnz
+                and np.array_equal(other.host_JA, p.host_JA) and np.array_equal(other.host_IA, p.host_IA))
+
+    @property
+    def rows(self):
+        return self.pattern.rows
+
+    @property
+    def cols(self):
+        return self.pattern.cols
+
+    @property
+    def batches(self):
+        return self.expr.shape[2]
+
+    def matvec(self, x: DeviceArray, out: DeviceArray | None = None, mode: int = capi.ASSIGN,
+               ctx=None) -> DeviceArray:
+        p = self.pattern
+        if x.dtype != self.expr.dtype:
+            raise TypeError("no implicit casting between the expression and the vector")
+        if x.rows != p.cols or x.cols != 1:
+            raise capi.CumatArgError(capi.CM_E_DIM, f"SpMV: x is {x.rows}x{x.cols}, need {p.cols}x1")
+        batches = max(self.batches, x.batches)
+        if self.batches not in (1, batches) or x.batches not in (1, batches):
+            raise capi.CumatArgError(capi.CM_E_DIM, "SpMV: batch counts do not broadcast")
+        if out is None:
+            out = DeviceArray(p.rows, 1, batches, dtype=x.np_dtype, ctx=x.ctx)
+        if out.dtype != x.dtype or out.rows != p.rows or out.cols != 1 or out.batches != batches:
+            raise capi.CumatArgError(capi.CM_E_DIM, "SpMV: output has the wrong size or dtype")
+        leaves, nl, code, nc = lower(self.expr, (p.rows, p.cols, batches))
+        if self.ell:
+            capi.call("cm_ell_expr_spmv", ctx_handle(ctx), leaves, nl, code, nc, x.dtype, p.rows, p.cols,
+                      p.npr, batches, p.indices.ptr, x.ptr, int(x.batches > 1), out.ptr, mode)
+        else:
+            capi.call("cm_csr_expr_spmv", ctx_handle(ctx), leaves, nl, code, nc, x.dtype, p.rows, p.cols,
+                      p.nnz, batches, p.JA.ptr, p.IA.ptr, x.ptr, int(x.batches > 1), out.ptr, mode)
+        return out
+
+    __matmul__ = matmul = matvec
+
+    def __mul__(self, x):
+        return self.matvec(x)
+
+
+def sparse_view(e, pattern) -> SparseExpressionView:
+    """expr.sparseView<CSR|ELLPACK>(pattern) (cuMat/src/SparseExpressionOpPlugin.inl:54-57)."""
+    return SparseExpressionView(e, pattern)
+
+
 class ELLMatrix:
     """ELLPACK matrix: column-major rows x nnz_per_row index matrix, -1 padding."""
 
@@ -146,15 +281,64 @@ class ELLMatrix:
         # [-1, cols) here rather than let one reach the device
         if ind.size and (int(ind.min()) < -1 or int(ind.max()) >= self.cols):
             raise capi.CumatArgError(capi.CM_E_ARG, "ELLPACK index outside [-1, cols)")
-        self.indices = DeviceArray.from_numpy(np.asfortranarray(ind).reshape(-1, order="F"), ctx=ctx)
+        # values: rows x npr, or batches x rows x npr; stored as cuMat's
+        # DataMatrix (rows x npr x batches, column-major, batch stride rows*npr)
         v = np.asarray(values_rowsxk)
-        self.values = DeviceArray.from_numpy(np.asfortranarray(v).reshape(-1, order="F"), ctx=ctx)
+        if v.ndim == 2:
+            v = v[None]
+        if v.ndim != 3 or v.shape[1:] != ind.shape:
+            raise capi.CumatArgError(capi.CM_E_DIM, "ELLPACK values must be rows x npr or batches x rows x npr")
+        self.indices = DeviceArray.from_numpy(np.asfortranarray(ind).reshape(-1, order="F"), ctx=ctx)
+        self.batches = int(v.shape[0])
+        flat = np.concatenate([np.asfortranarray(v[k]).reshape(-1, order="F") for k in range(self.batches)])
+        self.values = DeviceArray.from_numpy(flat, rows=max(flat.size // self.batches, 1), cols=1,
+                                             batches=self.batches, ctx=ctx)
         self.dtype = self.values.dtype
 
     def matvec(self, x: DeviceArray, out: DeviceArray | None = None, mode=capi.ASSIGN, ctx=None):
+        """y (mode)= A * x (ELLPACKMVKernel_StaticBatches,
+        cuMat/src/SparseProductEvaluation.h:194-324). The kernel takes one
+        scalar type for values, x and y, so all three must agree; a batched
+        matrix (values of shape rows x npr x batches) and a batched x
+        broadcast against each other as in the CSR product."""
+        if x.dtype != self.dtype:
+            raise capi.CumatArgError(capi.CM_E_ARG, f"ELLPACK SpMV: x has dtype {x.np_dtype}, "
+                                     f"the matrix {np.dtype(self.values.np_dtype)}")
+        if x.rows != self.cols or x.cols != 1:
+            raise capi.CumatArgError(capi.CM_E_DIM, f"ELLPACK SpMV: x is {x.rows}x{x.cols}, need {self.cols}x1")
+        batches = max(self.batches, x.batches)
+        if self.batches not in (1, batches) or x.batches not in (1, batches):
+            raise capi.CumatArgError(capi.CM_E_DIM, "ELLPACK SpMV: batch counts do not broadcast")
         if out is None:
-            out = DeviceArray(self.rows, 1, x.batches, dtype=x.np_dtype, ctx=x.ctx)
+            out = DeviceArray(self.rows, 1, batches, dtype=x.np_dtype, ctx=x.ctx)
+        if out.dtype != self.dtype or out.rows != self.rows or out.cols != 1 or out.batches != batches:
+            raise capi.CumatArgError(capi.CM_E_DIM, "ELLPACK SpMV: output has the wrong size or dtype")
         capi.call("cm_ell_spmv", ctx_handle(ctx), self.dtype, self.rows, self.cols, self.npr,
-                  x.batches, self.indices.ptr, self.values.ptr, 0, x.ptr, int(x.batches > 1),
-                  out.ptr, mode)
+                  batches, self.indices.ptr, self.values.ptr, int(self.batches > 1), x.ptr,
+                  int(x.batches > 1), out.ptr, mode)
         return out
+
+    __matmul__ = matmul = matvec
+
+    def direct(self):
+        """The stored values as an expression leaf read at each slot of this
+        matrix's own pattern (column-major rows x npr data matrix)."""
+        return SparseValueLeaf(self, self.rows * self.npr)
+
+    def _leaf(self):
+        return self.direct()
+
+    def __mul__(self, o):
+        if isinstance(o, DeviceArray):
+            return self.matvec(o)
+        return self._leaf() * o
+
+    def __rmul__(self, o): return o * self._leaf()
+    def __add__(self, o): return self._leaf() + _sparse_operand(o)
+    def __radd__(self, o): return _sparse_operand(o) + self._leaf()
+    def __sub__(self, o): return self._leaf() - _sparse_operand(o)
+    def __rsub__(self, o): return _sparse_operand(o) - self._leaf()
+    def __neg__(self): return -self._leaf()
+
+    def sparse_view(self, pattern=None):
+        return SparseExpressionView(self._leaf(), self if pattern is None else pattern)

@@ -62,8 +62,26 @@ class EdgeLaplacian:
         self.ei, self.ej = t(P["ei"]), t(P["ej"])
         self.device = dev
 
+    def _check(self, name: str, t: torch.Tensor, last: int, batches: int | None = None,
+               dtype: torch.dtype | None = None) -> None:
+        """Every tensor handed to the kernels must be a (B, last) tensor of a
+        supported dtype on this operator's device: the kernels receive raw
+        pointers and derive every extent from n, nnz and nedges."""
+        if t.dtype not in _DT:
+            raise capi.CumatArgError(capi.CM_E_ARG, f"{name}: dtype {t.dtype} (need float32 or float64)")
+        if dtype is not None and t.dtype != dtype:
+            raise capi.CumatArgError(capi.CM_E_ARG, f"{name}: dtype {t.dtype}, expected {dtype}")
+        if t.device.type != "cuda" or (self.device.index is not None and t.device.index != self.device.index):
+            raise capi.CumatArgError(capi.CM_E_ARG, f"{name}: on {t.device}, the operator is on {self.device}")
+        if t.dim() != 2 or t.shape[1] != last or (batches is not None and t.shape[0] != batches):
+            want = f"({batches if batches is not None else 'B'}, {last})"
+            raise capi.CumatArgError(capi.CM_E_DIM, f"{name}: shape {tuple(t.shape)}, need {want}")
+        if t.shape[0] < 1:
+            raise capi.CumatArgError(capi.CM_E_DIM, f"{name}: no batches")
+
     def assemble(self, theta: torch.Tensor) -> torch.Tensor:
         """CSR values of A(theta) per batch: (B, nnz)."""
+        self._check("theta", theta, self.nedges)
         theta = theta.contiguous()
         B = theta.shape[0]
         vals = torch.empty(B, self.nnz, dtype=theta.dtype, device=theta.device)
@@ -75,6 +93,8 @@ class EdgeLaplacian:
     def solve(self, vals: torch.Tensor, rhs: torch.Tensor, tol: float, max_iter: int):
         """Per-batch Jacobi-PCG with batch-specific values; returns
         (x, iterations per batch, error per batch)."""
+        self._check("rhs", rhs, self.n)
+        self._check("vals", vals, self.nnz, batches=rhs.shape[0], dtype=rhs.dtype)
         vals, rhs = vals.contiguous(), rhs.contiguous()
         B = rhs.shape[0]
         x = torch.empty_like(rhs)
@@ -88,6 +108,8 @@ class EdgeLaplacian:
         return x, list(its), list(errs)
 
     def edge_grad(self, lam: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        self._check("x", x, self.n)
+        self._check("lam", lam, self.n, batches=x.shape[0], dtype=x.dtype)
         lam, x = lam.contiguous(), x.contiguous()
         B = x.shape[0]
         g = torch.empty(B, self.nedges, dtype=x.dtype, device=x.device)
@@ -102,6 +124,10 @@ class CGSolve(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, theta, b, lap: EdgeLaplacian, tol: float, max_iter: int):
+        # theta and b must agree in dtype and batch count: the solver reads
+        # one (B, nnz) block of values per right-hand side
+        lap._check("b", b, lap.n)
+        lap._check("theta", theta, lap.nedges, batches=b.shape[0], dtype=b.dtype)
         vals = lap.assemble(theta)
         x, its, errs = lap.solve(vals, b, tol, max_iter)
         ctx.save_for_backward(vals, x)

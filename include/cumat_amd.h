@@ -89,7 +89,10 @@ enum cm_counter {
     CM_CNT_EVAL_MATMUL_SPARSE = 6,
     CM_CNT_KERNEL_LAUNCH = 7,
     CM_CNT_EVAL_MATMUL = 8,
-    CM_CNT_COUNT = 9
+    /* no reference counterpart: host waits on a stream or event issued by the
+     * library (what a stream-ordered caller wants to see stay at zero) */
+    CM_CNT_HOST_SYNC = 9,
+    CM_CNT_COUNT = 10
 };
 
 typedef struct cm_context* cm_ctx_t;
@@ -110,10 +113,18 @@ int cm_ctx_destroy(cm_ctx_t ctx);
 int cm_ctx_get_stream(cm_ctx_t ctx, void** stream);
 int cm_ctx_set_stream(cm_ctx_t ctx, void* stream);
 int cm_ctx_sync(cm_ctx_t ctx);
+/* Caching allocation, like cuMat's use of cub::CachingDeviceAllocator
+ * (cuMat/src/Context.h:223-229,271-286,311-325): cm_free is stream-ordered
+ * and never waits. A freed block returns to the cache with an event recorded
+ * on the freeing context's stream; work on that stream may reuse it at once
+ * (stream order), and another stream reuses it only after that event (a
+ * device-side wait, or a block whose event has completed). */
 int cm_malloc(cm_ctx_t ctx, size_t bytes, void** ptr);
 int cm_free(cm_ctx_t ctx, void* ptr);
-/* h2d/d2d are stream-ordered; d2h synchronizes the stream like
- * Matrix::copyToHost (cuMat/src/Matrix.h:847-858). */
+/* d2d is stream-ordered; h2d is stream-ordered from pinned host memory and
+ * waits for the copy from pageable memory (the caller may reuse the buffer
+ * on return); d2h synchronizes the stream like Matrix::copyToHost
+ * (cuMat/src/Matrix.h:847-858). */
 int cm_memcpy_h2d(cm_ctx_t ctx, void* dst, const void* src, size_t bytes);
 int cm_memcpy_d2h(cm_ctx_t ctx, void* dst, const void* src, size_t bytes);
 int cm_memcpy_d2d(cm_ctx_t ctx, void* dst, const void* src, size_t bytes);
@@ -257,6 +268,37 @@ int cm_ell_spmv(cm_ctx_t ctx, int dtype, int rows, int cols, int nnz_per_row,
                 int batches, const int* indices, const void* vals,
                 int vals_batched, const void* x, int x_batched, void* y,
                 int mode);
+
+/* Product whose sparse matrix is a component-wise expression, evaluated
+ * lazily inside the product's value loads: ONE launch, no value array
+ * written. Replaces ProductAssignment over SparseExpressionOp<CSR>
+ * (cuMat/src/SparseProductEvaluation.h:144-187; SparseExpressionOp,
+ * cuMat/src/SparseExpressionOp.h:32-72; sparseView,
+ * cuMat/src/SparseExpressionOpPlugin.inl:54-57).
+ *   y[row + rows*b] (mode)= sum_k v(row, IA[k], b, k) * x[IA[k] + cols*b'']
+ * where v is the program of cm_cwise_eval evaluated at stored entry k of
+ * row 'row' in batch b; a leaf with sr == -1 is a value array of this
+ * pattern (SparseMatrix::direct(), cuMat/src/SparseMatrix.h:230-233), read
+ * at element k + b*sb; other leaves are dense strided leaves at
+ * (row, IA[k], b). The program's result dtype must equal 'dtype' (the type
+ * of x and y: F32, F64 or I32); mode is CM_ASSIGN, CM_ADD or CM_SUB.
+ * Rows are summed in CSR order. */
+int cm_csr_expr_spmv(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves,
+                     const cm_instr* code, int ncode, int dtype,
+                     int rows, int cols, int nnz, int batches,
+                     const int* JA, const int* IA,
+                     const void* x, int x_batched, void* y, int mode);
+
+/* The ELLPACK form (ProductAssignment over SparseExpressionOp<ELLPACK>,
+ * cuMat/src/SparseProductEvaluation.h:261-324): slot (row, ci) has column
+ * indices[row + rows*ci] (< 0: padding, skipped); a value leaf (sr == -1)
+ * is read at row + rows*ci + b*sb (the column-major rows x nnz_per_row
+ * data matrix); slots are added in order ci = 0, 1, ... */
+int cm_ell_expr_spmv(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves,
+                     const cm_instr* code, int ncode, int dtype,
+                     int rows, int cols, int nnz_per_row, int batches,
+                     const int* indices,
+                     const void* x, int x_batched, void* y, int mode);
 
 /* Dense matrix-vector product y = A x per batch, the matrix-vector case of
  * the dense ProductOp, which cuMat evaluates with cuBLAS gemm

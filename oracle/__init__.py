@@ -55,7 +55,8 @@ def lib():
         L.orc_poisson2d.restype = i64
         for name in ("orc_csr_spmv_f32", "orc_csr_spmv_f64", "orc_csr_spmv_i32"):
             getattr(L, name).argtypes = [i32, i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, i32]
-        L.orc_ell_spmv_f32.argtypes = [i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, i32]
+        for name in ("orc_ell_spmv_f32", "orc_ell_spmv_f64", "orc_ell_spmv_i32"):
+            getattr(L, name).argtypes = [i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, i32]
         L.orc_reduce.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, i32, vp]
         L.orc_dot.argtypes = [i32, i64, i32, vp, i32, vp, i32, vp]
         L.orc_cwise_binary.argtypes = [i32, i32, i64, vp, vp, vp]
@@ -99,6 +100,8 @@ def ref():
         L.eref_lincomb8_f32.argtypes = [i64, vp, vp, vp]
         L.eref_axis_sum_f32.argtypes = [i32, i32, i32, i32, vp, vp]
         L.eref_threads_max.restype = i32
+        L.eref_batched_pcg_f32.argtypes = [i32, vp, vp, vp, vp, vp, vp, f64, i32, i32, i32, vp]
+        L.eref_batched_pcg_f32.restype = i32
         _ref = L
     return _ref
 
@@ -129,13 +132,26 @@ def csr_spmv(JA, IA, vals, x, rows, cols, batches=1, vals_batched=False, x_batch
     return y
 
 
+def ell_spmv(indices_colmajor, vals, x, rows, cols, nnz_per_row, batches=1,
+             vals_batched=False, x_batched=False, y=None, mode=0):
+    """ELLPACK product in the dtype of vals (float32, float64 or int32)."""
+    dt = np.dtype(vals.dtype)
+    fn = {np.dtype(np.float32): "orc_ell_spmv_f32", np.dtype(np.float64): "orc_ell_spmv_f64",
+          np.dtype(np.int32): "orc_ell_spmv_i32"}[dt]
+    if y is None:
+        y = np.zeros(rows * batches, dt)
+    ind = np.ascontiguousarray(indices_colmajor, np.int32)
+    vals = np.ascontiguousarray(vals)
+    x = np.ascontiguousarray(x, dt)
+    getattr(lib(), fn)(rows, cols, nnz_per_row, batches, _p(ind), _p(vals),
+                       int(vals_batched), _p(x), int(x_batched), _p(y), mode)
+    return y
+
+
 def ell_spmv_f32(indices_colmajor, vals, x, rows, cols, nnz_per_row, batches=1,
                  vals_batched=False, x_batched=False, y=None, mode=0):
-    if y is None:
-        y = np.zeros(rows * batches, np.float32)
-    lib().orc_ell_spmv_f32(rows, cols, nnz_per_row, batches, _p(indices_colmajor), _p(vals),
-                           int(vals_batched), _p(x), int(x_batched), _p(y), mode)
-    return y
+    return ell_spmv(indices_colmajor, np.asarray(vals, np.float32), x, rows, cols, nnz_per_row, batches,
+                    vals_batched, x_batched, y, mode)
 
 
 def reduce(a, op, axis, rows, cols, batches, in_row_major, out_row_major):

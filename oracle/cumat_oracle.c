@@ -188,6 +188,48 @@ void orc_ell_spmv_f32(int rows, int cols, int nnz_per_row, int batches,
     }
 }
 
+void orc_ell_spmv_f64(int rows, int cols, int nnz_per_row, int batches,
+                      const int32_t* indices, const double* vals, int vals_batched,
+                      const double* x, int x_batched, double* y, int mode)
+{
+    /* cuMat/src/SparseProductEvaluation.h:194-226, as orc_ell_spmv_f32 */
+    for (int b = 0; b < batches; ++b) {
+        const double* v = vals + (vals_batched ? (int64_t)b * rows * nnz_per_row : 0);
+        const double* xb = x + (x_batched ? (int64_t)b * cols : 0);
+        for (int r = 0; r < rows; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < nnz_per_row; ++k) {
+                int c = indices[r + (int64_t)rows * k];
+                if (c < 0) continue;
+                acc += v[r + (int64_t)rows * k] * xb[c];
+            }
+            double* yy = &y[r + (int64_t)rows * b];
+            *yy = assign_f64(mode, *yy, acc);
+        }
+    }
+}
+
+void orc_ell_spmv_i32(int rows, int cols, int nnz_per_row, int batches,
+                      const int32_t* indices, const int32_t* vals, int vals_batched,
+                      const int32_t* x, int x_batched, int32_t* y, int mode)
+{
+    /* cuMat/src/SparseProductEvaluation.h:194-226, int32 arithmetic wraps */
+    for (int b = 0; b < batches; ++b) {
+        const int32_t* v = vals + (vals_batched ? (int64_t)b * rows * nnz_per_row : 0);
+        const int32_t* xb = x + (x_batched ? (int64_t)b * cols : 0);
+        for (int r = 0; r < rows; ++r) {
+            uint32_t acc = 0;
+            for (int k = 0; k < nnz_per_row; ++k) {
+                int c = indices[r + (int64_t)rows * k];
+                if (c < 0) continue;
+                acc += (uint32_t)v[r + (int64_t)rows * k] * (uint32_t)xb[c];
+            }
+            int32_t* yy = &y[r + (int64_t)rows * b];
+            *yy = assign_i32(mode, *yy, (int32_t)acc);
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* reductions                                                          */
 /* ------------------------------------------------------------------ */

@@ -41,6 +41,12 @@ void orc_csr_spmv_i32(int rows, int cols, int nnz, int batches,
 void orc_ell_spmv_f32(int rows, int cols, int nnz_per_row, int batches,
                       const int32_t* indices, const float* vals, int vals_batched,
                       const float* x, int x_batched, float* y, int mode);
+void orc_ell_spmv_f64(int rows, int cols, int nnz_per_row, int batches,
+                      const int32_t* indices, const double* vals, int vals_batched,
+                      const double* x, int x_batched, double* y, int mode);
+void orc_ell_spmv_i32(int rows, int cols, int nnz_per_row, int batches,
+                      const int32_t* indices, const int32_t* vals, int vals_batched,
+                      const int32_t* x, int x_batched, int32_t* y, int mode);
 
 /* ---- reductions (cuMat/src/ReductionOps.h:57-160 axis semantics,
  * functors cuMat/src/ReductionOps.h:859-968, initial values

@@ -17,6 +17,7 @@
 #include <Eigen/IterativeLinearSolvers>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 namespace {
 template <typename S>
@@ -154,5 +155,37 @@ void eref_axis_sum_f32(int axis, int rows, int cols, int batches, const float* i
 }
 
 int eref_threads_max(void) { return Eigen::nbThreads(); }
+
+// The batched workload as the CPU would run it: Eigen's ConjugateGradient on
+// every system of a block-diagonal CSR (system s owns rows
+// [row_off[s], row_off[s+1]), global column indices), the systems spread over
+// 'nthreads' OpenMP threads (each solve single-threaded). 'stride' > 1 solves
+// only every stride-th system (a bounded sample). Returns the systems solved.
+int eref_batched_pcg_f32(int nsys, const int* row_off, const int* JA, const int* IA, const float* vals,
+                         const float* b, float* x, double tol, int max_iter, int nthreads, int stride,
+                         int* iters)
+{
+    if (stride < 1) stride = 1;
+    const int nsolve = (nsys + stride - 1) / stride;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+    for (int q = 0; q < nsolve; ++q) {
+        const int s = q * stride;
+        const int lo = row_off[s], hi = row_off[s + 1], n = hi - lo;
+        const int k0 = JA[lo], nnz = JA[hi] - k0;
+        std::vector<int> ja(n + 1), ia(nnz);
+        for (int i = 0; i <= n; ++i) ja[i] = JA[lo + i] - k0;
+        for (int k = 0; k < nnz; ++k) ia[k] = IA[k0 + k] - lo;
+        SpRow<float> A = make_matrix<float>(n, n, nnz, ja.data(), ia.data(), vals + k0);
+        Eigen::ConjugateGradient<SpRow<float>, Eigen::Lower | Eigen::Upper, Eigen::DiagonalPreconditioner<float>> cg;
+        cg.setTolerance((float)tol);
+        cg.setMaxIterations(max_iter);
+        cg.compute(A);
+        Eigen::Map<const Eigen::VectorXf> bb(b + lo, n);
+        Eigen::Map<Eigen::VectorXf> xx(x + lo, n);
+        xx = cg.solve(bb);
+        if (iters) iters[s] = (int)cg.iterations();
+    }
+    return nsolve;
+}
 
 }  // extern "C"

@@ -148,6 +148,42 @@ TEST_CASE("ELLPACK Matrix-Vector Product")
     VectorXf b = VectorXf::fromHost(std::vector<float>{2, 5, 3, -4, 1});
     VectorXf y = A * b;
     requireVector({22, 19, -10, 33}, y);
+    // ELLPACK MatrixExpression-Vector Product (tests/TestSparseMultOp.cu:329-375)
+    Profiling::instance().resetAll();
+    VectorXf y2 = (2 * A.direct()).sparseView<SparseFlags::ELLPACK>(A.getSparsityPattern()) * b;
+    REQUIRE(Profiling::instance().get(Profiling::EvalAny) == 1);
+    REQUIRE(Profiling::instance().get(Profiling::EvalMatmulSparse) == 1);
+    requireVector({44, 38, -20, 66}, y2);
+}
+
+TEST_CASE("CSR MatrixExpression-Vector Product")
+{
+    // tests/TestSparseMultOp.cu:104-153: sections "sparse matmul with search"
+    // and "sparse matmul with direct access", one evaluation each
+    SPattern pattern = fixturePattern();
+    SMatrix_t A(pattern);
+    A.getData().slice(0) = VectorXf::fromHost(std::vector<float>{1, 4, 2, 3, 5, 7, 8, 9, 6});
+    VectorXf b = VectorXf::fromHost(std::vector<float>{2, 5, 3, -4, 1});
+    Profiling::instance().resetAll();
+    VectorXf x2 = (2 * A).sparseView<SparseFlags::CSR>(A.getSparsityPattern()) * b;
+    REQUIRE(Profiling::instance().get(Profiling::EvalAny) == 1);
+    REQUIRE(Profiling::instance().get(Profiling::EvalMatmulSparse) == 1);
+    requireVector({44, 38, -20, 66}, x2);
+    Profiling::instance().resetAll();
+    VectorXf x3 = (2 * A.direct()).sparseView<SparseFlags::CSR>(A.getSparsityPattern()) * b;
+    REQUIRE(Profiling::instance().get(Profiling::EvalAny) == 1);
+    REQUIRE(Profiling::instance().get(Profiling::EvalMatmulSparse) == 1);
+    requireVector({44, 38, -20, 66}, x3);
+    // an expression over two matrices of the pattern, a row-dependent and a
+    // column-dependent vector: v = A + B - r(row) + c(col), x3 += view * b
+    SMatrix_t B(pattern);
+    B.getData().slice(0) = VectorXf::fromHost(std::vector<float>{1, 1, 1, 1, 1, 1, 1, 1, 1});
+    VectorXf r = VectorXf::fromHost(std::vector<float>{1, 2, 3, 4});
+    RowVectorXf c = RowVectorXf::fromHost(std::vector<float>{0, 1, 0, 1, 0});
+    x3 += (A.direct() + B.direct() - r + c).sparseView<SparseFlags::CSR>(pattern) * b;
+    // row 0: (1+1-1+0)*2 + (4+1-1+1)*5 = 27; row 1: (2+1-2+1)*5 + (3+1-2+0)*3 = 16
+    // row 2: (5+1-3+0)*2 + (7+1-3+1)*-4 + (8+1-3+0)*1 = -12; row 3: (9+1-4+0)*3 + (6+1-4+0)*1 = 21
+    requireVector({44 + 27, 38 + 16, -20 - 12, 66 + 21}, x3);
 }
 
 TEST_CASE("Sparse-Cwise and compound operations on CSR values")

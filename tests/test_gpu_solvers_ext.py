@@ -207,3 +207,105 @@ def test_large_batched_solve_x_update_in_p_kernel(gpu, orc, per_batch):
         for k in range(B):
             r = orc.csr_spmv(JA, IA, vals[k * IA.size:(k + 1) * IA.size], x[k * n:(k + 1) * n], n, n)
             assert np.linalg.norm(b[k * n:(k + 1) * n] - r) / np.linalg.norm(b[k * n:(k + 1) * n]) <= 2 * tol
+
+
+def _check_batched(orc, P, xh, ih, eh, tol=1e-6):
+    ro = P["row_off"]
+    for s in range(ro.size - 1):
+        lo, hi = ro[s], ro[s + 1]
+        n = hi - lo
+        JA = P["JA"][lo:hi + 1] - P["JA"][lo]
+        IA = P["IA"][P["JA"][lo]:P["JA"][hi]] - lo
+        V = P["vals"][P["JA"][lo]:P["JA"][hi]]
+        x64, _, _ = orc.pcg(JA, IA, V.astype(np.float64), P["b"][lo:hi].astype(np.float64), n, tol=1e-13,
+                            max_iter=5000)
+        assert eh[s] <= tol and ih[s] > 0, (s, ih[s], eh[s])
+        assert np.linalg.norm(xh[lo:hi] - x64) / np.linalg.norm(x64) <= 1e-5, s
+
+
+def test_batched_graph_cache_same_buffers_larger_systems(gpu, orc):
+    """Two graph-path calls on the same device buffers: small systems first,
+    then larger ones in the same block-size bucket (default block size 1024
+    for both). The second call must not replay the first call's captured
+    launch (its register tile and LDS size fit only the small systems)."""
+    cm = gpu
+    from cumat_amd.problems import batched_poisson
+    nsys = 40
+    small = batched_poisson(nsys, 91, 92, seed=71)   # 8281..8464 rows
+    large = batched_poisson(nsys, 118, 120, seed=72)  # 13924..14400 rows
+    cap_rows = int(large["row_off"][-1])
+    cap_nnz = int(large["IA"].size)
+    bufs = {"row_off": cm.DeviceArray(nsys + 1, dtype=np.int32), "JA": cm.DeviceArray(cap_rows + 1, dtype=np.int32),
+            "IA": cm.DeviceArray(cap_nnz, dtype=np.int32), "vals": cm.DeviceArray(cap_nnz, dtype=np.float32),
+            "b": cm.DeviceArray(cap_rows, dtype=np.float32)}
+    x = cm.DeviceArray(cap_rows, dtype=np.float32)
+    its = cm.DeviceArray(nsys, dtype=np.int32)
+    errs = cm.DeviceArray(nsys, dtype=np.float32)
+    for P in (small, large):
+        nrows, nnz = int(P["row_off"][-1]), int(P["IA"].size)
+        for k in bufs:
+            a = np.ascontiguousarray(P[k])
+            cm.capi.call("cm_memcpy_h2d", None, bufs[k].ptr, a.ctypes.data, a.nbytes)
+        # views of the leading parts of the same buffers (same addresses)
+        view = {k: cm.DeviceArray(np.asarray(P[k]).size, dtype=np.asarray(P[k]).dtype, ptr=bufs[k].ptr)
+                for k in bufs}
+        xv = cm.DeviceArray(nrows, dtype=np.float32, ptr=x.ptr)
+        xv.fill_zero()
+        cm.pcg_batched(view["row_off"], view["JA"], view["IA"], view["vals"], view["b"], xv, 1e-6, 3000, its, errs,
+                       use_graph=True)
+        assert nnz == view["IA"].size
+        _check_batched(orc, P, xv.numpy(), its.numpy(), errs.numpy())
+
+
+def test_batched_rejects_columns_outside_own_system(gpu):
+    """A system whose column index points into another system's rows is not
+    solved (the device reports iterations -1) and the call fails with an
+    argument error; the other systems are still solved."""
+    cm = gpu
+    from cumat_amd.problems import batched_poisson
+    P = batched_poisson(6, 10, 14, seed=3)
+    IA = P["IA"].copy()
+    ro, JA = P["row_off"], P["JA"]
+    IA[JA[ro[2]] + 1] = ro[3] + 1  # system 2 reaches into system 3
+    d = {k: cm.DeviceArray.from_numpy(P[k]) for k in ("row_off", "JA", "vals", "b")}
+    dIA = cm.DeviceArray.from_numpy(IA)
+    x = cm.DeviceArray(P["b"].size, dtype=np.float32)
+    its = cm.DeviceArray(6, dtype=np.int32)
+    errs = cm.DeviceArray(6, dtype=np.float32)
+    for g in (False, True):
+        with pytest.raises(cm.capi.CumatArgError):
+            cm.pcg_batched(d["row_off"], d["JA"], dIA, d["vals"], d["b"], x, 1e-6, 500, its, errs, use_graph=g)
+        ih = its.numpy()
+        assert ih[2] == -1 and (np.delete(ih, 2) > 0).all()
+
+
+def test_stream_ordered_frees_issue_no_host_sync(gpu):
+    """Temporaries of an expression chain return to the cache without a host
+    wait (stream-ordered frees); reuse of a freed block on the same stream is
+    ordered by the stream, so the results stay right."""
+    cm = gpu
+    n = 1 << 20
+    rng = np.random.default_rng(5)
+    a_h = rng.standard_normal(n).astype(np.float32)
+    b_h = rng.standard_normal(n).astype(np.float32)
+    a, b = cm.DeviceArray.from_numpy(a_h), cm.DeviceArray.from_numpy(b_h)
+    out = cm.DeviceArray(n, dtype=np.float32)
+    acc = cm.DeviceArray(n, dtype=np.float32)
+    acc.fill_zero()
+    cm.synchronize()
+    cm.reset_counters()
+    for k in range(20):
+        t = cm.DeviceArray(n, dtype=np.float32)       # cached block, reused every pass
+        cm.assign(t, a.expr() * float(k) + b.expr())
+        cm.assign(acc, acc.expr() + t.expr())
+        d = cm.DeviceArray(1, dtype=np.float32)
+        cm.dot(t, a, out=d)
+        t.free()                                      # stream-ordered: no wait
+        d.free()
+    cm.assign(out, acc.expr() * 1.0)
+    c = cm.counters()
+    assert c["HostSync"] == 0, c
+    assert c["DeviceMemFree"] == 40
+    exp = sum(a_h.astype(np.float64) * k + b_h for k in range(20))
+    got = out.numpy()
+    assert np.abs(got - exp).max() <= 1e-4 * np.abs(exp).max()

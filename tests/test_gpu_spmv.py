@@ -295,3 +295,115 @@ def test_csr_cwise_eval_batched_column_leaf(gpu):
     S.assign(S.direct() - R1.expr(), mode=cm.capi.ADD)
     exp = exp + (exp - r1[IA][None, :])
     np.testing.assert_array_equal(S.values.numpy().reshape(2, -1), exp)
+
+
+# ------------------------------------------------------------ ELLPACK
+def _ell_from_csr(JA, IA, V, width):
+    """rows x width ELLPACK index / value matrices of a CSR matrix (entries in
+    CSR order, -1 padding with value 666 as in the reference's fixture)."""
+    n = JA.size - 1
+    ind = -np.ones((n, width), np.int32)
+    val = np.full((n, width), 666, V.dtype)
+    cnt = np.diff(JA)
+    slot = np.arange(IA.size) - np.repeat(JA[:-1], cnt)
+    rows = np.repeat(np.arange(n), cnt)
+    ind[rows, slot] = IA
+    val[rows, slot] = V
+    return ind, val
+
+
+@pytest.mark.parametrize("case", ["matrix", "vector", "both"])
+def test_ell_golden_batched(gpu, case):
+    """tests/TestSparseMultOp.cu:377-519: the 4x5 fixture with a batched
+    matrix (scales 1,2,3), a batched vector (1,2,3) or both (1,2,-1 x 1,2,3)."""
+    cm = gpu
+    g = GOLD["ell_4x5"]
+    c = GOLD["csr_4x5"]
+    if case == "matrix":
+        ms, vs, es = c["batched_matrix_scales"], [1], c["batched_matrix_expected_scales"]
+    elif case == "vector":
+        ms, vs, es = [1], c["batched_vector_scales"], c["batched_vector_expected_scales"]
+    else:
+        ms, vs, es = c["batched_both_matrix_scales"], c["batched_both_vector_scales"], c["batched_both_expected_scales"]
+    vals = np.stack([np.array(g["values_rows"], np.float32) * s for s in ms])
+    E = cm.ELLMatrix(4, 5, g["indices_rows"], vals)
+    b = np.concatenate([np.array(g["b"], np.float32) * s for s in vs])
+    cm.reset_counters()
+    y = E.matvec(cm.DeviceArray.from_numpy(b, rows=5, cols=1, batches=len(vs))).numpy()
+    cnt = cm.counters()
+    assert cnt["EvalAny"] == 1 and cnt["EvalMatmulSparse"] == 1
+    exp = np.concatenate([np.array(g["y"], np.float32) * s for s in es])
+    assert np.array_equal(y, exp)
+
+
+@pytest.mark.parametrize("dims,width", [((300, 300), 5), ((40, 40, 40), 7)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.int32])
+def test_ell_poisson_width5_width7_bitexact(gpu, orc, dims, width, dtype):
+    """2D 5-point and 3D 7-point Laplacians in ELLPACK (the four-entry main
+    loop and the tail both run): integer-valued x and integer coefficients
+    make every product and sum exact, so the GPU equals the oracle bit for
+    bit; batched matrix and vector, and the += / -= modes."""
+    cm = gpu
+    from cumat_amd.problems import grid_laplacian
+    rng = np.random.default_rng(width)
+    P = grid_laplacian(dims, lambda s: rng.integers(1, 5, s).astype(np.float64), edges=False)
+    n = P["n"]
+    ind, v0 = _ell_from_csr(P["JA"], P["IA"], P["vals"], width)
+    B = 3
+    vals = np.stack([(v0 * (k + 1)).astype(dtype) for k in range(B)])
+    E = cm.ELLMatrix(n, n, ind, vals)
+    x = rng.integers(-20, 21, n * B).astype(dtype)
+    xd = cm.DeviceArray.from_numpy(x, rows=n, cols=1, batches=B)
+    ind_cm = np.asfortranarray(ind).reshape(-1, order="F")
+    vals_cm = np.concatenate([np.asfortranarray(vals[k]).reshape(-1, order="F") for k in range(B)])
+    y = E.matvec(xd).numpy()
+    ref = orc.ell_spmv(ind_cm, vals_cm, x, n, n, width, batches=B, vals_batched=True, x_batched=True)
+    assert np.array_equal(y, ref)
+    # agrees with the CSR product of the same matrix
+    Acsr = cm.SparseMatrix.from_csr(n, n, P["JA"], P["IA"], P["vals"].astype(dtype))
+    x0 = cm.DeviceArray.from_numpy(x[:n].copy())
+    assert np.array_equal(Acsr.matvec(x0).numpy(), y[:n])
+    # compound modes, batched vector against a one-batch matrix
+    E1 = cm.ELLMatrix(n, n, ind, vals[0])
+    y0 = rng.integers(-100, 101, n * B).astype(dtype)
+    for mode in (cm.capi.ADD, cm.capi.SUB):
+        yd = cm.DeviceArray.from_numpy(y0.copy(), rows=n, cols=1, batches=B)
+        E1.matvec(xd, out=yd, mode=mode)
+        ref = orc.ell_spmv(ind_cm, vals_cm[:n * width], x, n, n, width, batches=B, vals_batched=False,
+                           x_batched=True, y=y0.copy(), mode=mode)
+        assert np.array_equal(yd.numpy(), ref), mode
+
+
+def test_ell_argument_checks(gpu):
+    cm = gpu
+    g = GOLD["ell_4x5"]
+    E = cm.ELLMatrix(4, 5, g["indices_rows"], np.array(g["values_rows"], np.float32))
+    with pytest.raises(cm.capi.CumatArgError):
+        E.matvec(cm.DeviceArray.from_numpy(np.array(g["b"], np.float64)))  # dtype mismatch
+    with pytest.raises(cm.capi.CumatArgError):
+        E.matvec(cm.DeviceArray.from_numpy(np.ones(4, np.float32)))  # x too short
+    with pytest.raises(cm.capi.CumatArgError):
+        E.matvec(cm.DeviceArray.from_numpy(np.array(g["b"], np.float32)), out=cm.DeviceArray(3, dtype=np.float32))
+
+
+# ------------------------------------------------------------ Jacobi setup guard
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_csr_inverse_diagonal_missing_and_zero(gpu, dtype):
+    """invdiag = A_ii == 0 ? 1 : 1/A_ii (cwiseInverseCheck,
+    cuMat/src/UnaryOps.h:177): a row with no stored diagonal and a row whose
+    stored diagonal is 0 both give 1; the others give 1/A_ii exactly."""
+    cm = gpu
+    # row 0: diag 4; row 1: no diagonal entry; row 2: stored 0 on the
+    # diagonal; row 3: diag -2; row 4: empty
+    JA = np.array([0, 2, 4, 6, 8, 8], np.int32)
+    IA = np.array([0, 1, 0, 2, 1, 2, 0, 3], np.int32)
+    V = np.array([4, -1, -1, 5, 7, 0, 3, -2], dtype)
+    A = cm.SparseMatrix.from_csr(5, 5, JA, IA, V)
+    inv = A.inverse_diagonal().numpy()
+    exp = np.array([0.25, 1.0, 1.0, -0.5, 1.0], dtype)
+    assert np.array_equal(inv, exp)
+    # batched values: batch 1 has the diagonal scaled by 2
+    V2 = np.concatenate([V, V * 2])
+    A2 = cm.SparseMatrix(cm.SparsityPattern(5, 5, JA, IA), V2, batches=2)
+    inv2 = A2.inverse_diagonal().numpy()
+    assert np.array_equal(inv2, np.concatenate([exp, np.array([0.125, 1.0, 1.0, -0.25, 1.0], dtype)]))

@@ -168,3 +168,42 @@ def test_ell_pattern_indices_checked_before_upload():
     for bad in ([[0, 2], [1, -1]], [[0, -2], [1, 0]]):
         with pytest.raises(capi.CumatArgError):
             ELLMatrix(2, 2, bad, [[1.0, 1.0], [1.0, 1.0]])
+
+
+def test_torch_cg_rejects_mismatched_arguments():
+    """CGSolve / EdgeLaplacian check dtype, batch count, shape and device
+    before any pointer reaches the kernels (here on CPU tensors, so no call
+    is made at all)."""
+    torch = pytest.importorskip("torch")
+    from cumat_amd.problems import grid_laplacian
+    from cumat_amd.torch_cg import EdgeLaplacian, cg_solve
+    P = grid_laplacian((5, 6), lambda s: np.ones(s))
+    lap = EdgeLaplacian(P, "cpu")
+    n, E = P["n"], P["nedges"]
+    th = torch.ones(2, E, dtype=torch.float64)
+    b = torch.ones(2, n, dtype=torch.float64)
+    cases = [
+        (th.float(), b),                                        # theta float32, b float64
+        (th[:1], b),                                            # 1 batch of theta, 2 of b
+        (th, torch.ones(2, n + 1, dtype=torch.float64)),        # rhs of the wrong length
+        (th[:, :-1], b),                                        # theta of the wrong length
+        (th.to(torch.int32), b.to(torch.int32)),                # unsupported dtype
+        (th, b),                                                # CPU tensors (not a GPU device)
+    ]
+    for t, bb in cases:
+        with pytest.raises(capi.CumatArgError):
+            cg_solve(t, bb, lap)
+    with pytest.raises(capi.CumatArgError):
+        lap.edge_grad(torch.ones(2, n, dtype=torch.float32), torch.ones(2, n, dtype=torch.float64))
+
+
+def test_ell_host_checks():
+    """ELLMatrix rejects index / value shapes and out-of-range indices before
+    any upload."""
+    from cumat_amd.sparse import ELLMatrix
+    with pytest.raises(capi.CumatArgError):
+        ELLMatrix(3, 4, [[0, 1], [1, 5], [2, -1]], np.ones((3, 2), np.float32))   # index 5 >= cols
+    with pytest.raises(capi.CumatArgError):
+        ELLMatrix(3, 4, [[0, 1], [1, 2], [2, -2]], np.ones((3, 2), np.float32))   # index < -1
+    with pytest.raises(capi.CumatArgError):
+        ELLMatrix(3, 4, [[0, 1], [1, 2], [2, -1]], np.ones((3, 3), np.float32))   # values of another shape

@@ -214,3 +214,73 @@ def _fixed(theta):
 def _bil(P, lam, x):
     rows = np.repeat(np.arange(P["n"]), np.diff(P["JA"]))
     return float(lam @ np.bincount(rows, P["vals"] * x[P["IA"]], minlength=P["n"]))
+
+
+def test_spmv_i32_golden_and_closed_form(orc):
+    """orc_csr_spmv_i32: the reference's 4x5 fixture (tests/TestSparseMultOp.cu:
+    70-276) in int32, batched both ways, and the integer closed form of the
+    Dirichlet Poisson matrix: A * 1 = number of missing neighbours per row."""
+    g = GOLD["csr_4x5"]
+    JA, IA = np.array(g["JA"], np.int32), np.array(g["IA"], np.int32)
+    V = np.array(g["vals"], np.int32)
+    b = np.array(g["b"], np.int32)
+    assert list(orc.csr_spmv(JA, IA, V, b, 4, 5)) == [22, 19, -10, 33]
+    V3 = np.concatenate([V * s for s in g["batched_both_matrix_scales"]]).astype(np.int32)
+    b3 = np.concatenate([b * s for s in g["batched_both_vector_scales"]]).astype(np.int32)
+    y3 = orc.csr_spmv(JA, IA, V3, b3, 4, 5, batches=3, vals_batched=True, x_batched=True)
+    exp = np.concatenate([np.array(g["y"]) * s for s in g["batched_both_expected_scales"]])
+    assert np.array_equal(y3, exp.astype(np.int32))
+    # compound modes (+=, -=) on the fixture
+    y0 = np.array([1, 2, 3, 4], np.int32)
+    assert list(orc.csr_spmv(JA, IA, V, b, 4, 5, y=y0.copy(), mode=1)) == [23, 21, -7, 37]
+    assert list(orc.csr_spmv(JA, IA, V, b, 4, 5, y=y0.copy(), mode=2)) == [-21, -17, 13, -29]
+    # a transposed operand (columns read as rows) must give another answer
+    assert not np.array_equal(orc.csr_spmv(JA, IA, V, b[::-1].copy(), 4, 5), np.array(g["y"], np.int32))
+    for gg in (1, 3, 17):
+        from cumat_amd.problems import poisson2d
+        JAp, IAp, Vp = poisson2d(gg, 0.0, np.float64)
+        n = gg * gg
+        y1 = orc.csr_spmv(JAp, IAp, Vp.astype(np.int32), np.ones(n, np.int32), n, n)
+        assert np.array_equal(y1, (5 - np.diff(JAp)).astype(np.int32))
+    # int32 arithmetic wraps like the device's
+    big = orc.csr_spmv(np.array([0, 2], np.int32), np.array([0, 1], np.int32), np.array([2**30, 2**30], np.int32),
+                       np.array([2, 1], np.int32), 1, 2)
+    assert big[0] == np.int32(-2**30)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.int32])
+def test_ell_golden_all_dtypes(orc, dtype):
+    """orc_ell_spmv_{f32,f64,i32}: the reference's ELLPACK fixture
+    (tests/TestSparseMultOp.cu:278-519, padding value 666 never read), a
+    batched matrix with scales 1,2,3, and the += mode."""
+    e = GOLD["ell_4x5"]
+    ind = np.asfortranarray(np.array(e["indices_rows"], np.int32)).reshape(-1, order="F")
+    val = np.asfortranarray(np.array(e["values_rows"], dtype)).reshape(-1, order="F")
+    b = np.array(e["b"], dtype)
+    assert np.array_equal(orc.ell_spmv(ind, val, b, 4, 5, 3), np.array(e["y"], dtype))
+    v3 = np.concatenate([val * s for s in (1, 2, 3)]).astype(dtype)
+    y3 = orc.ell_spmv(ind, v3, b, 4, 5, 3, batches=3, vals_batched=True)
+    assert np.array_equal(y3, np.concatenate([np.array(e["y"], dtype) * s for s in (1, 2, 3)]))
+    y0 = np.array([1, 1, 1, 1], dtype)
+    assert np.array_equal(orc.ell_spmv(ind, val, b, 4, 5, 3, y=y0, mode=1), np.array(e["y"], dtype) + 1)
+
+
+def test_ell_matches_csr_on_poisson(orc):
+    """The ELLPACK restatement equals the CSR restatement on a 3D 7-point
+    Laplacian stored both ways (entries in the same order, so the same sums)."""
+    from cumat_amd.problems import grid_laplacian
+    rng = np.random.default_rng(3)
+    P = grid_laplacian((6, 7, 8), lambda s: np.exp(rng.standard_normal(s)), edges=False)
+    n = P["n"]
+    cnt = np.diff(P["JA"])
+    ind = -np.ones((n, 7), np.int32)
+    val = np.zeros((n, 7))
+    slot = np.arange(P["nnz"]) - np.repeat(P["JA"][:-1], cnt)
+    rows = np.repeat(np.arange(n), cnt)
+    ind[rows, slot] = P["IA"]
+    val[rows, slot] = P["vals"]
+    x = rng.standard_normal(n)
+    ye = orc.ell_spmv(np.asfortranarray(ind).reshape(-1, order="F"), np.asfortranarray(val).reshape(-1, order="F"),
+                      x, n, n, 7)
+    yc = orc.csr_spmv(P["JA"], P["IA"], P["vals"], x, n, n)
+    assert np.array_equal(ye, yc)
