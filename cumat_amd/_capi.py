@@ -91,6 +91,7 @@ _PROTOS = {
     "cm_reduce": ([_VP, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _VP, _I32, _VP], _I32),
     "cm_dot": ([_VP, _I32, _I64, _I32, _VP, _I32, _VP, _I32, _VP], _I32),
     "cm_squared_norm": ([_VP, _I32, _I64, _I32, _VP, _VP], _I32),
+    "cm_dot_noalloc": ([_VP, _I32, _I64, _I32, _VP, _I32, _VP, _I32, _VP], _I32),
     "cm_csr_spmv": ([_VP, _I32, _I32, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP, _I32,
                      _VP, _I32], _I32),
     "cm_ell_spmv": ([_VP, _I32, _I32, _I32, _I32, _I32, _VP, _VP, _I32, _VP, _I32, _VP, _I32],

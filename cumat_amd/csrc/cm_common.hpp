@@ -98,6 +98,9 @@ int dalloc(cm_ctx_t ctx, size_t bytes, void** out);
 int dfree(cm_ctx_t ctx, void* p);
 // Drops the captured batched-CG graphs of a context (before it is destroyed).
 void graph_cache_drop(cm_ctx_t ctx);
+// Looks up the device address of cm_dot_noalloc's fixed buffer on 'device'
+// (current); once per device, at context creation.
+int noalloc_scratch_init(int device);
 
 // Host wait for the context's stream (counted as CM_CNT_HOST_SYNC).
 int host_sync(cm_ctx_t ctx);

@@ -20,7 +20,43 @@
 
 #include <limits>
 
+// Fixed device buffer of cm_dot_noalloc: the per-chunk partial sums of one
+// batch (a split dot product has at most 4096 chunks, 8 bytes each for
+// FP64). One copy per GPU, alive until the process ends; every call on that
+// GPU shares it. In host code the name is a host-side stand-in: its device
+// address comes from hipGetSymbolAddress (noalloc_scratch_init below) and
+// only that address is handed to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64 * 1024;
+static __device__ __align__(256) unsigned char noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace cm {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mutex;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// Looks up this device's copy of noalloc_scratch (the device must be
+// current); called once per device when a context is created there.
+int noalloc_scratch_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+    CM_REQUIRE(device >= 0 && device < NOALLOC_MAX_DEVICES, CM_E_UNSUPPORTED, "device %d: at most %d devices",
+               device, NOALLOC_MAX_DEVICES);
+    if (g_noalloc_ptr[device]) return CM_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess || p == nullptr)
+        return fail(CM_E_HIP, "hipGetSymbolAddress(noalloc_scratch) on device %d failed: %s", device,
+                    hipGetErrorString(e));
+    g_noalloc_ptr[device] = p;
+    return CM_OK;
+}
+
+static void* noalloc_scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+    return (device >= 0 && device < NOALLOC_MAX_DEVICES) ? g_noalloc_ptr[device] : nullptr;
+}
 
 // ---------------------------------------------------------------- types
 template <typename Op, typename T> struct Identity;
@@ -808,6 +844,37 @@ int cm_dot(cm_ctx_t ctx, int dtype, int64_t n, int batches, const void* a, int a
 int cm_squared_norm(cm_ctx_t ctx, int dtype, int64_t n, int batches, const void* a, void* out)
 {
     return cm_dot(ctx, dtype, n, batches, a, 1, a, 1, out);
+}
+
+int cm_dot_noalloc(cm_ctx_t ctx, int dtype, int64_t n, int batches, const void* a, int a_batched, const void* b,
+                   int b_batched, void* out)
+{
+    CM_API_BEGIN
+    // the thread's default context would be created (a stream) on first use
+    CM_REQUIRE(ctx != nullptr, CM_E_ARG, "cm_dot_noalloc: needs a context from cm_ctx_create");
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(n >= 0 && batches >= 1, CM_E_DIM, "cm_dot_noalloc: bad sizes");
+    CM_REQUIRE(out && (n == 0 || (a && b)), CM_E_ARG, "cm_dot_noalloc: NULL pointer");
+    void* buf = cm::noalloc_scratch_ptr(ctx->device);
+    CM_REQUIRE(buf != nullptr, CM_E_STATE, "cm_dot_noalloc: device %d has no fixed buffer (context set-up failed)",
+               ctx->device);
+    cm::count(CM_CNT_EVAL_REDUCTION);
+    cm::count(CM_CNT_EVAL_ANY);
+    if (n == 0) return cm_fill_bytes(ctx, out, 0, cm::dtype_size(dtype) * batches);
+    const cm::PartBuf pb{buf, NOALLOC_SCRATCH_BYTES};
+    switch (dtype) {
+    case CM_F32:
+        return cm::dot_entry<float>(ctx, n, batches, (const float*)a, a_batched, (const float*)b, b_batched,
+                                    (float*)out, pb);
+    case CM_F64:
+        return cm::dot_entry<double>(ctx, n, batches, (const double*)a, a_batched, (const double*)b, b_batched,
+                                     (double*)out, pb);
+    case CM_I32:
+        return cm::dot_entry<int>(ctx, n, batches, (const int*)a, a_batched, (const int*)b, b_batched, (int*)out,
+                                  pb);
+    default: return cm::fail(CM_E_UNSUPPORTED, "cm_dot_noalloc: dtype %d", dtype);
+    }
+    CM_API_END
 }
 
 }  // extern "C"

@@ -199,6 +199,7 @@ static int init_context(cm_context* c, int device)
     hipDeviceProp_t prop;
     CM_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
+    CM_TRY(noalloc_scratch_init(device));
     return CM_OK;
 }
 

@@ -243,6 +243,22 @@ int cm_dot(cm_ctx_t ctx, int dtype, int64_t n, int batches,
 int cm_squared_norm(cm_ctx_t ctx, int dtype, int64_t n, int batches,
                     const void* a, void* out);
 
+/* No counterpart in the reference. cm_dot for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, kernels, summation order
+ * and results as cm_dot, but the partial sums of a split reduction live in a
+ * fixed 64 KiB device buffer of the library (one per GPU, kept until the
+ * process ends) instead of the caching allocator; it calls no hipMalloc,
+ * hipFree or allocating routine. ctx must come from cm_ctx_create (NULL is
+ * rejected: the thread's default context would be created on first use).
+ * Every call on one GPU shares that buffer: calls that overlap on a GPU (two
+ * streams at once) race and must be serialized by the caller. A reduction
+ * needing more than 64 KiB of partials is rejected (CM_E_DIM); the dot
+ * product of one batch needs at most 4096 x 8 bytes. */
+int cm_dot_noalloc(cm_ctx_t ctx, int dtype, int64_t n, int batches,
+                   const void* a, int a_batched, const void* b, int b_batched,
+                   void* out);
+
 /* ======================================================================
  * Sparse CSR matrix-vector product. Replaces CSRMVKernel_StaticBatches +
  * ProductAssignment<...SparseMatrix<CSR>...>
