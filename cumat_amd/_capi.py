@@ -89,6 +89,8 @@ _PROTOS = {
                        _I32, _I64, _I64, _I64, _I32], _I32),
     "cm_lincomb_f32": ([_VP, _I32, C.POINTER(_VP), C.POINTER(C.c_float), _VP, _I64], _I32),
     "cm_reduce": ([_VP, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _VP, _I32, _VP], _I32),
+    "cm_reduce_expr": ([_VP, C.POINTER(Leaf), _I32, C.POINTER(Instr), _I32, _I32, _I32, _I32, _I32, _I32, _I32,
+                        _I32, _VP], _I32),
     "cm_dot": ([_VP, _I32, _I64, _I32, _VP, _I32, _VP, _I32, _VP], _I32),
     "cm_squared_norm": ([_VP, _I32, _I64, _I32, _VP, _VP], _I32),
     "cm_dot_noalloc": ([_VP, _I32, _I64, _I32, _VP, _I32, _VP, _I32, _VP], _I32),

@@ -1,0 +1,274 @@
+// redexpr.hip -- reductions whose input is an unevaluated component-wise
+// expression (one launch, plus one combine launch for long reductions).
+//
+// cuMat's reductions consume the expression lazily: the input iterator calls
+// coeff() on the expression tree (StridedMatrixInputIterator,
+// cuMat/src/Iterator.h:99-105), and dot() is cwiseDot(rhs).sum<Row|Column>()
+// fused into the reduction (cuMat/src/ReductionOpsPlugin.inl:290-297,
+// cuMat/src/ReductionOps.h:564-598). Here the expression is the accumulator
+// program of cm_cwise_eval, evaluated at the reduction's element loads.
+//
+// The expression has no memory layout of its own, so the reduction walks a
+// virtual order in which the reduced axes are the inner ones: element j of
+// segment s has its kept coordinates from s (in the output's layout order)
+// and its reduced coordinates from j (row, column, batch, innermost first).
+// Every reduction is then a set of 'nseg' contiguous segments of length L:
+// a block per chunk of a segment evaluates the program for 8 (4 for 64-bit
+// programs) elements per thread and pass, folds them in a fixed order, and
+// long segments fold their per-chunk partials in a second launch, the same
+// order every run (results are bit-reproducible).
+#include "cwise_core.hpp"
+
+#include <limits>
+
+namespace cm {
+
+constexpr int RX_BS = 256;
+
+template <typename Op, typename T> struct RxIdentity;
+template <typename T> struct RxIdentity<OpSum, T> { __device__ static T v() { return T(0); } };
+template <typename T> struct RxIdentity<OpProd, T> { __device__ static T v() { return T(1); } };
+template <typename T> struct RxIdentity<OpMin, T> { __device__ static T v() { return std::numeric_limits<T>::max(); } };
+template <typename T> struct RxIdentity<OpMax, T> { __device__ static T v() { return std::numeric_limits<T>::lowest(); } };
+template <typename T> struct RxIdentity<OpAll, T> { __device__ static T v() { return T(1); } };
+template <typename T> struct RxIdentity<OpAny, T> { __device__ static T v() { return T(0); } };
+template <typename T> struct RxIdentity<OpBand, T> { __device__ static T v() { return ~T(0); } };
+template <typename T> struct RxIdentity<OpBor, T> { __device__ static T v() { return T(0); } };
+
+// Virtual order: up to three kept axes (kdim / kaxis, innermost first, in
+// the output layout's order) and up to three reduced axes (rdim / raxis).
+// Axis ids: 0 row, 1 column, 2 batch.
+struct RxShape {
+    long long kdim[3], rdim[3];
+    int kaxis[3], raxis[3];
+    int nk, nr;
+    long long L, nseg;
+};
+
+template <typename T> __device__ __forceinline__ T rx_get(const Reg<false>& r, int k);
+template <> __device__ __forceinline__ float rx_get<float>(const Reg<false>& r, int k) { return getf(r, k); }
+template <> __device__ __forceinline__ int rx_get<int>(const Reg<false>& r, int k) { return geti(r, k); }
+template <> __device__ __forceinline__ double rx_get<double>(const Reg<false>&, int) { return 0.0; }
+template <typename T> __device__ __forceinline__ T rx_get(const Reg<true>& r, int k);
+template <> __device__ __forceinline__ float rx_get<float>(const Reg<true>& r, int k) { return getf(r, k); }
+template <> __device__ __forceinline__ int rx_get<int>(const Reg<true>& r, int k) { return geti(r, k); }
+template <> __device__ __forceinline__ double rx_get<double>(const Reg<true>& r, int k) { return getd(r, k); }
+
+__device__ __forceinline__ Coords rx_coords(const RxShape& S, long long s, long long j)
+{
+    long long c[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (a < S.nk) {
+            const long long q = s / S.kdim[a];
+            c[S.kaxis[a]] = s - q * S.kdim[a];
+            s = q;
+        }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (a < S.nr) {
+            const long long q = j / S.rdim[a];
+            c[S.raxis[a]] = j - q * S.rdim[a];
+            j = q;
+        }
+    Coords q;
+    q.r = c[0];
+    q.c = c[1];
+    q.b = c[2];
+    return q;
+}
+
+template <typename Op, typename T, typename S, bool W, int NL>
+__global__ __launch_bounds__(RX_BS) void reduce_expr_kernel(const CwProgram P, const RxShape SH, int nchunk,
+                                                            long long chunk, T* __restrict__ partials,
+                                                            S* __restrict__ out)
+{
+    constexpr int V = CwVec<W>::v;
+    __shared__ T lds[RX_BS / 64];
+    const long long s = blockIdx.x / nchunk;
+    const int c = blockIdx.x % nchunk;
+    const long long beg = c * chunk;
+    const long long end = min(SH.L, beg + chunk);
+    T acc = RxIdentity<Op, T>::v();
+    for (long long j0 = beg + (long long)threadIdx.x * V; j0 < end; j0 += (long long)RX_BS * V) {
+        Bank<W, NL> lv;
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            const long long j = j0 + k;
+            const bool ok = j < end;
+            const Coords q = rx_coords(SH, s, ok ? j : beg);
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                Reg<W> t;
+                t.v[0] = 0;
+                if (l < P.nleaves) {
+                    bool z;
+                    const long long off = leaf_offset(P.leaves[l], q, z);
+                    load_raw<W>(P.leaves[l], z ? 0 : off, t, 0);
+                    if (z) t.v[0] = 0;
+                }
+                lv.k[k][l] = t.v[0];
+            }
+        }
+        Reg<W> val;
+        run_program<W, NL>(P, lv, val);
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+            if (j0 + k < end) acc = Op::apply(acc, rx_get<T>(val, k));
+    }
+    acc = block_reduce<Op, RX_BS>(acc, lds);
+    if (threadIdx.x == 0) {
+        if (nchunk == 1) out[s] = (S)acc;
+        else partials[blockIdx.x] = acc;
+    }
+}
+
+template <typename Op, typename T, typename S>
+__global__ __launch_bounds__(RX_BS) void reduce_expr_combine_kernel(const T* __restrict__ partials, int nchunk,
+                                                                    S* __restrict__ out)
+{
+    __shared__ T lds[RX_BS / 64];
+    const long long s = blockIdx.x;
+    T r = RxIdentity<Op, T>::v();
+    for (int k = threadIdx.x; k < nchunk; k += RX_BS) r = Op::apply(r, partials[s * nchunk + k]);
+    r = block_reduce<Op, RX_BS>(r, lds);
+    if (threadIdx.x == 0) out[s] = (S)r;
+}
+
+template <typename Op, typename T, typename S, bool W, int NL>
+static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P, const RxShape& SH, S* out)
+{
+    constexpr int V = CwVec<W>::v;
+    const long long cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
+    // chunks of at least one pass of the block; about 8 blocks per CU overall
+    long long nchunk = (cus * 8 + SH.nseg - 1) / SH.nseg;
+    const long long maxchunks = (SH.L + (long long)RX_BS * V - 1) / ((long long)RX_BS * V);
+    if (nchunk > maxchunks) nchunk = maxchunks;
+    if (nchunk > 4096) nchunk = 4096;
+    if (nchunk < 1) nchunk = 1;
+    long long chunk = (SH.L + nchunk - 1) / nchunk;
+    chunk = (chunk + V - 1) / V * V;
+    nchunk = (SH.L + chunk - 1) / chunk;
+    const long long blocks = SH.nseg * nchunk;
+    CM_REQUIRE(blocks < (1ll << 31), CM_E_DIM, "cm_reduce_expr: reduction too large");
+    void* part = nullptr;
+    if (nchunk > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * (size_t)blocks, &part));
+    reduce_expr_kernel<Op, T, S, W, NL><<<(unsigned)blocks, RX_BS, 0, ctx->stream>>>(P, SH, (int)nchunk, chunk,
+                                                                                     (T*)part, out);
+    CM_LAUNCH_CHECK();
+    if (nchunk > 1) {
+        reduce_expr_combine_kernel<Op, T, S><<<(unsigned)SH.nseg, RX_BS, 0, ctx->stream>>>((const T*)part,
+                                                                                          (int)nchunk, out);
+        CM_LAUNCH_CHECK();
+    }
+    return CM_OK;
+}
+
+// T is the accumulation type, S the stored result (unsigned char for bool
+// results, whose 0 / 1 values fold in int as the stored-bool reduction folds
+// them in unsigned char: sums wrap modulo 256 in both)
+template <typename T, typename S, bool W, int NL>
+static int rx_dispatch_op(cm_ctx_t ctx, int op, const CwProgram& P, const RxShape& SH, void* out)
+{
+    switch (op) {
+    case CM_RED_SUM: return launch_reduce_expr<OpSum, T, S, W, NL>(ctx, P, SH, (S*)out);
+    case CM_RED_PROD: return launch_reduce_expr<OpProd, T, S, W, NL>(ctx, P, SH, (S*)out);
+    case CM_RED_MIN: return launch_reduce_expr<OpMin, T, S, W, NL>(ctx, P, SH, (S*)out);
+    case CM_RED_MAX: return launch_reduce_expr<OpMax, T, S, W, NL>(ctx, P, SH, (S*)out);
+    case CM_RED_ALL: return launch_reduce_expr<OpAll, T, S, W, NL>(ctx, P, SH, (S*)out);
+    case CM_RED_ANY: return launch_reduce_expr<OpAny, T, S, W, NL>(ctx, P, SH, (S*)out);
+    default: break;
+    }
+    if constexpr (std::is_same<T, int>::value) {
+        switch (op) {
+        case CM_RED_BAND: return launch_reduce_expr<OpBand, int, S, W, NL>(ctx, P, SH, (S*)out);
+        case CM_RED_BOR: return launch_reduce_expr<OpBor, int, S, W, NL>(ctx, P, SH, (S*)out);
+        default: break;
+        }
+    }
+    return fail(CM_E_UNSUPPORTED, "cm_reduce_expr: bitwise reduction of a floating expression");
+}
+
+}  // namespace cm
+
+extern "C" {
+
+int cm_reduce_expr(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_instr* code, int ncode, int dtype,
+                   int op, int axis, int rows, int cols, int batches, int out_row_major, void* out)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(rows >= 1 && cols >= 1 && batches >= 1, CM_E_DIM, "cm_reduce_expr: bad sizes %d x %d x %d", rows,
+               cols, batches);
+    CM_REQUIRE(axis >= 0 && axis <= 7, CM_E_ARG, "cm_reduce_expr: axis %d out of range", axis);
+    CM_REQUIRE(op >= CM_RED_SUM && op <= CM_RED_BOR, CM_E_ARG, "cm_reduce_expr: op %d out of range", op);
+    CM_REQUIRE(dtype == CM_F32 || dtype == CM_F64 || dtype == CM_I32 || dtype == CM_BOOL, CM_E_UNSUPPORTED,
+               "cm_reduce_expr: dtype %d (F32, F64, I32 or BOOL)", dtype);
+    CM_REQUIRE(out != nullptr && code != nullptr && (leaves != nullptr || nleaves == 0), CM_E_ARG,
+               "cm_reduce_expr: NULL pointer");
+    cm::CwProgram P;
+    bool wide = false;
+    CM_TRY(cm::build_program(leaves, nleaves, code, ncode, P, wide));
+    // an instruction's dtype is its operands' type; logical operators and
+    // comparisons yield bool (0 / 1 in the low 32 bits of the lane)
+    const cm_instr& last = code[ncode - 1];
+    const int yields = (last.opcode == CM_OP_BINARY && last.fn >= CM_BIN_LAND) ? (int)CM_BOOL : last.dtype;
+    CM_REQUIRE(yields == dtype, CM_E_ARG, "cm_reduce_expr: the program yields dtype %d, the reduction is %d", yields,
+               dtype);
+    if (dtype == CM_F64) wide = true;
+    for (int l = 0; l < nleaves; ++l)
+        CM_REQUIRE(leaves[l].sr != -1, CM_E_ARG, "cm_reduce_expr: leaf %d reads sparse values directly", l);
+    cm::count(CM_CNT_EVAL_REDUCTION);
+    cm::count(CM_CNT_EVAL_ANY);
+    P.rows = rows;
+    P.cols = cols;
+    P.batches = batches;
+    cm::RxShape SH;
+    memset(&SH, 0, sizeof(SH));
+    const long long dim[3] = {rows, cols, batches};
+    const int red_bit[3] = {CM_AXIS_ROW, CM_AXIS_COL, CM_AXIS_BATCH};
+    // kept axes in the output's order (column major: row, col, batch; row
+    // major: col, row, batch); reduced axes row, col, batch
+    const int korder[3] = {out_row_major ? 1 : 0, out_row_major ? 0 : 1, 2};
+    SH.L = 1;
+    SH.nseg = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int ax = korder[a];
+        if (!(axis & red_bit[ax])) {
+            SH.kdim[SH.nk] = dim[ax];
+            SH.kaxis[SH.nk++] = ax;
+            SH.nseg *= dim[ax];
+        }
+        if (axis & red_bit[a]) {
+            SH.rdim[SH.nr] = dim[a];
+            SH.raxis[SH.nr++] = a;
+            SH.L *= dim[a];
+        }
+    }
+    const bool few = nleaves <= 4;
+    if (dtype == CM_F64)
+        return few ? cm::rx_dispatch_op<double, double, true, 4>(ctx, op, P, SH, out)
+                   : cm::rx_dispatch_op<double, double, true, CM_CWISE_MAX_LEAVES>(ctx, op, P, SH, out);
+    if (dtype == CM_F32 && wide)
+        return few ? cm::rx_dispatch_op<float, float, true, 4>(ctx, op, P, SH, out)
+                   : cm::rx_dispatch_op<float, float, true, CM_CWISE_MAX_LEAVES>(ctx, op, P, SH, out);
+    if (dtype == CM_F32)
+        return few ? cm::rx_dispatch_op<float, float, false, 4>(ctx, op, P, SH, out)
+                   : cm::rx_dispatch_op<float, float, false, CM_CWISE_MAX_LEAVES>(ctx, op, P, SH, out);
+    // int32 and bool results: a program with 64-bit values keeps them in the
+    // low half of its 64-bit lanes
+    if (dtype == CM_BOOL && wide)
+        return few ? cm::rx_dispatch_op<int, unsigned char, true, 4>(ctx, op, P, SH, out)
+                   : cm::rx_dispatch_op<int, unsigned char, true, CM_CWISE_MAX_LEAVES>(ctx, op, P, SH, out);
+    if (dtype == CM_BOOL)
+        return few ? cm::rx_dispatch_op<int, unsigned char, false, 4>(ctx, op, P, SH, out)
+                   : cm::rx_dispatch_op<int, unsigned char, false, CM_CWISE_MAX_LEAVES>(ctx, op, P, SH, out);
+    if (wide)
+        return few ? cm::rx_dispatch_op<int, int, true, 4>(ctx, op, P, SH, out)
+                   : cm::rx_dispatch_op<int, int, true, CM_CWISE_MAX_LEAVES>(ctx, op, P, SH, out);
+    return few ? cm::rx_dispatch_op<int, int, false, 4>(ctx, op, P, SH, out)
+               : cm::rx_dispatch_op<int, int, false, CM_CWISE_MAX_LEAVES>(ctx, op, P, SH, out);
+    CM_API_END
+}
+
+}  // extern "C"

@@ -60,6 +60,24 @@ __device__ __forceinline__ void sx_fetch(const CwProgram& P, Bank<W, NL>& lv, in
     }
 }
 
+// The same for a program whose leaves are value arrays of this pattern and
+// scalars (the common sparse view of same-pattern matrices): no coordinates.
+template <bool W, int NL>
+__device__ __forceinline__ void sx_fetch_values(const CwProgram& P, Bank<W, NL>& lv, int j, bool ok, long long kd,
+                                                long long b)
+{
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        Reg<W> t;
+        t.v[0] = 0;
+        if (l < P.nleaves && ok) {
+            const cm_leaf& L = P.leaves[l];
+            load_raw<W>(L, (L.sr == -1 ? kd : 0) + L.sb * b, t, 0);
+        }
+        lv.k[j][l] = t.v[0];
+    }
+}
+
 // Row of stored entry k within a chunk whose nr + 1 row starts are in rs
 // (the largest i with rs[i] <= k; empty rows share their start with the
 // next row and are skipped by taking the largest).
@@ -74,7 +92,9 @@ __device__ __forceinline__ int sx_row_of(const int* rs, int nr, int k)
     return lo;
 }
 
-template <typename T, bool W, int NL, bool NEED_ROW>
+// NEED_ROW: some leaf depends on the row; VALUES: every leaf is a value array
+// or a scalar (no coordinates at all)
+template <typename T, bool W, int NL, bool NEED_ROW, bool VALUES>
 __global__ __launch_bounds__(SX_BS) void csr_expr_spmv_kernel(const CwProgram P, const int* __restrict__ JA,
                                                               const int* __restrict__ IA, int rows, int cols,
                                                               const T* __restrict__ x, int x_batched,
@@ -112,11 +132,15 @@ __global__ __launch_bounds__(SX_BS) void csr_expr_spmv_kernel(const CwProgram P,
                 const int k = s + lane + 64 * j;
                 const bool ok = k < e;
                 col[j] = ok ? IA[k] : 0;
-                Coords q;
-                q.r = (NEED_ROW && ok) ? r0 + sx_row_of(rs, nr, k) : 0;
-                q.c = col[j];
-                q.b = b;
-                sx_fetch<W, NL>(P, lv, j, ok, q, k);
+                if constexpr (VALUES) {
+                    sx_fetch_values<W, NL>(P, lv, j, ok, k, b);
+                } else {
+                    Coords q;
+                    q.r = (NEED_ROW && ok) ? r0 + sx_row_of(rs, nr, k) : 0;
+                    q.c = col[j];
+                    q.b = b;
+                    sx_fetch<W, NL>(P, lv, j, ok, q, k);
+                }
             }
             Reg<W> val;
             run_program<W, NL>(P, lv, val);
@@ -221,16 +245,19 @@ static int sx_prepare(const cm_leaf* leaves, int nleaves, const cm_instr* code, 
 }
 
 template <typename T, bool W, int NL>
-static int launch_csr_expr(cm_ctx_t ctx, const CwProgram& P, bool need_row, int rows, int cols, int batches,
+static int launch_csr_expr(cm_ctx_t ctx, const CwProgram& P, int shape, int rows, int cols, int batches,
                            const int* JA, const int* IA, const void* x, int xb, void* y, int mode)
 {
     dim3 grid(walk_grid(div_up(div_up(rows, 64), SX_BS / 64), 2048), batches);
-    if (need_row)
-        csr_expr_spmv_kernel<T, W, NL, true><<<grid, SX_BS, 0, ctx->stream>>>(P, JA, IA, rows, cols, (const T*)x, xb,
-                                                                               (T*)y, mode);
+    if (shape == 2)
+        csr_expr_spmv_kernel<T, W, NL, false, true><<<grid, SX_BS, 0, ctx->stream>>>(P, JA, IA, rows, cols,
+                                                                                      (const T*)x, xb, (T*)y, mode);
+    else if (shape == 1)
+        csr_expr_spmv_kernel<T, W, NL, true, false><<<grid, SX_BS, 0, ctx->stream>>>(P, JA, IA, rows, cols,
+                                                                                      (const T*)x, xb, (T*)y, mode);
     else
-        csr_expr_spmv_kernel<T, W, NL, false><<<grid, SX_BS, 0, ctx->stream>>>(P, JA, IA, rows, cols, (const T*)x,
-                                                                                xb, (T*)y, mode);
+        csr_expr_spmv_kernel<T, W, NL, false, false><<<grid, SX_BS, 0, ctx->stream>>>(P, JA, IA, rows, cols,
+                                                                                       (const T*)x, xb, (T*)y, mode);
     CM_LAUNCH_CHECK();
     return CM_OK;
 }
@@ -288,13 +315,19 @@ int cm_csr_expr_spmv(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_
     P.rows = rows;
     P.cols = cols;
     P.batches = batches;
-    // a leaf needs the row coordinate unless it is a value array or depends
-    // on the column and batch only
-    bool need_row = false;
-    for (int l = 0; l < nleaves; ++l)
-        if (leaves[l].sr != -1 && (leaves[l].kind != CM_LEAF_STRIDED || leaves[l].sr != 0)) need_row = true;
-    return cm::sx_dispatch<cm::CsrExprLaunch>(dtype, wide, nleaves, ctx, P, need_row, rows, cols, batches, JA, IA,
-                                              x, x_batched, y, mode);
+    // kernel shape: 2 = value arrays and scalars only (no coordinates),
+    // 1 = some leaf depends on the row, 0 = leaves depend on the column and
+    // batch only
+    bool need_row = false, values_only = true;
+    for (int l = 0; l < nleaves; ++l) {
+        const cm_leaf& L = leaves[l];
+        if (L.sr == -1) continue;
+        if (L.kind != CM_LEAF_STRIDED || L.sr != 0) need_row = true;
+        if (L.kind != CM_LEAF_STRIDED || L.sr != 0 || L.sc != 0) values_only = false;
+    }
+    const int shape = values_only ? 2 : (need_row ? 1 : 0);
+    return cm::sx_dispatch<cm::CsrExprLaunch>(dtype, wide, nleaves, ctx, P, shape, rows, cols, batches, JA, IA, x,
+                                              x_batched, y, mode);
     CM_API_END
 }
 

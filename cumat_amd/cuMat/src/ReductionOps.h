@@ -5,9 +5,13 @@
 // squaredNorm and norm (cuMat/src/ReductionOpsPlugin.inl:124-450 with the
 // functors and initial values of cuMat/src/ReductionOps.h:859-968). The
 // result keeps the non-reduced dimensions and has size 1 on the reduced ones.
-// Unlike cuMat the reduction is evaluated when it is called (one cm_reduce
-// launch; an input that is an expression is evaluated first), and the result
-// is a Matrix, which every cuMat use of a reduction accepts.
+// Unlike cuMat the reduction is evaluated when it is called, and the result
+// is a Matrix, which every cuMat use of a reduction accepts. A stored matrix
+// runs cm_reduce / cm_dot; an expression input is consumed lazily, as
+// cuMat's input iterator does (cuMat/src/Iterator.h:99-105): its program is
+// evaluated inside the reduction (cm_reduce_expr), so no temporary is
+// written; dot / squaredNorm of expressions reduce cwiseDot / cwiseAbs2 the
+// same way (cuMat/src/ReductionOpsPlugin.inl:290-297).
 
 #include "Matrix.h"
 
@@ -27,6 +31,37 @@ Matrix<S, Dynamic, Dynamic, Dynamic, F> materialize(const DenseRef<S, F>& m)
 template <typename E>
 auto materialize(const MatrixBase<E>& e) { return e.eval(); }
 
+// stored inputs (a Matrix or a strided reference to one)
+template <typename T> struct is_stored : std::false_type {};
+template <typename S, int R, int C, int B, int F> struct is_stored<Matrix<S, R, C, B, F>> : std::true_type {};
+template <typename S, int F> struct is_stored<DenseRef<S, F>> : std::true_type {};
+
+// reduce_op over 'axis' of an expression, evaluated inside the reduction;
+// out has the kept dimensions in the given layout
+template <typename E>
+void reduceExprInto(const E& e, int axis, int op, typename E::Scalar* out, bool out_row_major)
+{
+    typedef typename E::Scalar S;
+    ProgramBuilder b;
+    e.emit(b, 0);
+    const Index R = e.rows(), C = e.cols(), B = e.batches();
+    std::vector<cm_leaf> leaves(b.leaves.size() ? b.leaves.size() : 1);
+    for (size_t k = 0; k < b.leaves.size(); ++k) {
+        const LeafDesc& l = b.leaves[k];
+        if (l.sr < 0 || l.sc < 0)
+            throw std::invalid_argument("cuMat: SparseMatrix::direct() is only valid in sparse assignments and views");
+        leaves[k].ptr = l.ptr;
+        leaves[k].dtype = l.dtype;
+        leaves[k].kind = l.kind;
+        leaves[k].sr = (l.rows == 1 && R != 1) ? 0 : l.sr;
+        leaves[k].sc = (l.cols == 1 && C != 1) ? 0 : l.sc;
+        leaves[k].sb = (l.batches == 1 && B != 1) ? 0 : l.sb;
+    }
+    CUMAT_AMD_CALL(cm_reduce_expr(Context::current().handle(), leaves.data(), (int)b.leaves.size(), b.code.data(),
+                                  (int)b.code.size(), DType<S>::value, op, axis, (int)R, (int)C, (int)B,
+                                  out_row_major ? 1 : 0, out));
+}
+
 template <typename M>
 struct MatrixTraits;
 template <typename S, int R, int C, int B, int F>
@@ -38,6 +73,17 @@ template <int _Axis, typename E>
 auto reduceStatic(const MatrixBase<E>& expr, int op)
 {
     static_assert(_Axis >= 0 && _Axis <= 7, "axis must be a combination of Axis::Row, Axis::Column, Axis::Batch");
+    if constexpr (!is_stored<E>::value) {
+        const E& e = expr.derived();
+        typedef typename E::Scalar S;
+        typedef Matrix<S, (_Axis & Axis::Row) ? 1 : (int)E::Rows, (_Axis & Axis::Column) ? 1 : (int)E::Columns,
+                       (_Axis & Axis::Batch) ? 1 : (int)E::Batches, E::Flags>
+            Out;
+        Out out((_Axis & Axis::Row) ? 1 : e.rows(), (_Axis & Axis::Column) ? 1 : e.cols(),
+                (_Axis & Axis::Batch) ? 1 : e.batches());
+        reduceExprInto(e, _Axis, op, out.data(), (E::Flags & RowMajor) != 0);
+        return out;
+    } else {
     auto in = materialize(expr.derived());
     typedef decltype(in) In;
     typedef typename In::Scalar S;
@@ -51,12 +97,21 @@ auto reduceStatic(const MatrixBase<E>& expr, int op)
                              (int)in.batches(), (T::Flags & RowMajor) ? 1 : 0, in.data(),
                              (T::Flags & RowMajor) ? 1 : 0, out.data()));
     return out;
+    }
 }
 
 template <typename E>
 auto reduceDynamic(const MatrixBase<E>& expr, int axis, int op)
 {
     CUMAT_ASSERT_ARGUMENT(axis >= 0 && axis <= 7);
+    if constexpr (!is_stored<E>::value) {
+        const E& e = expr.derived();
+        typedef Matrix<typename E::Scalar, Dynamic, Dynamic, Dynamic, E::Flags> Out;
+        Out out((axis & Axis::Row) ? 1 : e.rows(), (axis & Axis::Column) ? 1 : e.cols(),
+                (axis & Axis::Batch) ? 1 : e.batches());
+        reduceExprInto(e, axis, op, out.data(), (E::Flags & RowMajor) != 0);
+        return out;
+    } else {
     auto in = materialize(expr.derived());
     typedef decltype(in) In;
     typedef typename In::Scalar S;
@@ -68,6 +123,7 @@ auto reduceDynamic(const MatrixBase<E>& expr, int axis, int op)
                              (int)in.batches(), (T::Flags & RowMajor) ? 1 : 0, in.data(),
                              (T::Flags & RowMajor) ? 1 : 0, out.data()));
     return out;
+    }
 }
 
 }  // namespace internal
@@ -112,6 +168,14 @@ auto MatrixBase<_Derived>::dot(const MatrixBase<_Other>& rhs) const
     static_assert(std::is_same<typename _Derived::Scalar, typename _Other::Scalar>::value,
                   "No implicit casting is allowed in dot products");
     typedef typename _Derived::Scalar S;
+    if constexpr (!internal::is_stored<_Derived>::value || !internal::is_stored<_Other>::value) {
+        // cwiseDot(rhs).sum<Row | Column>() in one reduction
+        CUMAT_ASSERT_DIMENSION(derived().rows() == rhs.derived().rows() && derived().cols() == rhs.derived().cols());
+        const auto prod = derived().cwiseDot(rhs.derived());
+        Matrix<S, 1, 1, Dynamic, ColumnMajor> out(1, 1, prod.batches());
+        internal::reduceExprInto(prod, Axis::Row | Axis::Column, CM_RED_SUM, out.data(), false);
+        return out;
+    } else {
     auto a = internal::materialize(derived());
     auto b = internal::materialize(rhs.derived());
     CUMAT_ASSERT_DIMENSION(a.rows() * a.cols() == b.rows() * b.cols());
@@ -122,17 +186,26 @@ auto MatrixBase<_Derived>::dot(const MatrixBase<_Other>& rhs) const
                           (int)batches, a.data(), a.batches() > 1 ? 1 : 0, b.data(), b.batches() > 1 ? 1 : 0,
                           out.data()));
     return out;
+    }
 }
 
 template <typename _Derived>
 auto MatrixBase<_Derived>::squaredNorm() const
 {
     typedef typename _Derived::Scalar S;
+    if constexpr (!internal::is_stored<_Derived>::value) {
+        // cwiseAbs2().sum<Row | Column>() in one reduction
+        const auto sq = derived().cwiseAbs2();
+        Matrix<S, 1, 1, Dynamic, ColumnMajor> out(1, 1, sq.batches());
+        internal::reduceExprInto(sq, Axis::Row | Axis::Column, CM_RED_SUM, out.data(), false);
+        return out;
+    } else {
     auto a = internal::materialize(derived());
     Matrix<S, 1, 1, Dynamic, ColumnMajor> out(1, 1, a.batches());
     CUMAT_AMD_CALL(cm_squared_norm(Context::current().handle(), internal::DType<S>::value,
                                    (int64_t)(a.rows() * a.cols()), (int)a.batches(), a.data(), out.data()));
     return out;
+    }
 }
 
 template <typename _Derived>

@@ -234,6 +234,20 @@ int cm_reduce(cm_ctx_t ctx, int dtype, int op, int axis,
               int rows, int cols, int batches, int in_row_major,
               const void* in, int out_row_major, void* out);
 
+/* Reduction whose input is an unevaluated component-wise expression: the
+ * program of cm_cwise_eval (result dtype 'dtype': F32, F64, I32 or BOOL;
+ * bitwise operators take I32 / BOOL) is evaluated inside the reduction, so no temporary is written -- cuMat's
+ * lazy input iterator (cuMat/src/Iterator.h:99-105) and the fused
+ * cwiseDot(...).sum() of dot (cuMat/src/ReductionOpsPlugin.inl:290-297,
+ * cuMat/src/ReductionOps.h:564-598). The expression's logical size is
+ * rows x cols x batches (leaf strides as for cm_cwise_eval, 0 on broadcast
+ * axes); 'op' and 'axis' as in cm_reduce; the output keeps the non-reduced
+ * dimensions in the layout given by out_row_major. Deterministic. */
+int cm_reduce_expr(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves,
+                   const cm_instr* code, int ncode, int dtype, int op, int axis,
+                   int rows, int cols, int batches, int out_row_major,
+                   void* out);
+
 /* dot / squaredNorm per batch (cuMat/src/ReductionOpsPlugin.inl:395-431):
  * out[b] = sum_i a[i + n*b] * b[i + n*b]; a_batched/b_batched = 0 broadcasts
  * a batch-1 operand. Result stays on the device (batches values). */

@@ -251,6 +251,54 @@ TEST_CASE("float reductions, dot, squaredNorm, norm")
     }
 }
 
+TEST_CASE("reductions, dot and squaredNorm of expressions run inside the reduction")
+{
+    // cuMat reads a reduction's input through the expression tree
+    // (cuMat/src/Iterator.h:99-105); dot is cwiseDot(rhs).sum<Row|Column>()
+    // (cuMat/src/ReductionOpsPlugin.inl:290-297)
+    BMatrixXiR m = createTestMatrix();  // 4 x 3 x 2, values 1..24
+    BMatrixXiC mc = m;
+    auto h = m.toHostRowMajor();
+    Profiling::instance().resetAll();
+    auto s = (m * 2 - 1).sum<Axis::Row>();
+    REQUIRE(Profiling::instance().get(Profiling::EvalCwise) == 0);
+    REQUIRE(Profiling::instance().get(Profiling::EvalReduction) == 1);
+    auto hs = s.toHostRowMajor();
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 3; ++c) {
+            int e = 0;
+            for (int r = 0; r < 4; ++r) e += 2 * h[(b * 4 + r) * 3 + c] - 1;
+            REQUIRE(hs[b * 3 + c] == e);
+        }
+    for (int axis = 0; axis < 8; ++axis) {
+        auto got = (mc + mc).maxCoeff(axis).toHostRowMajor();
+        auto ref = BMatrixXiC(mc + mc).maxCoeff(axis).toHostRowMajor();
+        REQUIRE_MSG(got == ref, "axis " << axis);
+        auto gs = (m - 3).sum(axis).toHostRowMajor();
+        auto rs = BMatrixXiR(m - 3).sum(axis).toHostRowMajor();
+        REQUIRE_MSG(gs == rs, "axis " << axis);
+    }
+    const int n = 5000;
+    std::vector<float> ha(n), hb(n);
+    for (int i = 0; i < n; ++i) {
+        ha[i] = std::sin(0.01f * i);
+        hb[i] = std::cos(0.02f * i);
+    }
+    VectorXf a = VectorXf::fromHost(ha.data(), n, 1, 1), b = VectorXf::fromHost(hb.data(), n, 1, 1);
+    Profiling::instance().resetAll();
+    const float sn = (a - b).squaredNorm().getScalar();
+    const float dt = (a * 2.f).dot(b).getScalar();
+    REQUIRE(Profiling::instance().get(Profiling::EvalCwise) == 0);
+    double esn = 0, edt = 0, sdt = 0;
+    for (int i = 0; i < n; ++i) {
+        esn += (double)(ha[i] - hb[i]) * (ha[i] - hb[i]);
+        edt += 2.0 * ha[i] * hb[i];
+        sdt += std::fabs(2.0 * ha[i] * hb[i]);
+    }
+    REQUIRE(harness::close(sn, esn, 1e-5));
+    REQUIRE(std::fabs(dt - edt) <= 1e-5 * sdt);
+}
+
 TEST_CASE("dimension and type errors throw")
 {
     VectorXf a(5), b(6);

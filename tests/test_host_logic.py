@@ -207,3 +207,22 @@ def test_ell_host_checks():
         ELLMatrix(3, 4, [[0, 1], [1, 2], [2, -2]], np.ones((3, 2), np.float32))   # index < -1
     with pytest.raises(capi.CumatArgError):
         ELLMatrix(3, 4, [[0, 1], [1, 2], [2, -1]], np.ones((3, 3), np.float32))   # values of another shape
+
+
+def test_reduction_routes_expressions_and_checks_them_first():
+    """A whole stored array (or its plain leaf) takes the stored-array
+    reductions; a transposed, broadcast or computed input is an expression
+    reduced by cm_reduce_expr. Type and size errors are raised before any
+    call into the library."""
+    from cumat_amd import reductions as red
+    a = _arr(6, 4, 2, ptr=0x1000)
+    b = _arr(5, 4, 2, ptr=0x2000)
+    assert red._is_stored(a) and red._is_stored(a.expr())
+    assert not red._is_stored(a.expr().T)
+    assert not red._is_stored(a.expr() * 2)
+    assert not red._is_stored(a.expr().diagonal())
+    half = _arr(6, 4, 2, dtype=np.uint16, ptr=0x3000)  # bfloat16 storage
+    with pytest.raises(TypeError):
+        red.reduce(half.expr() * 2, "sum")  # bfloat16 expressions are not reduced in place
+    with pytest.raises(capi.CumatArgError):
+        red.dot(a.expr() * 2, b.expr())  # 6x4 against 5x4
