@@ -19,7 +19,8 @@ systems, the batched, mixed-precision and adjoint solvers, the reference's
 published benchmark workloads, and the CPU baseline: the reference's
 vendored Eigen (oracle/_ref) on the host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
+                    [--kernels-only | --solvers-only]
 """
 from __future__ import annotations
 
@@ -77,6 +78,8 @@ def main():
                     help="multi-kernel CG form: 1 = three launches per iteration, 3 = two (fused p update)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the SpMV roofline launches and a copy kernel of known size (counter runs)")
+    ap.add_argument("--solvers-only", action="store_true",
+                    help="only the batched, mixed-precision and adjoint workloads, one pass each (profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
@@ -92,6 +95,14 @@ def main():
     ctx = cm.Context(0)
     stream = ctx.stream()
     timer = HipTimer()
+    if args.solvers_only:
+        out = {}
+        for name, fn in (("batched_cg_4096", bench_batched), ("mixed_ir_160cubed", bench_mixed),
+                         ("adjoint_64x512sq", bench_adjoint)):
+            out[name] = fn(cm, capi, ctx)
+            log(name, out[name])
+        print(json.dumps(out), flush=True)
+        return
 
     # ------------------------------------------------------------ workload
     g = 2048
@@ -135,12 +146,15 @@ def main():
         Ak, xk, yk = copies[k % 4]
         Ak.matvec(xk, yk, ctx=ctx)
     ctx.sync()
-    reps = 200
-    timer.start(stream)
-    for k in range(reps):
-        Ak, xk, yk = copies[k % 4]
-        Ak.matvec(xk, yk, ctx=ctx)
-    spmv_ms = timer.stop_ms(stream) / reps
+    # median over 20 groups of 12 launches (three rounds of the four copies)
+    group_ms = []
+    for _ in range(20):
+        timer.start(stream)
+        for k in range(12):
+            Ak, xk, yk = copies[k % 4]
+            Ak.matvec(xk, yk, ctx=ctx)
+        group_ms.append(timer.stop_ms(stream) / 12)
+    spmv_ms = float(np.median(group_ms))
     spmv_bytes = 8 * nnz + 12 * n
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     log(f"SpMV 2048^2: {spmv_ms * 1000:.1f} us/launch, {spmv_gbs:.0f} GB/s ({spmv_gbs / HBM_PEAK_GBS:.1%})")
@@ -160,6 +174,8 @@ def main():
                           "copy_us": round(copy_ms * 1000, 2), "copy_bytes_read": 4 << 26,
                           "copy_bytes_written": 4 << 26}), flush=True)
         return
+    cfg1 = bench_config1(cm, capi, ctx)
+    log("128^2 CG (tol 1e-6):", cfg1)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "spmv_poisson2048_traffic.json")
     if os.path.exists(tf):
@@ -188,7 +204,8 @@ def main():
         "data": "synthetic (2D 5-point Poisson, b ~ U[-1,1] seed 20261202)",
         "config": {"workload": "Jacobi-PCG iteration on 2D Poisson 2048x2048 (n=4194304, nnz=20963328), FP32 CSR",
                    "n": n, "nnz": nnz, "cg_bytes_per_iteration": cg_bytes,
-                   "cg_effective_GBps": round(cg_bytes / (ms_per_step * 1e-3) / 1e9, 1)},
+                   "cg_effective_GBps": round(cg_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                   "cg_128x128_tol1e-6": cfg1},
         "roofline": {"bound": "hbm", "kernel": "csr_spmv_kernel<float,float> (2048^2 Poisson)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -331,6 +348,21 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
     sms = [cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx) for _ in range(2)]
     rv = cm.DeviceArray.from_numpy(rng.uniform(-1, 1, n).astype(np.float32), rows=1, cols=n, batches=1, ctx=ctx)
     nnz = IA.size
+    # ---- the fused sparse-expression product (2*A).sparseView(pattern) * x:
+    # the same bytes as the plain SpMV (values, IA, JA, x, y), one launch
+    views = [cm.sparse_view(2.0 * S.direct(), S.pattern) for S in sms]
+    for k in range(4):
+        views[k % 2].matvec(xs[k % 2], out=ys[k % 2], ctx=ctx)
+    ctx.sync()
+    timer.start(stream)
+    for k in range(reps):
+        views[k % 2].matvec(xs[k % 2], out=ys[k % 2], ctx=ctx)
+    ms = timer.stop_ms(stream) / reps
+    by = 8 * nnz + 12 * n
+    out["csr_expr_spmv_2048^2"] = {"expr": "(2*A).sparseView(pattern) * x", "us": round(ms * 1000, 1),
+                                   "GBps": round(by / (ms * 1e-3) / 1e9, 1),
+                                   "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log("csr_expr_spmv_2048^2", out["csr_expr_spmv_2048^2"])
     csr_cw = {}
     # bytes: values read + written; the column-dependent program also reads
     # IA and gathers the row vector (counted once)
@@ -365,7 +397,7 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
             t0 = time.perf_counter()
             cg.solve(bd, xd, ctx=ctx)
             ts_.append(time.perf_counter() - t0)
-        t = min(ts_)
+        t = float(np.median(ts_))
         out[label] = {"iterations": cg.iterations(), "error": cg.error(), "ms_to_solution": round(t * 1e3, 3),
                       "iters_per_s": round(cg.iterations() / t, 1)}
         log(label, out[label])
@@ -398,6 +430,32 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
             out[name] = {"error": repr(ex)}
             log(name, "failed:", repr(ex))
     return out
+
+
+def bench_config1(cm, capi, ctx):
+    """The plain Jacobi-PCG of the 128^2 Poisson system (n = 16,384, nnz =
+    81,408, FP32, b ~ U[-1,1], x0 = 0, tol 1e-6, max 1000 iterations) through
+    the default (automatic) path: time to solution and iterations/s, median of
+    7 solves after one warm-up."""
+    from cumat_amd.problems import poisson2d
+    JA, IA, V = poisson2d(128)
+    n = 128 * 128
+    b = np.random.default_rng(20261203).uniform(-1, 1, n).astype(np.float32)
+    A = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
+    bd = cm.DeviceArray.from_numpy(b, ctx=ctx)
+    xd = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
+    cg = cm.ConjugateGradient(A).setTolerance(1e-6).setMaxIterations(1000)
+    cg.solve(bd, xd, ctx=ctx)
+    ts = []
+    for _ in range(7):
+        ctx.sync()
+        t0 = time.perf_counter()
+        cg.solve(bd, xd, ctx=ctx)
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    return {"iterations": cg.iterations(), "error": cg.error(), "ms_to_solution": round(t * 1e3, 3),
+            "iters_per_s": round(cg.iterations() / t, 1), "path": cg.last_stats.get("path")}
 
 
 def _sub_csr(P, s):
@@ -679,14 +737,127 @@ def cpu_baseline(JA, IA, V, b, n, nnz):
             rows[name + "_1024x1024x64"] = {"ms": round(dt * 1e3, 2), "GBps": round(4 * N / dt / 1e9, 1),
                                             "threads": 1}
         del vecs
+        # the same 2048^2 CG on one thread (30 iterations)
+        h = L.eref_csr_create_f32(n, n, nnz, JA.ctypes.data, IA.ctypes.data, V.ctypes.data)
+        t0 = time.perf_counter()
+        it1 = L.eref_pcg_handle_f32(h, b.ctypes.data, x.ctypes.data, 0.0, 30, 1, C.byref(err))
+        rows["cg_2048^2_1thread"] = {"iters_per_s": round(it1 / (time.perf_counter() - t0), 2), "threads": 1}
+        L.eref_csr_destroy_f32(h)
         log("CPU Eigen rows:", rows)
+        configs = cpu_other_configs(oracle, L, threads)
+        log("CPU Eigen configs:", configs)
         return {"value": round(v, 3), "unit": "CG iterations/s", "cores": threads, "kind": "reference",
                 "sample": f"{it} Jacobi-PCG iterations of Eigen 3.3.90 (reference third-party) on the same "
                           f"2048^2 FP32 system, Eigen::setNbThreads({threads})",
-                "other_rows": rows}
+                "cpu_model": cpu_model(), "other_rows": rows, "configs": configs}
     except Exception as ex:  # the baseline is reported, never required
         log("cpu baseline failed:", ex)
         return None
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), r
+
+
+def cpu_other_configs(oracle, L, threads):
+    """Eigen (oracle/_ref) on the host cores for the other workloads, each on
+    a bounded sample: the 128^2 solve (1 thread and all threads, median of 5),
+    the 4096-system batch (all systems over all threads, every 16th system on
+    one thread), 20 FP64 iterations of the 160^3 system (time to 1e-12
+    extrapolated to the 1000 iterations FP64 CG needs there), and one 512^2
+    system's FP64 forward + adjoint solves (extrapolated to 64 systems)."""
+    from cumat_amd.problems import batched_poisson, grid_laplacian, lognormal_sampler, poisson2d
+    out = {}
+    err = C.c_double()
+    # config 1: 128^2 FP32, tol 1e-6, max 1000
+    JA, IA, V = poisson2d(128)
+    n = 128 * 128
+    b = np.random.default_rng(20261203).uniform(-1, 1, n).astype(np.float32)
+    x = np.zeros(n, np.float32)
+    for nt in (1, threads):
+        t, it = _median_time(lambda: L.eref_pcg_f32(n, IA.size, JA.ctypes.data, IA.ctypes.data, V.ctypes.data,
+                                                    b.ctypes.data, x.ctypes.data, 0, 1e-6, 1000, nt,
+                                                    C.byref(err)), 5)
+        out[f"cg128_{nt}threads"] = {"iterations": it, "ms_to_solution": round(t * 1e3, 3),
+                                     "iters_per_s": round(it / t, 1), "threads": nt}
+    # config 3: 4096 systems, side 16..96 (the GPU bench's seed), tol 1e-6
+    P = batched_poisson(4096, 16, 96, seed=20261205)
+    xb = np.zeros(P["b"].size, np.float32)
+    its = np.zeros(4096, np.int32)
+    args = (P["row_off"].ctypes.data, P["JA"].ctypes.data, P["IA"].ctypes.data, P["vals"].ctypes.data,
+            P["b"].ctypes.data, xb.ctypes.data, 1e-6, 5000)
+    t0 = time.perf_counter()
+    ns = L.eref_batched_pcg_f32(4096, *args, threads, 1, its.ctypes.data)
+    t = time.perf_counter() - t0
+    out["batched_4096_all_threads"] = {"systems": ns, "seconds": round(t, 3), "solves_per_s": round(ns / t, 1),
+                                       "threads": threads}
+    t0 = time.perf_counter()
+    ns = L.eref_batched_pcg_f32(4096, *args, 1, 16, its.ctypes.data)
+    t = time.perf_counter() - t0
+    out["batched_4096_1thread"] = {"systems": ns, "sample": "every 16th system", "seconds": round(t, 3),
+                                   "solves_per_s": round(ns / t, 1), "threads": 1}
+    del P, xb
+    # config 4: 160^3 log-normal(0,1), FP64 CG (the bench's seeds)
+    P = grid_laplacian((160, 160, 160), lognormal_sampler(np.random.default_rng(20261206), 0.0, 1.0), edges=False)
+    n = P["n"]
+    b = np.random.default_rng(20261207).uniform(-1, 1, n)
+    x = np.zeros(n)
+
+    def run(k):
+        return L.eref_pcg_f64(n, P["nnz"], P["JA"].ctypes.data, P["IA"].ctypes.data, P["vals"].ctypes.data,
+                              b.ctypes.data, x.ctypes.data, 0, 1e-300, k, threads, C.byref(err))
+    t_1, _ = _median_time(lambda: run(1), 1)    # matrix build + set-up + 1 iteration
+    t_21, it = _median_time(lambda: run(21), 1)
+    per = (t_21 - t_1) / 20
+    out["fp64_cg_160cubed"] = {"sample_iterations": 20, "ms_per_iteration": round(per * 1e3, 2),
+                               "est_seconds_to_1e-12_at_1000_iterations": round(per * 1000, 2),
+                               "threads": threads}
+    del P, x
+    # config 5: one 512^2 system, theta = exp(N(0, 0.5^2)), FP64 forward + adjoint at tol 1e-7
+    rng = np.random.default_rng(20261208)
+    P0 = grid_laplacian((512, 512), lambda s: np.ones(s), edges=False)
+    ne = P0["nedges"]
+    theta = np.exp(0.5 * rng.standard_normal(ne))
+    state = {"o": 0}
+
+    def samp(shape):
+        k = int(np.prod(shape))
+        o = theta[state["o"]:state["o"] + k].reshape(shape)
+        state["o"] += k
+        return o
+    P = grid_laplacian((512, 512), samp, edges=False)
+    n = P["n"]
+    b = rng.uniform(-1, 1, n)
+    xs = rng.standard_normal(n)
+    x = np.zeros(n)
+    lam = np.zeros(n)
+    t0 = time.perf_counter()
+    itf = L.eref_pcg_f64(n, P["nnz"], P["JA"].ctypes.data, P["IA"].ctypes.data, P["vals"].ctypes.data,
+                         b.ctypes.data, x.ctypes.data, 0, 1e-7, 20000, threads, C.byref(err))
+    t1 = time.perf_counter()
+    gx = x - xs
+    itb = L.eref_pcg_f64(n, P["nnz"], P["JA"].ctypes.data, P["IA"].ctypes.data, P["vals"].ctypes.data,
+                         gx.ctypes.data, lam.ctypes.data, 0, 1e-7, 20000, threads, C.byref(err))
+    t2 = time.perf_counter()
+    out["adjoint_512sq_fp64"] = {"systems_sampled": 1, "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+                                 "forward_iterations": itf, "backward_iterations": itb,
+                                 "est_seconds_64_systems": round(64 * (t2 - t0), 1), "threads": threads}
+    return out
 
 
 if __name__ == "__main__":

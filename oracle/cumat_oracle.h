@@ -3,8 +3,18 @@
  * hot path. TEST INFRASTRUCTURE ONLY: used by tests/, by the smoke check and
  * by bench.py's CPU-baseline leg as the checker, never by the product path.
  *
- * Every function names the reference file:line it restates. Pinning of each
- * function (golden vectors, Eigen, closed forms) is listed in oracle/README.md.
+ * Every function names the reference file:line it restates. Each is pinned
+ * by tests/test_oracle.py against the reference's own test vectors
+ * (tests/golden/reference_tests.json), the Eigen build of oracle/_ref or the
+ * committed Eigen runs (tests/golden/eigen_cg.json), and closed forms:
+ *   orc_csr_spmv_*    4x5 fixture (plain, batched, +=/-=), Dirichlet A*1
+ *   orc_ell_spmv_*    ELLPACK fixture in every dtype, equal to the CSR form
+ *   orc_reduce        integer fixtures of every op / axis / layout, sum 1..N
+ *   orc_dot           batched dot and squared-norm fixtures
+ *   orc_cwise_*, orc_assign  compound-assignment fixtures, C / and %
+ *   orc_csr_inverse_diagonal_f64  diagonal fixture, zero guard
+ *   orc_pcg_*         Eigen's ConjugateGradient (same iterations), closed forms
+ *   orc_edge_grad_f64 central differences of lambda^T A(theta) x (bilinear)
  */
 #ifndef CUMAT_ORACLE_H
 #define CUMAT_ORACLE_H
