@@ -14,8 +14,9 @@
 //    update fused into the SpMV, p.Ap partials; KB: as K2); consumers
 //    re-reduce the producers' partials in a fixed order, so no device-scope
 //    fence is needed. Kernels launched after convergence return at once;
-//    the host checks the status word every few iterations through pinned
-//    memory and stops launching.
+//    the kernels publish the status to host-mapped memory every few
+//    iterations, where the host reads it without a copy and stops
+//    launching.
 //  RESIDENT path (n <= 16384, one right-hand side): one workgroup runs the
 //    whole solve; p lives in LDS, x / r / M^-1 in registers, the reductions
 //    are DPP wave reductions + LDS, and no kernel boundary is crossed.

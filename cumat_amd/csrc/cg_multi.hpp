@@ -11,9 +11,16 @@
 //       and beta = rz / absNew[i&1]; p = M^-1 r + beta p; for large systems
 //       also x += alpha p (alpha re-derived from K1's partials), which saves
 //       K2 the x and p passes; block 0 records absNew[(i+1)&1] = rz, the
-//       status, the iteration count and the error. In that form a batch's
-//       last x update is recorded and applied by the finalize kernel, so no
-//       kernel depends on a decision another block of the same launch writes.
+//       status, the iteration count and the error. K3 decides from the
+//       state K1 recorded when the iteration started, so every block of a
+//       batch takes the same decision, and in a batch's last iteration it
+//       applies the x update only.
+// Setup: one vector pass (reads b and M^-1 without a guess; r_0 = b and
+// x_0 = 0 are not written where K2 / K3 can use b and zero instead) and a
+// one-block fold that initialises the state. The host learns the state from
+// a host-mapped mirror the kernels publish to (no copies): it keeps at most
+// two chunks of CG_CHUNK iterations in flight and reads the result after
+// the final synchronisation.
 // The FUSED form has two launches per iteration (p alternates between two
 // buffers); it saves the p re-read and a launch but triples the gathers of
 // the SpMV, which costs more on MI355X than it saves (94 us for KA against
@@ -43,6 +50,7 @@
 // most CG_GMAX partials.
 #pragma once
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <type_traits>
 
@@ -78,8 +86,11 @@ struct CgState {
     int bdone[CG_MAXB];   // per-batch mode: CgStatus of each batch
     int biters[CG_MAXB];
     double berr[CG_MAXB];
-    T alpha_fin[CG_MAXB]; // alpha of a batch's last iteration ...
-    int xpend[CG_MAXB];   // ... whose x update the finalize kernel applies
+    // status and per-batch status as K1 of the running iteration found them:
+    // K3 decides from these, since its block (0, 0) changes the live ones
+    // while the other blocks of the same launch still read
+    int status_k1;
+    int bdone_k1[CG_MAXB];
     int status;
     int iters;
     int it;        // index of the running iteration (three-launch form)
@@ -87,6 +98,23 @@ struct CgState {
     int batches;
     int per_batch;
     double err;
+};
+
+// The host's view of a solve, in host-mapped coherent memory: the kernels
+// that decide the state publish it here at the end of every CG_CHUNK-th
+// iteration and when the solve ends, so the host can throttle its launches
+// and read the result without a copy.
+constexpr int CG_CHUNK = 8;
+struct CgMirror {
+    int status;
+    int iters;
+    int it_done;  // last iteration whose convergence test has run (-1: none)
+    int pad;
+    double err;
+    int bdone[CG_MAXB];
+    int biters[CG_MAXB];
+    double berr[CG_MAXB];
+    int xset[CG_MAXB];  // batch's x holds a value (guess, zeros or an update)
 };
 
 // Sum of G partials (stride 'stride' between consecutive ones) in a fixed
@@ -267,9 +295,72 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_kernel(int n, const int* __res
     }
 }
 
+// Setup without a guess (x_0 = 0, r_0 = b, p_0 = M^-1 b), vectors of VW
+// elements: reads b and M^-1 only. r_0 and x_0 are written only when asked
+// (the three-launch form reads b for r_0 and writes x first in K3). Same
+// partial layout as cg_setup_kernel: (|b|^2, |r|^2, r.p) per block.
+template <typename T, int VW>
+__global__ __launch_bounds__(CG_BS) void cg_setup_zero_kernel(int n, const T* __restrict__ invd, int istride,
+                                                              const T* __restrict__ b, T* __restrict__ x,
+                                                              T* __restrict__ r, T* __restrict__ p, int write_x,
+                                                              int write_r, T* __restrict__ part)
+{
+    typedef T V __attribute__((ext_vector_type(VW)));
+    __shared__ T red[2 * CG_BS / 64];
+    const int bb = blockIdx.y;
+    const long long off = (long long)bb * n;
+    const long long nv = n / VW;
+    const V* bv = reinterpret_cast<const V*>(b + off);
+    const V* iv = reinterpret_cast<const V*>(invd + (long long)bb * istride);
+    V* xv = reinterpret_cast<V*>(x + off);
+    V* rv = reinterpret_cast<V*>(r + off);
+    V* pv = reinterpret_cast<V*>(p + off);
+    const long long stride = (long long)gridDim.x * CG_BS;
+    V vb[CG_VPT], vd[CG_VPT];
+    V sbb = V(0), srp = V(0);
+    for (long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x; i0 < nv; i0 += stride * CG_VPT) {
+#pragma unroll
+        for (int j = 0; j < CG_VPT; ++j) {
+            const long long i = min(i0 + j * stride, nv - 1);  // unconditional loads
+            vb[j] = bv[i];
+            vd[j] = iv[i];
+        }
+#pragma unroll
+        for (int j = 0; j < CG_VPT; ++j) {
+            const long long i = i0 + j * stride;
+            if (i < nv) {
+                const V q = vd[j] * vb[j];
+                pv[i] = q;
+                if (write_r) rv[i] = vb[j];
+                if (write_x) xv[i] = V(0);
+                sbb += vb[j] * vb[j];
+                srp += vb[j] * q;
+            }
+        }
+    }
+    T a = sbb[0], c = srp[0];
+#pragma unroll
+    for (int k = 1; k < VW; ++k) {
+        a += sbb[k];
+        c += srp[k];
+    }
+    block_sum2<CG_BS>(a, c, red);
+    if (threadIdx.x == 0) {
+        const long long base = 3ll * ((long long)bb * gridDim.x + blockIdx.x);
+        part[base + 0] = a;
+        part[base + 1] = a;  // r_0 = b
+        part[base + 2] = c;
+    }
+}
+
+// Folds the setup's partials and initialises the whole state (and the
+// host's mirror of it): early outs for a zero right-hand side or a guess
+// that already converged, the thresholds and r.z of iteration 0.
+// x_set: x holds a value already (the guess, or zeros written by the setup).
 template <typename T>
 __global__ __launch_bounds__(CG_BS) void cg_setup_finalize_kernel(int G, int B, T tol, const T* __restrict__ part,
-                                                                  CgState<T>* st)
+                                                                  CgState<T>* st, int max_iter, int per_batch,
+                                                                  CgMirror* mirror, int x_set)
 {
     __shared__ T red[CG_BS / 64];
     __shared__ T tot[3][CG_MAXB];
@@ -297,19 +388,51 @@ __global__ __launch_bounds__(CG_BS) void cg_setup_finalize_kernel(int G, int B, 
             st->biters[k] = 0;
             st->berr[k] = zero ? 0.0 : sqrt((double)q);
             all_done = all_done && st->bdone[k] != CG_RUNNING;
+            mirror->bdone[k] = st->bdone[k];
+            mirror->biters[k] = 0;
+            mirror->berr[k] = st->berr[k];
+            mirror->xset[k] = x_set;
         }
         st->iters = 0;
+        st->it = -1;
+        st->max_iter = max_iter;
         st->batches = B;
-        if (st->per_batch ? (all_done && all_zero) : all_zero) { st->status = CG_EARLY_ZERO; st->err = 0.0; }
-        else if (st->per_batch ? all_done : all_conv) { st->status = CG_EARLY_GUESS; st->err = sqrt((double)m); }
-        else if (st->max_iter <= 0) { st->status = CG_MAXIT; st->err = sqrt((double)m); }
-        else { st->status = CG_RUNNING; st->err = sqrt((double)m); }
+        st->per_batch = per_batch;
+        int status;
+        if (per_batch ? (all_done && all_zero) : all_zero) { status = CG_EARLY_ZERO; st->err = 0.0; }
+        else if (per_batch ? all_done : all_conv) { status = CG_EARLY_GUESS; st->err = sqrt((double)m); }
+        else if (max_iter <= 0) { status = CG_MAXIT; st->err = sqrt((double)m); }
+        else { status = CG_RUNNING; st->err = sqrt((double)m); }
+        st->status = status;
+        st->status_k1 = status;
+        mirror->iters = 0;
+        mirror->it_done = -1;
+        mirror->err = st->err;
+        mirror->status = status;
+    }
+}
+
+// K1 of every form records the state it starts from (see CgState) and owns
+// the iteration counter that K2 and K3 read: no launch argument changes
+// from one iteration to the next, so iterations can be replayed from a
+// captured graph (K1 itself never reads the counter, K2 / K3 read it after
+// this launch has completed)
+template <typename T>
+__device__ __forceinline__ void cg_k1_snapshot(CgState<T>* st, int bb)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (bb == 0) {
+            const int s0 = st->status;
+            st->status_k1 = s0;
+            if (s0 == CG_RUNNING) st->it = st->it + 1;
+        }
+        st->bdone_k1[bb] = st->bdone[bb];
     }
 }
 
 // ---------------------------------------------------------------- K1 (three-launch form)
 // tmp = A p, per-block partial p.tmp
-template <typename VT, typename T>
+template <typename VT, typename T, int MNT = 0>
 __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __restrict__ JA, const int* __restrict__ IA,
                                                         const VT* __restrict__ vals, long long vstride,
                                                         const T* __restrict__ p, T* __restrict__ tmp,
@@ -317,11 +440,7 @@ __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __rest
                                                         const T* __restrict__ diag)
 {
     const int bb = blockIdx.y;
-    // K1 owns the iteration counter that K2 and K3 read: no launch argument
-    // changes from one iteration to the next, so iterations can be replayed
-    // from a captured graph (K1 itself never reads it, K2 / K3 read it after
-    // this launch has completed)
-    if (blockIdx.x == 0 && bb == 0 && threadIdx.x == 0 && st->status == CG_RUNNING) st->it = st->it + 1;
+    cg_k1_snapshot(st, bb);
     if (batch_inactive(st, bb)) return;
     constexpr int WPB = CG_BS / 64;
     __shared__ T lds[WPB][CG_EPT * 64];
@@ -335,11 +454,11 @@ __global__ __launch_bounds__(CG_BS) void cg_spmv_kernel(int n, const int* __rest
     ChunkWalk w(div_up_dev(n, 64), WPB);
     if (w.valid()) {
         WaveChunk<VT, CG_EPT> cur, nxt;
-        chunk_load<VT, CG_EPT, false>(cur, JA, IA, vb, w.cur * 64, n);
+        chunk_load<VT, CG_EPT, MNT>(cur, JA, IA, vb, w.cur * 64, n);
         for (;;) {
             const int nc = w.peek_next();
             const bool more = nc < w.hi;
-            if (more) chunk_load<VT, CG_EPT, false>(nxt, JA, IA, vb, nc * 64, n);
+            if (more) chunk_load<VT, CG_EPT, MNT>(nxt, JA, IA, vb, nc * 64, n);
             T ap = chunk_compute<VT, T, CG_EPT>(cur, JA, IA, vb, [pb](int c) { return pb[c]; }, slice);
             if (lane < cur.nr) {
                 const long long row = cur.r0 + lane;
@@ -366,7 +485,7 @@ __global__ __launch_bounds__(CG_BS) void cg_dense_kernel(int n, const T* __restr
 {
     static_assert(CG_BS == DN_BS, "dense tiles use the CG block size");
     const int bb = blockIdx.y;
-    if (blockIdx.x == 0 && bb == 0 && threadIdx.x == 0 && st->status == CG_RUNNING) st->it = st->it + 1;
+    cg_k1_snapshot(st, bb);
     if (batch_inactive(st, bb)) return;
     __shared__ T red[4 * 64];
     __shared__ T red2[CG_BS / 64];
@@ -419,7 +538,8 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
                                                         const T* __restrict__ r, const T* __restrict__ pprev,
                                                         T* __restrict__ pnext, T* __restrict__ tmp,
                                                         T* __restrict__ part1, const T* __restrict__ part2,
-                                                        CgState<T>* st, const T* __restrict__ diag)
+                                                        CgState<T>* st, const T* __restrict__ diag,
+                                                        CgMirror* __restrict__ mirror)
 {
     const int bb = blockIdx.y;
     if (st->status != CG_RUNNING) return;
@@ -477,6 +597,12 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
                     st->biters[bb] = it;
                 }
             }
+            const bool pub = all || it >= st->max_iter || (prev % CG_CHUNK) == CG_CHUNK - 1;
+            if (per_batch && !own_prev && (pub || own_conv)) {
+                mirror->berr[bb] = st->berr[bb];
+                mirror->biters[bb] = st->biters[bb];
+                mirror->bdone[bb] = st->bdone[bb];
+            }
             if (bb == 0) {
                 if (!per_batch) st->err = sqrt((double)m);
                 if (all) {
@@ -485,6 +611,12 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
                 } else {
                     st->iters = it;
                     if (it >= st->max_iter) st->status = CG_MAXIT;
+                }
+                if (pub) {
+                    mirror->err = st->err;
+                    mirror->iters = st->iters;
+                    mirror->it_done = prev;
+                    mirror->status = st->status;
                 }
             }
             // with the reference's rule a converged batch keeps iterating until
@@ -522,6 +654,20 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
 
 // Vector width of K2 / K3: VW elements per load (VW = 16 / sizeof(T) when n
 // is a multiple of it and every vector is 16-byte aligned, else 1).
+// loads / stores of a vector stream, non-temporal (streaming) when NT
+template <bool NT, typename V>
+__device__ __forceinline__ V cg_ld(const V* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void cg_st(V* p, V v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 template <typename T, int VW> struct CgVec { typedef T type __attribute__((ext_vector_type(VW))); };
 template <typename T> struct CgVec<T, 1> { typedef T type; };
 template <typename T, int VW>
@@ -537,13 +683,18 @@ __device__ __forceinline__ T cg_hsum(const typename CgVec<T, VW>::type& v)
 // reduced, so the streaming loads overlap that reduction's latency.
 // UPDATE_X = false (three-launch form): x += alpha p is left to K3, which
 // reads p anyway, so this pass reads r, Ap, M^-1 and writes r only.
-template <typename T, bool UPDATE_X, int VW = 1>
+// VNT: streams loaded / stored non-temporally (bit 0 r, 1 tmp, 2 M^-1,
+// 3 the r store)
+template <typename T, bool UPDATE_X, int VW = 1, int VNT = 0>
 __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int G1, const T* __restrict__ part1,
                                                           const T* __restrict__ invd, int istride,
                                                           const T* __restrict__ p, const T* __restrict__ tmp,
                                                           T* __restrict__ x, T* __restrict__ r,
-                                                          T* __restrict__ part2, const CgState<T>* st)
+                                                          T* __restrict__ part2, const CgState<T>* st,
+                                                          const T* __restrict__ r0)
 {
+    // r0: when set, iteration 0 reads r_0 from it (r_0 = b without a guess:
+    // the setup then does not write r)
     typedef typename CgVec<T, VW>::type V;
     const int bb = blockIdx.y;
     if (batch_inactive(st, bb)) return;
@@ -557,6 +708,7 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int
     const V* pb = reinterpret_cast<const V*>(p + off);
     const V* tb = reinterpret_cast<const V*>(tmp + off);
     V* rb = reinterpret_cast<V*>(r + off);
+    const V* rs = (it == 0 && r0) ? reinterpret_cast<const V*>(r0 + off) : rb;
     const V* ib = reinterpret_cast<const V*>(invd + (long long)bb * istride);
     const long long stride = (long long)gridDim.x * CG_BS;
     long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
@@ -570,9 +722,9 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int
                 xv[j] = ok ? xb[i] : V(0);
                 pv[j] = ok ? pb[i] : V(0);
             }
-            rv[j] = ok ? rb[i] : V(0);
-            tv[j] = ok ? tb[i] : V(0);
-            dv[j] = ok ? ib[i] : V(0);
+            rv[j] = ok ? cg_ld<(VNT & 1) != 0>(rs + i) : V(0);
+            tv[j] = ok ? cg_ld<(VNT & 2) != 0>(tb + i) : V(0);
+            dv[j] = ok ? cg_ld<(VNT & 4) != 0>(ib + i) : V(0);
         }
     };
     const T rz = st->absNew[it & 1][bb];
@@ -587,7 +739,7 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int
             if (i < nv) {
                 const V rn = rv[j] - alpha * tv[j];
                 if (UPDATE_X) xo[i] = xv[j] + alpha * pv[j];
-                rb[i] = rn;
+                cg_st<(VNT & 8) != 0>(rb + i, rn);
                 vrr += rn * rn;
                 vrz += rn * (dv[j] * rn);
             }
@@ -608,31 +760,41 @@ __global__ __launch_bounds__(CG_BS) void cg_update_kernel(int n, int it_arg, int
 // convergence test of iteration it and p = M^-1 r + beta p
 // XK3: K3 also performs x += alpha p (K2 then skips x and p: one vector pass
 // less per iteration; pays off for large systems, where the pass costs more
-// than the extra partial reduction K3 needs for alpha).
-template <typename T, bool XK3, int VW = 1>
+// than the extra partial reduction K3 needs for alpha). In a batch's last
+// iteration (converged, or the iteration limit) K3 applies only that x
+// update. x0_zero: x_0 = 0 was not written, so iteration 0 stores alpha p
+// without reading x.
+// The early outs and the decisions read the state as K1 of this iteration
+// found it (status_k1, bdone_k1), never what block (0, 0) of this launch
+// writes, so every block of a batch reaches the same decision.
+// Block (0, 0) publishes the global state and block (0, b) batch b's to
+// the host mirror at the end of every CG_CHUNK-th iteration and at the end.
+// VNT: streams loaded / stored non-temporally (bit 0 p, 1 r, 2 M^-1, 3 x,
+// 4 the x store, 5 the p store)
+template <typename T, bool XK3, int VW = 1, int VNT = 0>
 __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, int B, int G2, const T* __restrict__ part2,
                                                            const T* __restrict__ invd, int istride,
                                                            const T* __restrict__ r, T* __restrict__ p,
                                                            CgState<T>* st, int G1, const T* __restrict__ part1,
-                                                           T* __restrict__ x)
+                                                           T* __restrict__ x, CgMirror* __restrict__ mirror,
+                                                           int x0_zero)
 {
     const int bb = blockIdx.y;
-    if (st->status != CG_RUNNING) return;
+    if (st->status_k1 != CG_RUNNING) return;
     const int it = it_arg >= 0 ? it_arg : st->it;  // < 0: the counter kept by K1
     const bool per_batch = st->per_batch != 0;
     // a batch that finished in an earlier iteration has nothing to update;
     // block (0,0) still runs, it owns the global status
-    if (per_batch && st->bdone[bb] != CG_RUNNING && !(blockIdx.x == 0 && bb == 0)) return;
+    if (per_batch && st->bdone_k1[bb] != CG_RUNNING && !(blockIdx.x == 0 && bb == 0)) return;
     __shared__ T red[3 * CG_BS / 64];
     __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB], thr_s[CG_MAXB], rhs_s[CG_MAXB];
     __shared__ int done_s[CG_MAXB];
     // the state the decisions below read, fetched before the reductions so
-    // that its latency overlaps theirs (and every block sees the values as
-    // they were when this launch started)
+    // that its latency overlaps theirs
     if (threadIdx.x < B) {
         thr_s[threadIdx.x] = st->thr[threadIdx.x];
         rhs_s[threadIdx.x] = st->rhs2[threadIdx.x];
-        done_s[threadIdx.x] = per_batch ? st->bdone[threadIdx.x] : (int)CG_RUNNING;
+        done_s[threadIdx.x] = per_batch ? st->bdone_k1[threadIdx.x] : (int)CG_RUNNING;
     }
     const T absOld = st->absNew[it & 1][bb];
     const int max_iter = st->max_iter;
@@ -645,16 +807,18 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
     const V* ib = reinterpret_cast<const V*>(invd + (long long)bb * istride);
     const long long stride = (long long)gridDim.x * CG_BS;
     long long i0 = (long long)blockIdx.x * CG_BS + threadIdx.x;
+    // x_0 = 0 unwritten: iteration 0 stores alpha p without reading x
+    const bool read_x = XK3 && !(x0_zero && it == 0);
     V pv[CG_VPT], rv[CG_VPT], dv[CG_VPT], xv[CG_VPT];
     auto load = [&](long long base) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = base + j * stride;
             const bool ok = i < nv;
-            pv[j] = ok ? pb[i] : V(0);
-            rv[j] = ok ? rb[i] : V(0);
-            dv[j] = ok ? ib[i] : V(0);
-            if (XK3) xv[j] = ok ? xb[i] : V(0);
+            pv[j] = ok ? cg_ld<(VNT & 1) != 0>(pb + i) : V(0);
+            rv[j] = ok ? cg_ld<(VNT & 2) != 0>(rb + i) : V(0);
+            dv[j] = ok ? cg_ld<(VNT & 4) != 0>(ib + i) : V(0);
+            if (XK3) xv[j] = (ok && read_x) ? cg_ld<(VNT & 8) != 0>(xb + i) : V(0);
         }
     };
     load(i0);
@@ -691,51 +855,53 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
     }
     const bool own_prev = done_s[bb] != CG_RUNNING;
     const bool own_conv = own_prev || rr_s[bb] < thr_s[bb];
+    // this batch's last iteration: converged, or the iteration limit
+    const bool last = (per_batch ? own_conv : all) || it + 1 >= max_iter;
+    const bool publish = last || (it % CG_CHUNK) == CG_CHUNK - 1 || (per_batch && own_conv);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (per_batch && !own_prev) {
-            st->berr[bb] = sqrt((double)(rr_s[bb] / rhs_s[bb]));
-            if (own_conv) {
-                st->bdone[bb] = CG_CONVERGED;
-                st->biters[bb] = it;  // the converging step is not counted
-            } else {
-                st->biters[bb] = it + 1;
+            const double e = sqrt((double)(rr_s[bb] / rhs_s[bb]));
+            const int bi = own_conv ? it : it + 1;  // the converging step is not counted
+            st->berr[bb] = e;
+            st->biters[bb] = bi;
+            if (own_conv) st->bdone[bb] = CG_CONVERGED;
+            if (publish) {
+                mirror->berr[bb] = e;
+                mirror->biters[bb] = bi;
+                mirror->bdone[bb] = own_conv ? (int)CG_CONVERGED : (int)CG_RUNNING;
             }
         }
+        if (XK3 && !own_prev && read_x == false) mirror->xset[bb] = 1;
         if (bb == 0) {
-            if (!per_batch) st->err = sqrt((double)m);
-            if (all) {
-                st->status = CG_CONVERGED;
-                st->iters = it;
-            } else {
-                st->iters = it + 1;
-                if (it + 1 >= max_iter) st->status = CG_MAXIT;
+            const int status = all ? (int)CG_CONVERGED : (it + 1 >= max_iter ? (int)CG_MAXIT : (int)CG_RUNNING);
+            const int iters = all ? it : it + 1;
+            const double err = per_batch ? st->err : sqrt((double)m);
+            st->err = err;
+            st->iters = iters;
+            st->status = status;
+            const bool pub_all = publish || status != CG_RUNNING;
+            if (pub_all) {
+                mirror->err = err;
+                mirror->iters = iters;
+                mirror->it_done = it;
+                mirror->status = status;
             }
         }
         // with the reference's rule a converged batch keeps iterating until
         // all have converged, so its r.z is recorded like any other
         if (!(per_batch && own_conv)) st->absNew[(it + 1) & 1][bb] = rz_s[bb];
     }
-    // The last iteration of this batch (converged, or the iteration limit):
-    // its x update is left to cg_finalize_kernel, which applies the recorded
-    // alpha to the (then unchanged) p. Every block reaches the same decision
-    // from the same partials, so a block that returned at the top because
-    // block 0 already flagged the end in this launch skips nothing needed.
-    const bool last = (per_batch ? own_conv : all) || it + 1 >= max_iter;
-    if (last) {
-        if (XK3 && blockIdx.x == 0 && threadIdx.x == 0 && !own_prev) {
-            st->alpha_fin[bb] = alpha;
-            st->xpend[bb] = 1;
-        }
-        return;
-    }
+    // block (0, 0) of a batch that finished earlier only evaluates; without
+    // XK3 the last iteration's x update was K2's
+    if (own_prev || (last && !XK3)) return;
     const T beta = rz_s[bb] / absOld;
     while (i0 < nv) {
 #pragma unroll
         for (int j = 0; j < CG_VPT; ++j) {
             const long long i = i0 + j * stride;
             if (i < nv) {
-                if (XK3) xb[i] = xv[j] + alpha * pv[j];
-                pb[i] = dv[j] * rv[j] + beta * pv[j];
+                if (XK3) cg_st<(VNT & 16) != 0>(xb + i, xv[j] + alpha * pv[j]);
+                if (!last) cg_st<(VNT & 32) != 0>(pb + i, dv[j] * rv[j] + beta * pv[j]);
             }
         }
         i0 += stride * CG_VPT;
@@ -743,33 +909,14 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
     }
 }
 
-// End of a solve, per batch: x = 0 for a zero right-hand side, or the
-// pending x += alpha p of the batch's last iteration (three-launch form).
+// After a solve: x = 0 for the batches in 'mask' (a zero right-hand side,
+// or no iteration ran on an x_0 = 0 that was never written)
 template <typename T>
-__global__ void cg_finalize_kernel(int n, T* x, const T* p, const CgState<T>* st)
+__global__ void cg_zero_batches_kernel(int n, T* x, unsigned long long mask)
 {
     const int bb = blockIdx.y;
-    const bool zero = st->status == CG_EARLY_ZERO || (st->per_batch && st->bdone[bb] == CG_EARLY_ZERO);
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const long long k = (long long)bb * n + i;
-    if (zero) x[k] = T(0);
-    else if (st->xpend[bb]) x[k] = x[k] + st->alpha_fin[bb] * p[k];
-}
-
-template <typename T>
-__global__ void cg_state_init_kernel(CgState<T>* st, int max_iter, int batches, int per_batch)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->status = CG_RUNNING;
-        st->iters = 0;
-        st->it = -1;
-        st->max_iter = max_iter;
-        st->batches = batches;
-        st->per_batch = per_batch;
-        st->err = 0.0;
-        for (int k = 0; k < batches && k < CG_MAXB; ++k) st->xpend[k] = 0;
-    }
+    if (i < n && ((mask >> bb) & 1ull)) x[(long long)bb * n + i] = T(0);
 }
 
 template <typename T>
@@ -793,6 +940,9 @@ __global__ void invdiag_kernel(int n, long long nnz, const int* __restrict__ JA,
 // solver launches directly).
 // A dense operator in place of the CSR matrix (cm_pcg_dense_solve): n x n,
 // batch stride bstride (0: one matrix for all right-hand sides)
+// K3 updates x (instead of K2) from this many unknowns on
+inline bool cg_xk3(int n, int B) { return (long long)n * B >= CG_XK3_MIN; }
+
 template <typename T>
 struct DenseOp {
     const T* A = nullptr;
@@ -803,7 +953,8 @@ struct DenseOp {
 template <typename VT, typename T>
 static void launch_cg_iteration(cm_ctx_t ctx, int n, int B, int G1, int G2, const int* JA, const int* IA,
                                 const VT* vals, long long vstride, const T* invd, int istride, T* p, T* t, T* x, T* r,
-                                T* part1, T* part2, CgState<T>* st, const T* diag, const DenseOp<T>* dense = nullptr)
+                                T* part1, T* part2, CgState<T>* st, const T* diag, const DenseOp<T>* dense,
+                                const T* r0, CgMirror* mirror, int x0_zero)
 {
     if (dense && dense->row_major)
         cg_dense_kernel<T, true><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, dense->A, dense->bstride, p, t, part1, st);
@@ -816,21 +967,22 @@ static void launch_cg_iteration(cm_ctx_t ctx, int n, int B, int G1, int G2, cons
     constexpr int VW = 16 / (int)sizeof(T);
     const bool wide = n % VW == 0 && ((((uintptr_t)invd | (uintptr_t)p | (uintptr_t)t | (uintptr_t)x |
                                         (uintptr_t)r) & 15) == 0);
-    if ((long long)n * B >= CG_XK3_MIN && wide) {
-        cg_update_kernel<T, false, VW><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p,
-                                                                               t, x, r, part2, st);
-        cg_pupdate_kernel<T, true, VW><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride,
-                                                                               r, p, st, G1, part1, x);
-    } else if ((long long)n * B >= CG_XK3_MIN) {
+    if (cg_xk3(n, B) && wide && (((uintptr_t)r0 & 15) == 0)) {
+        // Ap is dead after this pass: streaming loads keep it out of the caches
+        cg_update_kernel<T, false, VW, 2><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride,
+                                                                                  p, t, x, r, part2, st, r0);
+        cg_pupdate_kernel<T, true, VW><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(
+            n, -1, B, G2, part2, invd, istride, r, p, st, G1, part1, x, mirror, x0_zero);
+    } else if (cg_xk3(n, B)) {
         cg_update_kernel<T, false><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x,
-                                                                           r, part2, st);
+                                                                           r, part2, st, r0);
         cg_pupdate_kernel<T, true><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r, p,
-                                                                           st, G1, part1, x);
+                                                                           st, G1, part1, x, mirror, x0_zero);
     } else {
         cg_update_kernel<T, true><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, G1, part1, invd, istride, p, t, x,
-                                                                          r, part2, st);
+                                                                          r, part2, st, r0);
         cg_pupdate_kernel<T, false><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, -1, B, G2, part2, invd, istride, r,
-                                                                            p, st, G1, part1, x);
+                                                                            p, st, G1, part1, x, mirror, 0);
     }
 }
 
@@ -895,87 +1047,122 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         istride = opt.vals_batched ? n : 0;
     }
     const bool fused = opt.fused;
+    // without a guess, x_0 = 0 and r_0 = b are not written: K2 reads b for
+    // r_0, and in the three-launch form with the x update in K3, iteration 0
+    // stores alpha p without reading x (fused and small forms get x_0 = 0
+    // from the setup)
+    const bool xk3 = !fused && cg_xk3(n, B);
+    const int x0_zero = (!use_guess && xk3) ? 1 : 0;
+    const T* r0 = use_guess ? nullptr : b;
+    void* mirror_host = nullptr;
+    void* mirror_dev = nullptr;
+    CM_TRY(pinned(ctx, &mirror_host, &mirror_dev));
+    static_assert(sizeof(CgMirror) <= CM_PINNED_BYTES, "the solve mirror must fit the pinned page");
+    volatile CgMirror* hm = (volatile CgMirror*)mirror_host;
+    CgMirror* mirror = (CgMirror*)mirror_dev;
+    // the previous solve on this context left its final state here
+    hm->status = CG_RUNNING;
+    hm->it_done = -1;
     int launches = 0;
     auto t0 = std::chrono::steady_clock::now();
-    cg_state_init_kernel<T><<<1, 64, 0, ctx->stream>>>(st, max_iter, B, opt.per_batch);
-    CM_LAUNCH_CHECK();
-    const T* ax_pre = nullptr;
-    if (dense && use_guess) {
-        // t = A x_0, read by the setup kernel
-        // one column range: the same summation order as the iteration kernel
-        if (dense->row_major)
-            dense_gemv_kernel<T, true><<<dim3(dense_grid(n), B), DN_BS, 0, ctx->stream>>>(
-                n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN, 1, n, nullptr);
+    int Gs = G1;
+    if (use_guess) {
+        const T* ax_pre = nullptr;
+        if (dense) {
+            // t = A x_0, read by the setup kernel
+            // one column range: the same summation order as the iteration kernel
+            if (dense->row_major)
+                dense_gemv_kernel<T, true><<<dim3(dense_grid(n), B), DN_BS, 0, ctx->stream>>>(
+                    n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN, 1, n, nullptr);
+            else
+                dense_gemv_kernel<T, false><<<dim3(dense_grid(n), B), DN_BS, 0, ctx->stream>>>(
+                    n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN, 1, n, nullptr);
+            CM_LAUNCH_CHECK();
+            ax_pre = t;
+            launches += 1;
+        }
+        cg_setup_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, invd, istride, b,
+                                                                       x, r, p, 1, part1, diag, ax_pre);
+    } else {
+        constexpr int VW = 16 / (int)sizeof(T);
+        Gs = G2;
+        if (n % VW == 0 && ((((uintptr_t)invd | (uintptr_t)b | (uintptr_t)x | (uintptr_t)r | (uintptr_t)p) & 15) == 0))
+            cg_setup_zero_kernel<T, VW><<<dim3(Gs, B), CG_BS, 0, ctx->stream>>>(n, invd, istride, b, x, r, p,
+                                                                                !x0_zero, 0, part1);
         else
-            dense_gemv_kernel<T, false><<<dim3(dense_grid(n), B), DN_BS, 0, ctx->stream>>>(
-                n, n, dense->A, dense->bstride, x, 1, t, CM_ASSIGN, 1, n, nullptr);
-        CM_LAUNCH_CHECK();
-        ax_pre = t;
-        launches += 1;
+            cg_setup_zero_kernel<T, 1><<<dim3(Gs, B), CG_BS, 0, ctx->stream>>>(n, invd, istride, b, x, r, p,
+                                                                               !x0_zero, 0, part1);
     }
-    cg_setup_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, invd, istride, b, x, r,
-                                                                   p, use_guess, part1, diag, ax_pre);
     CM_LAUNCH_CHECK();
-    cg_setup_finalize_kernel<T><<<1, CG_BS, 0, ctx->stream>>>(G1, B, (T)tol, part1, st);
+    cg_setup_finalize_kernel<T><<<1, CG_BS, 0, ctx->stream>>>(Gs, B, (T)tol, part1, st, max_iter,
+                                                              opt.per_batch ? 1 : 0, mirror, x0_zero ? 0 : 1);
     CM_LAUNCH_CHECK();
-    launches += 3;
-    int* hstat = nullptr;
-    CM_TRY(pinned(ctx, (void**)&hstat));
-    hstat[0] = 0;
-    hipEvent_t ev;
-    CM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    const int CHUNK = 8;
-    bool pending = false;
+    launches += 2;
+    // At most two chunks of iterations in flight: before issuing more, wait
+    // until the mirror shows the chunk before the last one tested (kernels
+    // publish every CG_CHUNK-th iteration), and stop once the solve ended.
+    // If the stream drains first, every launched iteration has run.
+    auto wait_mirror = [&](int need) {
+        count(CM_CNT_HOST_SYNC);
+        for (unsigned k = 1;; ++k) {
+            if (hm->status != CG_RUNNING || hm->it_done >= need) return;
+            if ((k & 255) == 0 && hipStreamQuery(ctx->stream) == hipSuccess) return;
+            __builtin_ia32_pause();
+        }
+    };
     int it = 0;
     while (it < max_iter) {
-        int todo = std::min(CHUNK, max_iter - it);
+        int todo = std::min(CG_CHUNK, max_iter - it);
         for (int k = 0; k < todo; ++k, ++it) {
             if (!fused) {
                 launch_cg_iteration<VT, T>(ctx, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x, r, part1,
-                                           part2, st, diag, dense);
+                                           part2, st, diag, dense, r0, mirror, x0_zero);
                 continue;
             }
             T* pcur = pbuf[it & 1];
             if (it == 0)
                 cg_iter_kernel<VT, T, true><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(
-                    n, it, B, G2, JA, IA, vals, vstride, invd, istride, r, pcur, pcur, t, part1, part2, st, diag);
+                    n, it, B, G2, JA, IA, vals, vstride, invd, istride, r, pcur, pcur, t, part1, part2, st, diag,
+                    mirror);
             else
                 cg_iter_kernel<VT, T, false><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(
                     n, it, B, G2, JA, IA, vals, vstride, invd, istride, r, pbuf[(it - 1) & 1], pcur, t, part1,
-                    part2, st, diag);
+                    part2, st, diag, mirror);
             cg_update_kernel<T, true><<<dim3(G2, B), CG_BS, 0, ctx->stream>>>(n, it, G1, part1, invd, istride, pcur,
-                                                                              t, x, r, part2, st);
+                                                                              t, x, r, part2, st, r0);
         }
         CM_LAUNCH_CHECK();
         launches += (fused ? 2 : 3) * todo;
-        // lagged status check: read the previous chunk's word without draining
-        // the queue, then request the current one
-        if (pending) {
-            count(CM_CNT_HOST_SYNC);
-            CM_HIP(hipEventSynchronize(ev));
-            if (hstat[0] != CG_RUNNING) break;
+        if (it > CG_CHUNK && it < max_iter) {
+            wait_mirror(it - CG_CHUNK - 1);
+            if (hm->status != CG_RUNNING) break;
         }
-        CM_HIP(hipMemcpyAsync(hstat, &st->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        CM_HIP(hipEventRecord(ev, ctx->stream));
-        pending = true;
     }
     // completes the last launched iteration (convergence test / iteration
     // limit); a no-op once the status is final
     if (fused && it >= 1) {
         cg_iter_kernel<VT, T, false><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(
             n, it, B, G2, JA, IA, vals, vstride, invd, istride, r, pbuf[(it - 1) & 1], pbuf[it & 1], t, part1, part2,
-            st, diag);
+            st, diag, mirror);
         CM_LAUNCH_CHECK();
         launches += 1;
     }
-    cg_finalize_kernel<T><<<dim3(div_up(n, 256), B), 256, 0, ctx->stream>>>(n, x, p, st);
-    CM_LAUNCH_CHECK();
-    launches += 1;
-    CgState<T> hs;
-    CM_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     CM_TRY(::cm::host_sync(ctx));
-    CM_HIP(hipEventDestroy(ev));
+    // x = 0 where the right-hand side was zero, or where x_0 = 0 was never
+    // written because no iteration ran for the batch
+    unsigned long long zmask = 0;
+    for (int k = 0; k < B; ++k)
+        if (hm->status == CG_EARLY_ZERO || (opt.per_batch && hm->bdone[k] == CG_EARLY_ZERO) || !hm->xset[k])
+            zmask |= 1ull << k;
+    if (zmask) {
+        cg_zero_batches_kernel<T><<<dim3(div_up(n, 256), B), 256, 0, ctx->stream>>>(n, x, zmask);
+        CM_LAUNCH_CHECK();
+        launches += 1;
+        CM_TRY(::cm::host_sync(ctx));
+    }
     auto t1 = std::chrono::steady_clock::now();
+    CgMirror hs;
+    memcpy(&hs, (const void*)hm, sizeof(hs));
     if (opt.per_batch) {
         double m = 0;
         int mi = 0;

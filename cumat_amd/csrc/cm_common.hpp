@@ -76,7 +76,8 @@ struct cm_context {
     // Device scratch (grow-only) for reduction partials and solver state.
     cm::Scratch scratch[4];
     // Pinned host word(s) for flag read-back.
-    void* pinned = nullptr;
+    void* pinned = nullptr;      // host-mapped, coherent page (CM_PINNED_BYTES)
+    void* pinned_dev = nullptr;  // its device address
     // Cached device properties.
     int cu_count = 0;
 };
@@ -91,7 +92,10 @@ int fill_zero(cm_ctx_t ctx, void* p, size_t bytes);
 // Grow-only scratch slot 'slot' of at least 'bytes' bytes (256-aligned).
 int scratch(cm_ctx_t ctx, int slot, size_t bytes, void** out);
 // Pinned host memory of the context (at least 256 bytes).
-int pinned(cm_ctx_t ctx, void** out);
+// A per-context page of host memory that kernels write and the host reads
+// while they run (coherent: device stores reach it without a cache flush).
+constexpr size_t CM_PINNED_BYTES = 4096;
+int pinned(cm_ctx_t ctx, void** host, void** dev);
 // Caching device allocator (the same pool as cm_malloc / cm_free).
 int dalloc(cm_ctx_t ctx, size_t bytes, void** out);
 // Stream-ordered on ctx's stream; ctx == nullptr: no queued work uses p.

@@ -272,10 +272,23 @@ int scratch(cm_ctx_t ctx, int slot, size_t bytes, void** out)
 int dalloc(cm_ctx_t ctx, size_t bytes, void** out) { return device_alloc(ctx->device, bytes, ctx->stream, out); }
 int dfree(cm_ctx_t ctx, void* p) { return device_free(p, ctx ? ctx->stream : nullptr, ctx != nullptr); }
 
-int pinned(cm_ctx_t ctx, void** out)
+int pinned(cm_ctx_t ctx, void** host, void** dev)
 {
-    if (!ctx->pinned) CM_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
-    *out = ctx->pinned;
+    if (!ctx->pinned) {
+        void* h = nullptr;
+        CM_HIP(hipHostMalloc(&h, CM_PINNED_BYTES, hipHostMallocMapped | hipHostMallocCoherent));
+        void* d = nullptr;
+        const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+        if (e != hipSuccess) {
+            hipHostFree(h);
+            return fail(CM_E_HIP, "hipHostGetDevicePointer: %s", hipGetErrorString(e));
+        }
+        memset(h, 0, CM_PINNED_BYTES);
+        ctx->pinned = h;
+        ctx->pinned_dev = d;
+    }
+    *host = ctx->pinned;
+    *dev = ctx->pinned_dev;
     return CM_OK;
 }
 

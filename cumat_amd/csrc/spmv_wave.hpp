@@ -56,11 +56,12 @@ __device__ __forceinline__ X ld_matrix(const X* p)
     else return *p;
 }
 
-// NT: non-temporal (streaming) loads of the matrix. A single product streams
-// a matrix that is not read again soon; an iterative solver reads the same
-// matrix every iteration, and plain loads let the 256 MB last-level cache
-// keep part of it from one iteration to the next.
-template <typename VT, int EPT, bool NT = true>
+// NT: which matrix streams are loaded non-temporally (streaming: bit 0 the
+// column indices, bit 1 the values). A single product streams a matrix that
+// is not read again soon; an iterative solver reads the same matrix every
+// iteration, and plain loads let the 256 MB last-level cache keep part of it
+// from one iteration to the next.
+template <typename VT, int EPT, int NT = 3>
 __device__ __forceinline__ void chunk_load(WaveChunk<VT, EPT>& C, const int* __restrict__ JA,
                                            const int* __restrict__ IA, const VT* __restrict__ vals, int r0,
                                            int nrows)
@@ -77,8 +78,8 @@ __device__ __forceinline__ void chunk_load(WaveChunk<VT, EPT>& C, const int* __r
         for (int j = 0; j < EPT; ++j) {
             const int k = C.s + lane + 64 * j;
             const bool ok = k < C.e;
-            C.col[j] = ok ? ld_matrix<NT>(IA + k) : 0;
-            C.v[j] = ok ? ld_matrix<NT>(vals + k) : VT(0);
+            C.col[j] = ok ? ld_matrix<(NT & 1) != 0>(IA + k) : 0;
+            C.v[j] = ok ? ld_matrix<(NT & 2) != 0>(vals + k) : VT(0);
         }
     }
 }

@@ -2,6 +2,7 @@
 oracle is the plain method restated in oracle/ (FP64 CG per system), closed
 forms, and central finite differences for the gradient."""
 import ctypes as C
+import gc
 
 import numpy as np
 import pytest
@@ -292,6 +293,9 @@ def test_stream_ordered_frees_issue_no_host_sync(gpu):
     out = cm.DeviceArray(n, dtype=np.float32)
     acc = cm.DeviceArray(n, dtype=np.float32)
     acc.fill_zero()
+    gc.collect()  # arrays of earlier tests must not be freed inside the counted loop
+    warm = cm.DeviceArray(1, dtype=np.float32)
+    cm.dot(a, b, out=warm)  # sizes the reduction scratch before the counted loop
     cm.synchronize()
     cm.reset_counters()
     for k in range(20):
