@@ -169,6 +169,50 @@ __device__ __forceinline__ void reduce_partials_pair(const T* __restrict__ part,
     block_sum2<CG_BS>(a, b, lds);
 }
 
+// Several right-hand sides (B > 1): the (|r|^2, r.z) pair sums of all
+// batches in one pass instead of B block reductions. CG_BS / B threads per
+// batch each fold a fixed stride of the batch's G pairs in order (G <=
+// CG_GMAX / B, so at most CG_PPT + 1 loads each), then thread k adds its
+// batch's sums in thread order: the same fixed order, and so the same
+// result, in every block. lds: 2 * CG_BS entries; rr / rz get B entries
+// (valid after a barrier).
+template <typename T>
+__device__ __forceinline__ void reduce_pairs_batched(const T* __restrict__ part, int G, int B, T* rr, T* rz, T* lds)
+{
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    const int tpb = CG_BS / B;
+    const int k = threadIdx.x / tpb, sub = threadIdx.x - k * tpb;
+    T a = T(0), c = T(0);
+    if (k < B) {
+        const T* pb = part + 2ll * k * G;
+        T2 v[CG_PPT + 1];
+#pragma unroll
+        for (int j = 0; j <= CG_PPT; ++j) {
+            const int i = sub + j * tpb;
+            v[j] = (i < G) ? *reinterpret_cast<const T2*>(pb + 2ll * i) : T2{T(0), T(0)};
+        }
+#pragma unroll
+        for (int j = 0; j <= CG_PPT; ++j)
+            if (sub + j * tpb < G) {
+                a += v[j].x;
+                c += v[j].y;
+            }
+    }
+    lds[threadIdx.x] = a;
+    lds[CG_BS + threadIdx.x] = c;
+    __syncthreads();
+    if ((int)threadIdx.x < B) {
+        const int b0 = threadIdx.x * tpb;
+        T sa = lds[b0], sc = lds[CG_BS + b0];
+        for (int j = 1; j < tpb; ++j) {
+            sa += lds[b0 + j];
+            sc += lds[CG_BS + b0 + j];
+        }
+        rr[threadIdx.x] = sa;
+        rz[threadIdx.x] = sc;
+    }
+}
+
 // One right-hand side: the p.Ap partials (stride 1) and the (|r|^2, r.z)
 // pairs loaded together and summed with shared barriers; lds needs
 // 3*CG_BS/64 entries. Same order and results as reduce_partials followed by
@@ -551,6 +595,7 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
     __shared__ T lds[WPB][CG_EPT * 64];
     __shared__ T red[2 * CG_BS / 64];
     __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB];
+    __shared__ T pairs[2 * CG_BS];
     T* slice = lds[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const long long off = (long long)bb * n;
@@ -566,10 +611,12 @@ __global__ __launch_bounds__(CG_BS) void cg_iter_kernel(int n, int it, int B, in
     T beta = T(0);
     if (!FIRST) {
         const int prev = it - 1;
-        for (int k = 0; k < B; ++k) {
+        if (B == 1) {
             T a, c;
-            reduce_partials_pair(part2 + 2ll * k * G2, G2, a, c, red);
-            if (threadIdx.x == 0) { rr_s[k] = a; rz_s[k] = c; }
+            reduce_partials_pair(part2, G2, a, c, red);
+            if (threadIdx.x == 0) { rr_s[0] = a; rz_s[0] = c; }
+        } else {
+            reduce_pairs_batched(part2, G2, B, rr_s, rz_s, pairs);
         }
         __syncthreads();
         bool all = true;
@@ -789,6 +836,7 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
     __shared__ T red[3 * CG_BS / 64];
     __shared__ T rr_s[CG_MAXB], rz_s[CG_MAXB], thr_s[CG_MAXB], rhs_s[CG_MAXB];
     __shared__ int done_s[CG_MAXB];
+    __shared__ T pairs[2 * CG_BS];
     // the state the decisions below read, fetched before the reductions so
     // that its latency overlaps theirs
     if (threadIdx.x < B) {
@@ -833,10 +881,12 @@ __global__ __launch_bounds__(CG_BS) void cg_pupdate_kernel(int n, int it_arg, in
         if (threadIdx.x == 0) { rr_s[0] = a; rz_s[0] = c; }
     } else {
         if (XK3) pap = reduce_partials(part1 + (long long)bb * G1, G1, 1, red);
-        for (int k = 0; k < B; ++k) {
+        if (B == 1) {
             T a, c;
-            reduce_partials_pair(part2 + 2ll * k * G2, G2, a, c, red);
-            if (threadIdx.x == 0) { rr_s[k] = a; rz_s[k] = c; }
+            reduce_partials_pair(part2, G2, a, c, red);
+            if (threadIdx.x == 0) { rr_s[0] = a; rz_s[0] = c; }
+        } else {
+            reduce_pairs_batched(part2, G2, B, rr_s, rz_s, pairs);
         }
     }
     __syncthreads();
@@ -1011,11 +1061,6 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     CM_REQUIRE(B >= 1 && B <= CG_MAXB, CM_E_UNSUPPORTED, "CG: %d right-hand sides (max %d)", B, CG_MAXB);
     Temps tmp(ctx);
     const long long N = (long long)n * B;
-    void *r_, *p_, *t_, *st_, *p1_, *p2_;
-    CM_TRY(tmp.get(sizeof(T) * N, &r_));
-    CM_TRY(tmp.get(sizeof(T) * 2 * N, &p_));  // p of even / odd iterations
-    CM_TRY(tmp.get(sizeof(T) * N, &t_));
-    CM_TRY(tmp.get(sizeof(CgState<T>), &st_));
     // grids: multiples of 8 (one share per XCD); with B batches each gets at
     // most CG_GMAX/B blocks so that every reduction folds <= CG_GMAX partials.
     // K1 walks 64-row chunks with 4 wavefronts per block; the setup walks
@@ -1023,27 +1068,46 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     const int gcap = std::max(8, CG_GMAX / B);
     const int G1 = walk_grid(div_up(div_up(n, 64), CG_BS / 64), gcap);
     const int G2 = walk_grid(div_up(n, (long long)CG_BS * CG_VPT), std::min(gcap, CG_G2MAX));
-    CM_TRY(tmp.get(sizeof(T) * 3 * (size_t)B * std::max(G1, G2), &p1_));
-    CM_TRY(tmp.get(sizeof(T) * 2 * (size_t)B * G2, &p2_));
-    T* r = (T*)r_;
-    T* pbuf[2] = {(T*)p_, (T*)p_ + N};
+    // reduce_pairs_batched folds at most CG_PPT + 1 pairs per thread
+    CM_REQUIRE(B == 1 || G2 <= (CG_PPT + 1) * (CG_BS / B), CM_E_UNSUPPORTED, "CG: grid %d too large for %d batches",
+               G2, B);
+    // one block for all work vectors (one allocator round trip per solve):
+    // r, p of even / odd iterations, Ap, the state, the two partial sets,
+    // and the inverse diagonal when the caller gives none; 256-byte aligned
+    const bool own_invd = invd == nullptr;
+    const int nbi = opt.vals_batched ? B : 1;
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t sz[7] = {al(sizeof(T) * N), al(sizeof(T) * 2 * N), al(sizeof(T) * N), al(sizeof(CgState<T>)),
+                          al(sizeof(T) * 3 * (size_t)B * std::max(G1, G2)), al(sizeof(T) * 2 * (size_t)B * G2),
+                          own_invd ? al(sizeof(T) * (size_t)n * nbi) : 0};
+    size_t total = 0;
+    for (size_t v : sz) total += v;
+    char* blk = nullptr;
+    CM_TRY(tmp.get(total, (void**)&blk));
+    char* cur = blk;
+    auto take = [&](int k) {
+        char* q = cur;
+        cur += sz[k];
+        return q;
+    };
+    T* r = (T*)take(0);
+    T* p_ = (T*)take(1);
+    T* pbuf[2] = {p_, p_ + N};
     T* p = pbuf[0];
-    T* t = (T*)t_;
-    T* part1 = (T*)p1_;
-    T* part2 = (T*)p2_;
-    CgState<T>* st = (CgState<T>*)st_;
+    T* t = (T*)take(2);
+    CgState<T>* st = (CgState<T>*)take(3);
+    T* part1 = (T*)take(4);
+    T* part2 = (T*)take(5);
+    T* id_ = (T*)take(6);
     const long long vstride = opt.vals_batched ? (long long)nnz : 0;
     // a caller-given inverse diagonal has one row of n per batch when the
     // values are batched
     int istride = opt.vals_batched ? n : 0;
-    if (!invd) {
-        void* id_;
-        const int nb = opt.vals_batched ? B : 1;
-        CM_TRY(tmp.get(sizeof(T) * (size_t)n * nb, &id_));
-        invdiag_kernel<T><<<dim3(div_up(n, 256), nb), 256, 0, ctx->stream>>>(n, vstride, JA, IA, (const T*)vals,
-                                                                             (T*)id_);
+    if (own_invd) {
+        invdiag_kernel<T><<<dim3(div_up(n, 256), nbi), 256, 0, ctx->stream>>>(n, vstride, JA, IA, (const T*)vals,
+                                                                              id_);
         CM_LAUNCH_CHECK();
-        invd = (const T*)id_;
+        invd = id_;
         istride = opt.vals_batched ? n : 0;
     }
     const bool fused = opt.fused;
@@ -1160,6 +1224,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         launches += 1;
         CM_TRY(::cm::host_sync(ctx));
     }
+    tmp.drained = true;  // the stream is idle: the work block returns to the cache at once
     auto t1 = std::chrono::steady_clock::now();
     CgMirror hs;
     memcpy(&hs, (const void*)hm, sizeof(hs));

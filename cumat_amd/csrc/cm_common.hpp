@@ -114,6 +114,9 @@ int host_sync(cm_ctx_t ctx);
 struct Temps {
     cm_ctx_t ctx;
     std::vector<void*> ptrs;
+    // set once the caller has synchronised the stream after the last use:
+    // the blocks then return to the cache without a recorded event
+    bool drained = false;
     explicit Temps(cm_ctx_t c) : ctx(c) {}
     int get(size_t bytes, void** out)
     {
@@ -123,7 +126,7 @@ struct Temps {
     }
     ~Temps()
     {
-        for (void* p : ptrs) (void)dfree(ctx, p);
+        for (void* p : ptrs) (void)dfree(drained ? nullptr : ctx, p);
     }
 };
 
