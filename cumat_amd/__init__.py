@@ -27,7 +27,7 @@ __all__ = [
     "capi", "DeviceArray", "Context", "default_context", "synchronize", "Expr", "as_expr",
     "assign", "cast", "select", "lower", "evaluate", "SparseMatrix", "SparsityPattern",
     "ELLMatrix", "SparseExpressionView", "sparse_view", "reduce", "dot", "squared_norm", "norm", "ConjugateGradient",
-    "DiagonalPreconditioner", "pcg_batched", "pcg_ir", "dense", "counters", "reset_counters",
+    "DiagonalPreconditioner", "pcg_batched", "pcg_ir", "dense", "counters", "reset_counters", "jit_mode",
     "device_info",
 ]
 
@@ -35,7 +35,8 @@ __all__ = [
 def counters() -> dict:
     """Profiling counters (cuMat::Profiling, cuMat/src/Profiling.h:23-109)."""
     names = ["DeviceMemAlloc", "DeviceMemFree", "EvalAny", "EvalCwise", "EvalCwiseSparse",
-             "EvalReduction", "EvalMatmulSparse", "KernelLaunch", "EvalMatmul", "HostSync"]
+             "EvalReduction", "EvalMatmulSparse", "KernelLaunch", "EvalMatmul", "HostSync",
+             "JitCompile", "JitLaunch"]
     out = {}
     for i, n in enumerate(names):
         v = C.c_longlong(0)
@@ -46,6 +47,13 @@ def counters() -> dict:
 
 def reset_counters() -> None:
     capi.call("cm_counters_reset")
+
+
+def jit_mode(mode: int = -1) -> int:
+    """Run-time generated kernels for component-wise programs (cm_jit_mode):
+    0 never, 1 from 2^18 elements on (default), 2 always; a negative value
+    only queries. Returns the previous mode."""
+    return int(capi.lib().cm_jit_mode(int(mode)))
 
 
 def device_info(device: int = 0) -> dict:

@@ -88,6 +88,7 @@ _PROTOS = {
     "cm_cwise_eval": ([_VP, C.POINTER(Leaf), _I32, C.POINTER(Instr), _I32, _VP, _I32, _I32, _I32,
                        _I32, _I64, _I64, _I64, _I32], _I32),
     "cm_lincomb_f32": ([_VP, _I32, C.POINTER(_VP), C.POINTER(C.c_float), _VP, _I64], _I32),
+    "cm_jit_mode": ([_I32], _I32),
     "cm_reduce": ([_VP, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _VP, _I32, _VP], _I32),
     "cm_reduce_expr": ([_VP, C.POINTER(Leaf), _I32, C.POINTER(Instr), _I32, _I32, _I32, _I32, _I32, _I32, _I32,
                         _I32, _VP], _I32),

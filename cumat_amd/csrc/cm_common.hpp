@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "cumat_amd.h"
+#include "cwise_ops.hpp"
 
 namespace cm {
 
@@ -306,27 +307,7 @@ template <typename T> struct Load<unsigned short, T> {
 template <typename T>
 __device__ __forceinline__ T assign_mode(int mode, T old, T v)
 {
-    switch (mode) {
-    case CM_ADD: return old + v;
-    case CM_SUB: return old - v;
-    case CM_MUL: return old * v;
-    case CM_DIV: return old / v;
-    default: return v;
-    }
-}
-template <>
-__device__ __forceinline__ int assign_mode<int>(int mode, int old, int v)
-{
-    switch (mode) {
-    case CM_ADD: return (int)((unsigned)old + (unsigned)v);
-    case CM_SUB: return (int)((unsigned)old - (unsigned)v);
-    case CM_MUL: return (int)((unsigned)old * (unsigned)v);
-    case CM_DIV: return old / v;
-    case CM_MOD: return old % v;
-    case CM_AND: return old & v;
-    case CM_OR: return old | v;
-    default: return v;
-    }
+    return ops::assign1<T>(mode, old, v);
 }
 
 }  // namespace cm

@@ -16,6 +16,9 @@
 // evaluation into the non-zeros of a CSR matrix
 // (CwiseCSREvaluationKernel, cuMat/src/SparseEvaluation.h:14-34).
 #include "cwise_core.hpp"
+#include "jit.hpp"
+
+#include <cinttypes>
 
 namespace cm {
 
@@ -264,6 +267,191 @@ static int launch_csr_w(cm_ctx_t ctx, const CwProgram& P, const int* JA, const i
     return CM_OK;
 }
 
+
+// ------------------------------------------------ generated kernels
+// The flat case of cwise_kernel<false, NL, true> (32-bit programs; every
+// leaf is direct or a broadcast scalar and the destination is contiguous)
+// as straight-line code for one program: the same element mapping (8 per
+// thread, two 16-byte groups 256 elements apart), the same 16-byte
+// non-temporal loads and stores where the interpreter takes them, the
+// same per-element semantics (cwise_ops.hpp). Returns CM_OK when launched,
+// JIT_DECLINED when the interpreter should run the program.
+constexpr int JIT_DECLINED = 1;
+
+struct JitCwiseArgs {
+    const void* leaf[CM_CWISE_MAX_LEAVES];
+    void* dst;
+    long long n;
+};
+
+static const char* jit_raw_load(int dtype)
+{
+    // raw register bits of one stored element (the interpreter's load_raw)
+    return dtype == CM_BOOL ? "(R)((const unsigned char*)A.leaf[%d])[%s]" : "((const uint32_t*)A.leaf[%d])[%s]";
+}
+
+// Source and cache key of the generated kernel cm_jit_cwise for P.
+int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& body)
+{
+    const int dst_dt = P.dst_dtype;
+    if (dst_dt != CM_F32 && dst_dt != CM_I32 && dst_dt != CM_BOOL) return JIT_DECLINED;
+    const bool dst_al = ((uintptr_t)P.dst & 15) == 0;
+    key.clear();
+    body.clear();
+    char buf[256];
+    snprintf(buf, sizeof(buf), "cwflat32|d%d%c|m%d|", dst_dt, dst_al ? 'a' : 'u', P.mode);
+    key += buf;
+    // leaf value expressions and the per-leaf load code
+    std::string decls, loads_full, loads_tail, scalars;
+    std::string names[CM_CWISE_MAX_LEAVES];
+    const char* namep[CM_CWISE_MAX_LEAVES];
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL) return JIT_DECLINED;
+        const bool scalar = (P.leaf_scalar >> l) & 1;
+        const bool al = L.dtype != CM_BOOL && ((uintptr_t)L.ptr & 15) == 0;
+        snprintf(buf, sizeof(buf), "L%d%c%c|", L.dtype, scalar ? 's' : 'd', al ? 'a' : 'u');
+        key += buf;
+        if (scalar) {
+            snprintf(buf, sizeof(buf), "c%d", l);
+            names[l] = buf;
+            char ld[160];
+            snprintf(ld, sizeof(ld), jit_raw_load(L.dtype), l, "0");
+            scalars += "    const R c" + std::to_string(l) + " = " + ld + ";\n";
+        } else {
+            snprintf(buf, sizeof(buf), "v%d[k]", l);
+            names[l] = buf;
+            decls += "        R v" + std::to_string(l) + "[8];\n";
+            char ld[160];
+            if (al) {
+                // streamed once: 16-byte non-temporal loads of both groups
+                snprintf(buf, sizeof(buf),
+                         "            u = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[%d] + e0));\n"
+                         "            v%d[0] = u.x; v%d[1] = u.y; v%d[2] = u.z; v%d[3] = u.w;\n", l, l, l, l, l);
+                loads_full += buf;
+                snprintf(buf, sizeof(buf),
+                         "            u = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[%d] + e0 + 256));\n"
+                         "            v%d[4] = u.x; v%d[5] = u.y; v%d[6] = u.z; v%d[7] = u.w;\n", l, l, l, l, l);
+                loads_full += buf;
+            } else {
+                snprintf(ld, sizeof(ld), jit_raw_load(L.dtype), l, "EL(e0, k)");
+                loads_full += std::string("#pragma unroll\n            for (int k = 0; k < 8; ++k) v") +
+                              std::to_string(l) + "[k] = " + ld + ";\n";
+            }
+            snprintf(ld, sizeof(ld), jit_raw_load(L.dtype), l, "min(EL(e0, k), n - 1)");
+            loads_tail += std::string("#pragma unroll\n            for (int k = 0; k < 8; ++k) v") +
+                          std::to_string(l) + "[k] = " + ld + ";\n";
+        }
+        namep[l] = names[l].c_str();
+    }
+    for (int i = 0; i < P.ncode; ++i) {
+        const cm_instr& I = P.code[i];
+        uint64_t fb;
+        memcpy(&fb, &I.fimm, 8);
+        snprintf(buf, sizeof(buf), "%d,%d,%d,%d,%d,%d,%" PRIx64 ",%" PRIx64 ";", I.opcode, I.dtype, I.fn, I.src,
+                 I.arg, I.flags, fb, (uint64_t)I.iimm);
+        key += buf;
+    }
+    std::string prog;
+    if (!jit::program_body(P, namep, prog)) return JIT_DECLINED;
+    // store of element k at e (the interpreter's store_elem)
+    std::string store_one;
+    if (dst_dt == CM_F32) {
+        store_one = "{ float* p = (float*)A.dst + e; const float v = getf(res[k]); "
+                    "*p = (MODE == CM_ASSIGN) ? v : assign1<float>(MODE, *p, v); }";
+    } else if (dst_dt == CM_I32) {
+        store_one = "{ int* p = (int*)A.dst + e; const int v = geti(res[k]); "
+                    "*p = (MODE == CM_ASSIGN) ? v : assign1<int>(MODE, *p, v); }";
+    } else {
+        store_one = "{ unsigned char* p = (unsigned char*)A.dst + e; int v = geti(res[k]) != 0; "
+                    "if (MODE == CM_AND) v = v && *p; else if (MODE == CM_OR) v = v || *p; *p = (unsigned char)v; }";
+    }
+    std::string store_vec;
+    if (dst_al && dst_dt != CM_BOOL) {
+        const char* comb = dst_dt == CM_F32
+                               ? "__float_as_uint(assign1<float>(MODE, __uint_as_float(old[j]), __uint_as_float(res[4 * q + j])))"
+                               : "(uint32_t)assign1<int>(MODE, (int)old[j], (int)res[4 * q + j])";
+        store_vec = std::string(
+                        "        if (full) {\n"
+                        "#pragma unroll\n"
+                        "            for (int q = 0; q < 2; ++q) {\n"
+                        "                u4v* d = (u4v*)((uint32_t*)A.dst + e0 + 256 * q);\n"
+                        "                u4v u;\n"
+                        "                if (MODE == CM_ASSIGN) {\n"
+                        "                    u.x = res[4 * q]; u.y = res[4 * q + 1]; u.z = res[4 * q + 2]; u.w = res[4 * q + 3];\n"
+                        "                } else {\n"
+                        "                    const u4v old = *d;\n"
+                        "#pragma unroll\n"
+                        "                    for (int j = 0; j < 4; ++j) u[j] = ") +
+                    comb +
+                    ";\n"
+                    "                }\n"
+                    "                __builtin_nontemporal_store(u, d);\n"
+                    "            }\n"
+                    "            continue;\n"
+                    "        }\n";
+    }
+    snprintf(buf, sizeof(buf), "constexpr int MODE = %d;\n", P.mode);
+    body = std::string("typedef unsigned int u4v __attribute__((ext_vector_type(4)));\n") + buf +
+           "struct JitCwiseArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) +
+           "]; void* dst; long long n; };\n"
+           "#define EL(e0, k) ((e0) + ((k) >> 2) * 256 + ((k) & 3))\n"
+           "extern \"C\" __global__ __launch_bounds__(256) void cm_jit_cwise(const JitCwiseArgs A)\n{\n"
+           "    typedef uint32_t R;\n"
+           "    const long long n = A.n;\n"
+           "    const long long stride = (long long)gridDim.x * (256 * 8);\n" +
+           scalars +
+           "    const long long tid = (long long)blockIdx.x * 256 + threadIdx.x;\n"
+           "    for (long long e0 = (tid >> 6) * 512 + (tid & 63) * 4; e0 < n; e0 += stride) {\n"
+           "        const bool full = EL(e0, 7) < n;\n" +
+           decls +
+           "        if (full) {\n"
+           "            u4v u;\n            (void)u;\n" +
+           loads_full +
+           "        } else {\n" +
+           loads_tail +
+           "        }\n" +
+           "        R res[8];\n"
+           "#pragma unroll\n"
+           "        for (int k = 0; k < 8; ++k) {\n"
+           "            R acc = 0, t = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;\n"
+           "            (void)t; (void)s0; (void)s1; (void)s2; (void)s3;\n" +
+           prog +
+           "            res[k] = acc;\n"
+           "        }\n" +
+           store_vec +
+           "#pragma unroll\n"
+           "        for (int k = 0; k < 8; ++k) {\n"
+           "            const long long e = EL(e0, k);\n"
+           "            if (e < n) " + store_one + "\n"
+           "        }\n"
+           "    }\n"
+           "}\n";
+    return CM_OK;
+}
+
+static int jit_cwise_flat(cm_ctx_t ctx, const CwProgram& P)
+{
+    std::string key, body;
+    if (jit_cwise_flat_source(P, key, body) != CM_OK) return JIT_DECLINED;
+    jit::Kernel k;
+    if (jit::get(key, body, "cm_jit_cwise", &k) != 0) return JIT_DECLINED;
+    JitCwiseArgs A;
+    memset(&A, 0, sizeof(A));
+    for (int l = 0; l < P.nleaves; ++l) A.leaf[l] = P.leaves[l].ptr;
+    A.dst = P.dst;
+    A.n = P.n;
+    // the interpreter's grid: one block per 2048 elements, at most 64 per CU
+    long long blocks = (P.n + 2047) / 2048;
+    const long long cap = (long long)(ctx->cu_count > 0 ? ctx->cu_count : 256) * 64;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    if (jit::launch(k, (unsigned)blocks, 256, ctx->stream, &A, sizeof(A)) != 0)
+        return fail(CM_E_HIP, "cm_cwise_eval: launch of the generated kernel failed");
+    count(CM_CNT_KERNEL_LAUNCH);
+    return CM_OK;
+}
+
 // ------------------------------------------------ linear combination
 constexpr int LC_MAX = 16;
 struct LincombArgs {
@@ -440,6 +628,10 @@ static int cwise_dense(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const c
         cm::LincombArgs A;
         int k = 0;
         if (cm::match_lincomb(P, A, k)) return cm::lincomb_run(ctx, A, k, (float*)dst, n);
+    }
+    if (!wide && flat && P.dst_direct && cm::jit::want(n)) {
+        const int s = cm::jit_cwise_flat(ctx, P);
+        if (s != cm::JIT_DECLINED) return s;
     }
     if (wide) return cm::launch_dense_w<true>(ctx, P, flat);
     return cm::launch_dense_w<false>(ctx, P, flat);

@@ -6,6 +6,7 @@
 // (cuMat/src/Matrix.h:587-627).
 #pragma once
 #include "cm_common.hpp"
+#include "cwise_ops.hpp"
 
 #include <cmath>
 #include <type_traits>
@@ -74,39 +75,54 @@ __device__ __forceinline__ void load_raw(const cm_leaf& L, long long off, Reg<W>
     }
 }
 
+// The per-element semantics live in cwise_ops.hpp (shared with the
+// run-time generated kernels); the functions below dispatch the runtime
+// dtype / functor codes of an instruction onto them, one wave-uniform switch
+// per instruction and a loop over the register's elements inside each case.
+#define CW_EACH(...) _Pragma("unroll") for (int k = 0; k < CwVec<W>::v; ++k) { __VA_ARGS__; } break
+
 // Convert register contents between dtypes (CastingOp, cuMat/src/UnaryOps.h:248-312).
+template <bool W, int FROM>
+__device__ __forceinline__ void convert_from(Reg<W>& r, int to)
+{
+    switch (to) {
+    case CM_F32: CW_EACH(r.v[k] = ops::convert1<FROM, CM_F32>(r.v[k]));
+    case CM_F64: CW_EACH(r.v[k] = ops::convert1<FROM, CM_F64>(r.v[k]));
+    case CM_I32: CW_EACH(r.v[k] = ops::convert1<FROM, CM_I32>(r.v[k]));
+    case CM_BOOL: CW_EACH(r.v[k] = ops::convert1<FROM, CM_BOOL>(r.v[k]));
+    default: break;
+    }
+}
 template <bool W>
 __device__ __forceinline__ void convert(Reg<W>& r, int from, int to)
 {
     if (from == to) return;
-#pragma unroll
-    for (int k = 0; k < CwVec<W>::v; ++k) {
-        double d = 0.0;
-        long long i = 0;
-        bool isf = false;
-        switch (from) {
-        case CM_F32: d = getf(r, k); isf = true; break;
-        case CM_F64: d = getd(r, k); isf = true; break;
-        case CM_I32: i = geti(r, k); break;
-        case CM_BOOL: i = geti(r, k) != 0; break;
-        default: break;
-        }
-        switch (to) {
-        case CM_F32:
-            if (from == CM_F64) setf(r, k, (float)d);
-            else setf(r, k, isf ? (float)d : (float)i);
-            break;
-        case CM_F64: setd(r, k, isf ? d : (double)i); break;
-        case CM_I32: seti(r, k, isf ? (int)d : (int)i); break;
-        case CM_BOOL: seti(r, k, isf ? (d != 0.0) : (i != 0)); break;
-        default: break;
-        }
+    switch (from) {
+    case CM_F32: convert_from<W, CM_F32>(r, to); break;
+    case CM_F64: convert_from<W, CM_F64>(r, to); break;
+    case CM_I32: convert_from<W, CM_I32>(r, to); break;
+    case CM_BOOL: convert_from<W, CM_BOOL>(r, to); break;
+    default: convert_from<W, -1>(r, to); break;  // unknown source: converts 0
     }
 }
 
 // ------------------------------------------------------- binary functors
 // cuMat/src/BinaryOps.h:230-243 and the comparison functors
 // (cuMat/src/BinaryOps.h:336-341); comparisons and logical ops yield bool.
+template <bool W, int DT>
+__device__ __forceinline__ void binary_dt(int fn, Reg<W>& a, const Reg<W>& x, const Reg<W>& y)
+{
+#define CW_BIN(FN) case FN: CW_EACH(a.v[k] = ops::binary1<DT, FN>(a.v[k], x.v[k], y.v[k]))
+    switch (fn) {
+    CW_BIN(CM_BIN_ADD); CW_BIN(CM_BIN_SUB); CW_BIN(CM_BIN_MUL); CW_BIN(CM_BIN_DIV);
+    CW_BIN(CM_BIN_MOD); CW_BIN(CM_BIN_POW); CW_BIN(CM_BIN_MIN); CW_BIN(CM_BIN_MAX);
+    CW_BIN(CM_BIN_AND); CW_BIN(CM_BIN_OR); CW_BIN(CM_BIN_XOR); CW_BIN(CM_BIN_LAND);
+    CW_BIN(CM_BIN_LOR); CW_BIN(CM_BIN_EQ); CW_BIN(CM_BIN_NE); CW_BIN(CM_BIN_LT);
+    CW_BIN(CM_BIN_GT); CW_BIN(CM_BIN_LE); CW_BIN(CM_BIN_GE);
+    default: break;
+    }
+#undef CW_BIN
+}
 template <bool W>
 __device__ __forceinline__ void binary(int fn, int dt, Reg<W>& a, const Reg<W>& b, bool rev)
 {
@@ -118,151 +134,37 @@ __device__ __forceinline__ void binary(int fn, int dt, Reg<W>& a, const Reg<W>& 
         x.v[k] = rev ? b.v[k] : a.v[k];
         y.v[k] = rev ? a.v[k] : b.v[k];
     }
-#define CW_LOOP(STMT) _Pragma("unroll") for (int k = 0; k < CwVec<W>::v; ++k) { STMT; } break
-    if (dt == CM_F32) {
-        switch (fn) {
-        case CM_BIN_ADD: CW_LOOP(setf(a, k, getf(x, k) + getf(y, k)));
-        case CM_BIN_SUB: CW_LOOP(setf(a, k, getf(x, k) - getf(y, k)));
-        case CM_BIN_MUL: CW_LOOP(setf(a, k, getf(x, k) * getf(y, k)));
-        case CM_BIN_DIV: CW_LOOP(setf(a, k, getf(x, k) / getf(y, k)));
-        case CM_BIN_POW: CW_LOOP(setf(a, k, powf(getf(x, k), getf(y, k))));
-        case CM_BIN_MIN: CW_LOOP(setf(a, k, getf(x, k) < getf(y, k) ? getf(x, k) : getf(y, k)));
-        case CM_BIN_MAX: CW_LOOP(setf(a, k, getf(x, k) > getf(y, k) ? getf(x, k) : getf(y, k)));
-        case CM_BIN_EQ: CW_LOOP(seti(a, k, getf(x, k) == getf(y, k)));
-        case CM_BIN_NE: CW_LOOP(seti(a, k, getf(x, k) != getf(y, k)));
-        case CM_BIN_LT: CW_LOOP(seti(a, k, getf(x, k) < getf(y, k)));
-        case CM_BIN_GT: CW_LOOP(seti(a, k, getf(x, k) > getf(y, k)));
-        case CM_BIN_LE: CW_LOOP(seti(a, k, getf(x, k) <= getf(y, k)));
-        case CM_BIN_GE: CW_LOOP(seti(a, k, getf(x, k) >= getf(y, k)));
-        default: break;
-        }
-    } else if (dt == CM_F64) {
-        if constexpr (W) {
-            switch (fn) {
-            case CM_BIN_ADD: CW_LOOP(setd(a, k, getd(x, k) + getd(y, k)));
-            case CM_BIN_SUB: CW_LOOP(setd(a, k, getd(x, k) - getd(y, k)));
-            case CM_BIN_MUL: CW_LOOP(setd(a, k, getd(x, k) * getd(y, k)));
-            case CM_BIN_DIV: CW_LOOP(setd(a, k, getd(x, k) / getd(y, k)));
-            case CM_BIN_POW: CW_LOOP(setd(a, k, pow(getd(x, k), getd(y, k))));
-            case CM_BIN_MIN: CW_LOOP(setd(a, k, getd(x, k) < getd(y, k) ? getd(x, k) : getd(y, k)));
-            case CM_BIN_MAX: CW_LOOP(setd(a, k, getd(x, k) > getd(y, k) ? getd(x, k) : getd(y, k)));
-            case CM_BIN_EQ: CW_LOOP(a.v[k] = (uint32_t)(getd(x, k) == getd(y, k)));
-            case CM_BIN_NE: CW_LOOP(a.v[k] = (uint32_t)(getd(x, k) != getd(y, k)));
-            case CM_BIN_LT: CW_LOOP(a.v[k] = (uint32_t)(getd(x, k) < getd(y, k)));
-            case CM_BIN_GT: CW_LOOP(a.v[k] = (uint32_t)(getd(x, k) > getd(y, k)));
-            case CM_BIN_LE: CW_LOOP(a.v[k] = (uint32_t)(getd(x, k) <= getd(y, k)));
-            case CM_BIN_GE: CW_LOOP(a.v[k] = (uint32_t)(getd(x, k) >= getd(y, k)));
-            default: break;
-            }
-        }
-    } else {  // I32 and BOOL: 32-bit integer arithmetic with wrap-around
-        switch (fn) {
-        case CM_BIN_ADD: CW_LOOP(a.v[k] = (uint32_t)x.v[k] + (uint32_t)y.v[k]);
-        case CM_BIN_SUB: CW_LOOP(a.v[k] = (uint32_t)x.v[k] - (uint32_t)y.v[k]);
-        case CM_BIN_MUL: CW_LOOP(a.v[k] = (uint32_t)x.v[k] * (uint32_t)y.v[k]);
-        case CM_BIN_DIV: CW_LOOP(seti(a, k, geti(y, k) != 0 ? geti(x, k) / geti(y, k) : 0));
-        case CM_BIN_MOD: CW_LOOP(seti(a, k, geti(y, k) != 0 ? geti(x, k) % geti(y, k) : 0));
-        case CM_BIN_MIN: CW_LOOP(seti(a, k, geti(x, k) < geti(y, k) ? geti(x, k) : geti(y, k)));
-        case CM_BIN_MAX: CW_LOOP(seti(a, k, geti(x, k) > geti(y, k) ? geti(x, k) : geti(y, k)));
-        case CM_BIN_AND: CW_LOOP(a.v[k] = (uint32_t)x.v[k] & (uint32_t)y.v[k]);
-        case CM_BIN_OR: CW_LOOP(a.v[k] = (uint32_t)x.v[k] | (uint32_t)y.v[k]);
-        case CM_BIN_XOR: CW_LOOP(a.v[k] = (uint32_t)x.v[k] ^ (uint32_t)y.v[k]);
-        case CM_BIN_LAND: CW_LOOP(a.v[k] = (uint32_t)(geti(x, k) != 0 && geti(y, k) != 0));
-        case CM_BIN_LOR: CW_LOOP(a.v[k] = (uint32_t)(geti(x, k) != 0 || geti(y, k) != 0));
-        case CM_BIN_EQ: CW_LOOP(a.v[k] = (uint32_t)(geti(x, k) == geti(y, k)));
-        case CM_BIN_NE: CW_LOOP(a.v[k] = (uint32_t)(geti(x, k) != geti(y, k)));
-        case CM_BIN_LT: CW_LOOP(a.v[k] = (uint32_t)(geti(x, k) < geti(y, k)));
-        case CM_BIN_GT: CW_LOOP(a.v[k] = (uint32_t)(geti(x, k) > geti(y, k)));
-        case CM_BIN_LE: CW_LOOP(a.v[k] = (uint32_t)(geti(x, k) <= geti(y, k)));
-        case CM_BIN_GE: CW_LOOP(a.v[k] = (uint32_t)(geti(x, k) >= geti(y, k)));
-        default: break;
-        }
-    }
+    if (dt == CM_F32) binary_dt<W, CM_F32>(fn, a, x, y);
+    else if (dt == CM_F64) binary_dt<W, CM_F64>(fn, a, x, y);
+    else binary_dt<W, CM_I32>(fn, a, x, y);  // I32 and BOOL: 32-bit integer arithmetic
 }
 
 // -------------------------------------------------------- unary functors
 // cuMat/src/UnaryOps.h:128-225 (device math is HIP's own).
+template <bool W, int DT>
+__device__ __forceinline__ void unary_dt(int fn, Reg<W>& a)
+{
+#define CW_UN(FN) case FN: CW_EACH(a.v[k] = ops::unary1<DT, FN>(a.v[k]))
+    switch (fn) {
+    CW_UN(CM_UN_NEG); CW_UN(CM_UN_ABS); CW_UN(CM_UN_ABS2); CW_UN(CM_UN_INVERSE);
+    CW_UN(CM_UN_INVERSE_CHECK); CW_UN(CM_UN_SQRT); CW_UN(CM_UN_RSQRT); CW_UN(CM_UN_CBRT);
+    CW_UN(CM_UN_EXP); CW_UN(CM_UN_LOG); CW_UN(CM_UN_LOG1P); CW_UN(CM_UN_LOG10);
+    CW_UN(CM_UN_SIN); CW_UN(CM_UN_COS); CW_UN(CM_UN_TAN); CW_UN(CM_UN_ASIN);
+    CW_UN(CM_UN_ACOS); CW_UN(CM_UN_ATAN); CW_UN(CM_UN_SINH); CW_UN(CM_UN_COSH);
+    CW_UN(CM_UN_TANH); CW_UN(CM_UN_FLOOR); CW_UN(CM_UN_CEIL); CW_UN(CM_UN_ROUND);
+    CW_UN(CM_UN_NOT); CW_UN(CM_UN_BITNOT); CW_UN(CM_UN_SIGN);
+    default: break;
+    }
+#undef CW_UN
+}
 template <bool W>
 __device__ __forceinline__ void unary(int fn, int dt, Reg<W>& a)
 {
-    if (dt == CM_F32) {
-        switch (fn) {
-        case CM_UN_NEG: CW_LOOP(setf(a, k, -getf(a, k)));
-        case CM_UN_ABS: CW_LOOP(setf(a, k, fabsf(getf(a, k))));
-        case CM_UN_ABS2: CW_LOOP(setf(a, k, getf(a, k) * getf(a, k)));
-        case CM_UN_INVERSE: CW_LOOP(setf(a, k, 1.0f / getf(a, k)));
-        case CM_UN_INVERSE_CHECK: CW_LOOP(setf(a, k, getf(a, k) == 0.0f ? 1.0f : 1.0f / getf(a, k)));
-        case CM_UN_SQRT: CW_LOOP(setf(a, k, sqrtf(getf(a, k))));
-        case CM_UN_RSQRT: CW_LOOP(setf(a, k, 1.0f / sqrtf(getf(a, k))));
-        case CM_UN_CBRT: CW_LOOP(setf(a, k, cbrtf(getf(a, k))));
-        case CM_UN_EXP: CW_LOOP(setf(a, k, expf(getf(a, k))));
-        case CM_UN_LOG: CW_LOOP(setf(a, k, logf(getf(a, k))));
-        case CM_UN_LOG1P: CW_LOOP(setf(a, k, log1pf(getf(a, k))));
-        case CM_UN_LOG10: CW_LOOP(setf(a, k, log10f(getf(a, k))));
-        case CM_UN_SIN: CW_LOOP(setf(a, k, sinf(getf(a, k))));
-        case CM_UN_COS: CW_LOOP(setf(a, k, cosf(getf(a, k))));
-        case CM_UN_TAN: CW_LOOP(setf(a, k, tanf(getf(a, k))));
-        case CM_UN_ASIN: CW_LOOP(setf(a, k, asinf(getf(a, k))));
-        case CM_UN_ACOS: CW_LOOP(setf(a, k, acosf(getf(a, k))));
-        case CM_UN_ATAN: CW_LOOP(setf(a, k, atanf(getf(a, k))));
-        case CM_UN_SINH: CW_LOOP(setf(a, k, sinhf(getf(a, k))));
-        case CM_UN_COSH: CW_LOOP(setf(a, k, coshf(getf(a, k))));
-        case CM_UN_TANH: CW_LOOP(setf(a, k, tanhf(getf(a, k))));
-        case CM_UN_FLOOR: CW_LOOP(setf(a, k, floorf(getf(a, k))));
-        case CM_UN_CEIL: CW_LOOP(setf(a, k, ceilf(getf(a, k))));
-        case CM_UN_ROUND: CW_LOOP(setf(a, k, roundf(getf(a, k))));
-        case CM_UN_SIGN: CW_LOOP(setf(a, k, (float)((getf(a, k) > 0.f) - (getf(a, k) < 0.f))));
-        case CM_UN_NOT: CW_LOOP(seti(a, k, getf(a, k) == 0.0f));
-        default: break;
-        }
-    } else if (dt == CM_F64) {
-        if constexpr (W) {
-            switch (fn) {
-            case CM_UN_NEG: CW_LOOP(setd(a, k, -getd(a, k)));
-            case CM_UN_ABS: CW_LOOP(setd(a, k, fabs(getd(a, k))));
-            case CM_UN_ABS2: CW_LOOP(setd(a, k, getd(a, k) * getd(a, k)));
-            case CM_UN_INVERSE: CW_LOOP(setd(a, k, 1.0 / getd(a, k)));
-            case CM_UN_INVERSE_CHECK: CW_LOOP(setd(a, k, getd(a, k) == 0.0 ? 1.0 : 1.0 / getd(a, k)));
-            case CM_UN_SQRT: CW_LOOP(setd(a, k, sqrt(getd(a, k))));
-            case CM_UN_RSQRT: CW_LOOP(setd(a, k, 1.0 / sqrt(getd(a, k))));
-            case CM_UN_CBRT: CW_LOOP(setd(a, k, cbrt(getd(a, k))));
-            case CM_UN_EXP: CW_LOOP(setd(a, k, exp(getd(a, k))));
-            case CM_UN_LOG: CW_LOOP(setd(a, k, log(getd(a, k))));
-            case CM_UN_LOG1P: CW_LOOP(setd(a, k, log1p(getd(a, k))));
-            case CM_UN_LOG10: CW_LOOP(setd(a, k, log10(getd(a, k))));
-            case CM_UN_SIN: CW_LOOP(setd(a, k, sin(getd(a, k))));
-            case CM_UN_COS: CW_LOOP(setd(a, k, cos(getd(a, k))));
-            case CM_UN_TAN: CW_LOOP(setd(a, k, tan(getd(a, k))));
-            case CM_UN_ASIN: CW_LOOP(setd(a, k, asin(getd(a, k))));
-            case CM_UN_ACOS: CW_LOOP(setd(a, k, acos(getd(a, k))));
-            case CM_UN_ATAN: CW_LOOP(setd(a, k, atan(getd(a, k))));
-            case CM_UN_SINH: CW_LOOP(setd(a, k, sinh(getd(a, k))));
-            case CM_UN_COSH: CW_LOOP(setd(a, k, cosh(getd(a, k))));
-            case CM_UN_TANH: CW_LOOP(setd(a, k, tanh(getd(a, k))));
-            case CM_UN_FLOOR: CW_LOOP(setd(a, k, floor(getd(a, k))));
-            case CM_UN_CEIL: CW_LOOP(setd(a, k, ceil(getd(a, k))));
-            case CM_UN_ROUND: CW_LOOP(setd(a, k, round(getd(a, k))));
-            case CM_UN_SIGN: CW_LOOP(setd(a, k, (double)((getd(a, k) > 0.0) - (getd(a, k) < 0.0))));
-            case CM_UN_NOT: CW_LOOP(a.v[k] = (uint32_t)(getd(a, k) == 0.0));
-            default: break;
-            }
-        }
-    } else {
-        switch (fn) {
-        case CM_UN_NEG: CW_LOOP(a.v[k] = (uint32_t)(0u - (uint32_t)a.v[k]));
-        case CM_UN_ABS: CW_LOOP(seti(a, k, geti(a, k) < 0 ? (int)(0u - (uint32_t)a.v[k]) : geti(a, k)));
-        case CM_UN_ABS2: CW_LOOP(a.v[k] = (uint32_t)a.v[k] * (uint32_t)a.v[k]);
-        case CM_UN_INVERSE: CW_LOOP(seti(a, k, geti(a, k) != 0 ? 1 / geti(a, k) : 0));
-        case CM_UN_INVERSE_CHECK: CW_LOOP(seti(a, k, geti(a, k) == 0 ? 1 : 1 / geti(a, k)));
-        case CM_UN_FLOOR: case CM_UN_CEIL: case CM_UN_ROUND: break;
-        case CM_UN_NOT: CW_LOOP(a.v[k] = (uint32_t)(geti(a, k) == 0));
-        case CM_UN_BITNOT: CW_LOOP(a.v[k] = ~(uint32_t)a.v[k]);
-        case CM_UN_SIGN: CW_LOOP(seti(a, k, (geti(a, k) > 0) - (geti(a, k) < 0)));
-        default: break;
-        }
-    }
-#undef CW_LOOP
+    if (dt == CM_F32) unary_dt<W, CM_F32>(fn, a);
+    else if (dt == CM_F64) unary_dt<W, CM_F64>(fn, a);
+    else unary_dt<W, CM_I32>(fn, a);
 }
+#undef CW_EACH
 
 // ------------------------------------------------------- program runner
 // Leaf values and the stack live in register vectors that are indexed with a
@@ -355,16 +257,14 @@ __device__ __forceinline__ int run_program(const CwProgram& P, const Bank<W, NL>
             // acc = acc + leaf * imm  (one linear-combination term)
             fetch_leaf<W, NL>(lv, I.arg, t);
             if (I.dtype == CM_F32) {
-                const float c = (float)I.fimm;
 #pragma unroll
-                for (int k = 0; k < CwVec<W>::v; ++k) setf(acc, k, getf(acc, k) + getf(t, k) * c);
+                for (int k = 0; k < CwVec<W>::v; ++k) acc.v[k] = ops::fma1<CM_F32>(acc.v[k], t.v[k], I.fimm, I.iimm);
             } else if (I.dtype == CM_F64) {
 #pragma unroll
-                for (int k = 0; k < CwVec<W>::v; ++k) setd(acc, k, getd(acc, k) + getd(t, k) * I.fimm);
+                for (int k = 0; k < CwVec<W>::v; ++k) acc.v[k] = ops::fma1<CM_F64>(acc.v[k], t.v[k], I.fimm, I.iimm);
             } else {
-                const uint32_t c = (uint32_t)I.iimm;
 #pragma unroll
-                for (int k = 0; k < CwVec<W>::v; ++k) acc.v[k] = (uint32_t)acc.v[k] + (uint32_t)t.v[k] * c;
+                for (int k = 0; k < CwVec<W>::v; ++k) acc.v[k] = ops::fma1<CM_I32>(acc.v[k], t.v[k], I.fimm, I.iimm);
             }
             break;
         }

@@ -214,10 +214,7 @@ __global__ __launch_bounds__(SX_BS) void ell_expr_spmv_kernel(const CwProgram P,
         run_program<W, NL>(P, lv, val);
 #pragma unroll
         for (int j = 0; j < EPT; ++j)
-            if (col[j] >= 0) {
-                const T p = reg_get<T>(val, j) * xb[col[j]];
-                acc += p;
-            }
+            if (col[j] >= 0) acc += reg_get<T>(val, j) * xb[col[j]];  // one expression: fused, as in the plain ELLPACK product
     }
     T* yy = y + r + (long long)rows * b;
     *yy = (mode == CM_ASSIGN) ? acc : assign_mode<T>(mode, *yy, acc);

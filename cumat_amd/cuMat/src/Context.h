@@ -71,8 +71,11 @@ public:
         EvalReduction = CM_CNT_EVAL_REDUCTION,
         EvalMatmulSparse = CM_CNT_EVAL_MATMUL_SPARSE,
         KernelLaunch = CM_CNT_KERNEL_LAUNCH,
-        // no counterpart in cuMat: host waits issued by the library
-        HostSync = CM_CNT_HOST_SYNC
+        // no counterpart in cuMat: host waits issued by the library, and
+        // kernels compiled / launched for component-wise programs at run time
+        HostSync = CM_CNT_HOST_SYNC,
+        JitCompile = CM_CNT_JIT_COMPILE,
+        JitLaunch = CM_CNT_JIT_LAUNCH
     };
     static Profiling& instance()
     {

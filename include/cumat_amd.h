@@ -92,7 +92,11 @@ enum cm_counter {
     /* no reference counterpart: host waits on a stream or event issued by the
      * library (what a stream-ordered caller wants to see stay at zero) */
     CM_CNT_HOST_SYNC = 9,
-    CM_CNT_COUNT = 10
+    /* no reference counterpart: kernels compiled at run time for a
+     * component-wise program, and launches of such kernels (cm_jit_mode) */
+    CM_CNT_JIT_COMPILE = 10,
+    CM_CNT_JIT_LAUNCH = 11,
+    CM_CNT_COUNT = 12
 };
 
 typedef struct cm_context* cm_ctx_t;
@@ -215,6 +219,16 @@ int cm_cwise_eval(cm_ctx_t ctx,
                   int rows, int cols, int batches,
                   int64_t dst_sr, int64_t dst_sc, int64_t dst_sb,
                   int mode);
+
+/* No counterpart in the reference (cuMat compiles one kernel per expression
+ * type at build time). Large component-wise evaluations (cm_cwise_eval on a
+ * destination and leaves of one contiguous layout) run a kernel generated for
+ * the program and compiled at run time with hipRTC, cached per device by the
+ * program's signature; results are bit-identical to the built-in
+ * interpreter. mode: 0 never, 1 from 2^18 elements on (default; the
+ * environment variable CUMAT_JIT sets the initial value), 2 always; any other
+ * value only queries. Returns the previous mode. */
+int cm_jit_mode(int mode);
 
 /* Pre-instantiated fused linear combination out = sum_i coeff[i]*vec[i]
  * (the expression of benchmarks/linear_combination/Implementation_cuMat.cu,

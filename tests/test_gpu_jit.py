@@ -1,0 +1,225 @@
+"""Run-time generated component-wise kernels (cm_jit_mode) against the
+program interpreter, bit for bit: every binary and unary functor in FP32 and
+int32, casts between all 32-bit dtypes, select, the FMA term, every
+assignment mode, bool destinations, broadcast scalars, unaligned leaves and
+lengths that end inside a thread's group of elements. Both paths evaluate
+the instruction semantics of cwise_ops.hpp; the oracle pins the interpreter
+(test_gpu_cwise.py), this file pins the generated kernels to it."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = (1 << 20) + 2051  # not a multiple of a thread's 2048-element span
+
+
+def _instr(op, dt, fn=0, src=0, arg=0, flags=0, fimm=0.0, iimm=0):
+    from cumat_amd import _capi as capi
+    i = capi.Instr()
+    i.opcode, i.dtype, i.fn, i.src, i.arg, i.flags, i.fimm, i.iimm = op, dt, fn, src, arg, flags, fimm, iimm
+    return i
+
+
+class _Leaf:
+    def __init__(self, cm, arr, offset=0, scalar=False):
+        self.arr = arr
+        self.scalar = scalar
+        pad = np.zeros(offset, arr.dtype)
+        self.dev = cm.DeviceArray.from_numpy(np.concatenate([pad, arr]))
+        self.ptr = self.dev.ptr + offset * arr.dtype.itemsize
+
+
+def _eval(cm, leaves, code, dst_np_dtype, mode, init, jit):
+    from cumat_amd import _capi as capi
+    from cumat_amd.device import np_to_dtype
+    n = init.size
+    L = (capi.Leaf * max(1, len(leaves)))()
+    for i, lf in enumerate(leaves):
+        L[i].ptr = lf.ptr
+        L[i].dtype = np_to_dtype(lf.arr.dtype)
+        L[i].kind = capi.LEAF_STRIDED
+        L[i].sr, L[i].sc, L[i].sb = (0, 0, 0) if lf.scalar else (1, n, n)
+    prog = (capi.Instr * len(code))(*code)
+    dst = cm.DeviceArray.from_numpy(init.astype(dst_np_dtype))
+    old = cm.jit_mode(jit)
+    try:
+        capi.call("cm_cwise_eval", None, L, len(leaves), prog, len(code), dst.ptr, np_to_dtype(np.dtype(dst_np_dtype)),
+                  n, 1, 1, 1, n, n, mode)
+        out = dst.numpy()
+    finally:
+        cm.jit_mode(old)
+    return out
+
+
+def _same(cm, leaves, code, dst_dtype, mode=0, init=None, n=N):
+    if init is None:
+        init = np.zeros(n, dst_dtype)
+    cm.reset_counters()
+    ref = _eval(cm, leaves, code, dst_dtype, mode, init, jit=0)
+    assert cm.counters()["JitLaunch"] == 0
+    got = _eval(cm, leaves, code, dst_dtype, mode, init, jit=2)
+    assert cm.counters()["JitLaunch"] == 1, "the generated kernel did not run"
+    assert ref.tobytes() == got.tobytes()
+    return got
+
+
+@pytest.fixture(scope="module")
+def data(gpu):
+    cm = gpu
+    rng = np.random.default_rng(2027)
+    f = rng.standard_normal(N).astype(np.float32) * 3
+    g = rng.uniform(0.1, 4, N).astype(np.float32)
+    g[::97] = 0.0  # zeros for division, inverse and comparisons
+    i = rng.integers(-1000, 1000, N).astype(np.int32)
+    j = rng.integers(-20, 20, N).astype(np.int32)  # includes zeros (division by zero gives 0)
+    b = rng.integers(0, 2, N).astype(np.bool_)
+    return {k: _Leaf(cm, v) for k, v in dict(f=f, g=g, i=i, j=j, b=b).items()}
+
+
+@pytest.mark.parametrize("fn", list(range(19)))
+def test_binary_f32(gpu, data, fn):
+    from cumat_amd import _capi as capi
+    if fn in (capi.BIN_MOD, capi.BIN_AND, capi.BIN_OR, capi.BIN_XOR, capi.BIN_LAND, capi.BIN_LOR):
+        pytest.skip("not an FP32 functor")
+    cmp = fn >= capi.BIN_EQ
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F32, fn, capi.SRC_LEAF, 1)]
+    _same(gpu, [data["f"], data["g"]], code, np.bool_ if cmp else np.float32)
+    # reversed operands against an immediate
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F32, fn, capi.SRC_IMM, 0, flags=1, fimm=1.75)]
+    _same(gpu, [data["g"]], code, np.bool_ if cmp else np.float32)
+
+
+@pytest.mark.parametrize("fn", list(range(19)))
+def test_binary_i32(gpu, data, fn):
+    from cumat_amd import _capi as capi
+    if fn == capi.BIN_POW:
+        pytest.skip("not an int32 functor")
+    code = [_instr(capi.OP_LOAD, capi.I32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.I32, fn, capi.SRC_LEAF, 1)]
+    _same(gpu, [data["i"], data["j"]], code, np.int32)
+
+
+@pytest.mark.parametrize("fn", list(range(27)))
+def test_unary(gpu, data, fn):
+    from cumat_amd import _capi as capi
+    if fn != capi.UN_BITNOT:
+        code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0), _instr(capi.OP_UNARY, capi.F32, fn)]
+        _same(gpu, [data["g"] if fn in (capi.UN_ACOS, capi.UN_ASIN) else data["f"]], code, np.float32)
+    code = [_instr(capi.OP_LOAD, capi.I32, src=capi.SRC_LEAF, arg=0), _instr(capi.OP_UNARY, capi.I32, fn)]
+    _same(gpu, [data["j"]], code, np.int32)
+
+
+def test_casts_select_fma_and_scalars(gpu, data):
+    """int -> float with a broadcast scalar, exp, an FMA term, a select on a
+    bool leaf, a cast to bool, a logical or; bool destination with |=."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    s = _Leaf(cm, np.array([0.37], np.float32), scalar=True)
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),       # float(i)
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_MUL, capi.SRC_LEAF, 3),  # * scalar
+            _instr(capi.OP_UNARY, capi.F32, capi.UN_TANH),
+            _instr(capi.OP_FMA, capi.F32, arg=2, fimm=0.5),                    # + f * 0.5
+            _instr(capi.OP_PUSH, capi.F32, arg=1),
+            _instr(capi.OP_LOAD, capi.BOOL, src=capi.SRC_LEAF, arg=1),
+            _instr(capi.OP_SELECT, capi.F32, src=capi.SRC_IMM, arg=1, fimm=2.5),
+            _instr(capi.OP_CAST, capi.I32, arg=capi.F32)]
+    leaves = [data["i"], data["b"], data["f"], s]
+    _same(cm, leaves, code, np.int32)
+    code_b = code[:-1] + [_instr(capi.OP_CAST, capi.BOOL, arg=capi.F32),
+                          _instr(capi.OP_BINARY, capi.BOOL, capi.BIN_LOR, capi.SRC_LEAF, 1)]
+    init = np.zeros(N, np.bool_)
+    init[::3] = True
+    _same(cm, leaves, code_b, np.bool_, mode=capi.OR, init=init)
+    _same(cm, leaves, code_b, np.bool_, mode=capi.AND, init=init)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+def test_compound_modes_f32(gpu, data, mode):
+    from cumat_amd import _capi as capi
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_MUL, capi.SRC_LEAF, 1),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_ADD, capi.SRC_LEAF, 2)]
+    init = np.random.default_rng(mode).uniform(0.5, 2, N).astype(np.float32)
+    _same(gpu, [data["f"], data["g"], data["f"]], code, np.float32, mode=mode, init=init)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6, 7])
+def test_compound_modes_i32(gpu, data, mode):
+    from cumat_amd import _capi as capi
+    code = [_instr(capi.OP_LOAD, capi.I32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.I32, capi.BIN_ADD, capi.SRC_IMM, 0, iimm=25)]
+    init = np.random.default_rng(mode).integers(-500, 500, N).astype(np.int32)
+    # j + 25 lies in 5..44: no zero divisor for /= and %=
+    _same(gpu, [data["j"]], code, np.int32, mode=mode, init=init)
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+def test_unaligned_leaves_and_destination(gpu, data, offset):
+    """Leaves and destination off the 16-byte grid take element loads and
+    stores in both paths; the stack keeps a partial result."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    a = _Leaf(cm, data["f"].arr, offset=offset)
+    b = _Leaf(cm, data["g"].arr, offset=0)
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_ADD, capi.SRC_LEAF, 1),
+            _instr(capi.OP_PUSH, capi.F32, arg=0),
+            _instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=1),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_SUB, capi.SRC_LEAF, 0),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_MUL, capi.SRC_STACK, 0, flags=1)]
+    _same(cm, [a, b], code, np.float32)
+    # unaligned destination: evaluate into a view one element in
+    from cumat_amd import _capi as capi2
+    from cumat_amd.device import np_to_dtype
+    n = N - 7
+    L = (capi2.Leaf * 2)()
+    for i, lf in enumerate([a, b]):
+        L[i].ptr, L[i].dtype, L[i].kind = lf.ptr, capi2.F32, capi2.LEAF_STRIDED
+        L[i].sr, L[i].sc, L[i].sb = 1, n, n
+    prog = (capi2.Instr * len(code))(*code)
+    outs = []
+    for jit in (0, 2):
+        dst = cm.DeviceArray.from_numpy(np.zeros(n + offset, np.float32))
+        old = cm.jit_mode(jit)
+        try:
+            capi2.call("cm_cwise_eval", None, L, 2, prog, len(code), dst.ptr + 4 * offset, np_to_dtype(np.float32),
+                       n, 1, 1, 1, n, n, 0)
+        finally:
+            cm.jit_mode(old)
+        outs.append(dst.numpy())
+    assert outs[0].tobytes() == outs[1].tobytes()
+
+
+@pytest.mark.parametrize("n", [1, 5, 257, 2048, 2049, 4099])
+def test_short_lengths(gpu, data, n):
+    from cumat_amd import _capi as capi
+    cm = gpu
+    a = _Leaf(cm, data["f"].arr[:n])
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0), _instr(capi.OP_UNARY, capi.F32, capi.UN_EXP)]
+    _same(cm, [a], code, np.float32, n=n)
+
+
+def test_auto_mode_threshold_and_cache(gpu, data):
+    """Mode 1 compiles a program once and reuses it; small evaluations stay
+    on the interpreter."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_MAX, capi.SRC_IMM, 0, fimm=0.125)]
+    init = np.zeros(N, np.float32)
+    cm.reset_counters()
+    r1 = _eval(cm, [data["f"]], code, np.float32, 0, init, jit=1)
+    r2 = _eval(cm, [data["f"]], code, np.float32, 0, init, jit=1)
+    c = cm.counters()
+    assert c["JitLaunch"] == 2 and c["JitCompile"] <= 1
+    assert r1.tobytes() == r2.tobytes()
+    small = np.zeros(1000, np.float32)
+    a = _Leaf(cm, data["f"].arr[:1000])
+    cm.reset_counters()
+    _eval(cm, [a], code, np.float32, 0, small, jit=1)
+    assert cm.counters()["JitLaunch"] == 0
+    assert np.array_equal(r1, np.maximum(data["f"].arr, np.float32(0.125)))
