@@ -305,12 +305,13 @@ int get(const std::string& key, const std::string& body, const char* name, Kerne
     return ok ? 0 : -1;
 }
 
-int launch(const Kernel& k, unsigned grid, unsigned block, hipStream_t stream, const void* args, size_t args_bytes)
+int launch(const Kernel& k, unsigned grid, unsigned block, hipStream_t stream, const void* args, size_t args_bytes,
+           unsigned grid_y)
 {
     size_t sz = args_bytes;
     void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, const_cast<void*>(args), HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz,
                    HIP_LAUNCH_PARAM_END};
-    if (hipModuleLaunchKernel(k.fn, grid, 1, 1, block, 1, 1, 0, stream, nullptr, cfg) != hipSuccess) return -1;
+    if (hipModuleLaunchKernel(k.fn, grid, grid_y, 1, block, 1, 1, 0, stream, nullptr, cfg) != hipSuccess) return -1;
     count(CM_CNT_JIT_LAUNCH);
     return 0;
 }

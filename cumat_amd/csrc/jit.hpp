@@ -39,8 +39,10 @@ int compile(const std::string& body, const std::string& arch, std::vector<char>&
 // Compiled kernel 'name' of 'body' (cached by key on the current device);
 // 0 on success.
 int get(const std::string& key, const std::string& body, const char* name, Kernel* out);
-// Launch with the kernel arguments packed in one buffer; 0 on success.
-int launch(const Kernel& k, unsigned grid, unsigned block, hipStream_t stream, const void* args, size_t args_bytes);
+// Launch (grid x grid_y blocks) with the kernel arguments packed in one
+// buffer; 0 on success.
+int launch(const Kernel& k, unsigned grid, unsigned block, hipStream_t stream, const void* args, size_t args_bytes,
+           unsigned grid_y = 1);
 
 }  // namespace jit
 }  // namespace cm

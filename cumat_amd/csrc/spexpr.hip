@@ -19,6 +19,7 @@
 // SparseMatrix::direct(), cuMat/src/SparseMatrix.h:230-233): element
 // k + b*sb; any other leaf is a dense strided leaf at (r, c, b).
 #include "cwise_core.hpp"
+#include "jit.hpp"
 #include "spmv_core.hpp"
 #include "spmv_wave.hpp"
 
@@ -290,6 +291,205 @@ template <typename T, bool W, int NL> struct EllExprLaunch {
     template <typename... A> static int run(A&&... a) { return launch_ell_expr<T, W, NL>(a...); }
 };
 
+
+// ------------------------------------------------- generated kernels
+// csr_expr_spmv_kernel<T, false, NL, false, true> (32-bit programs whose
+// leaves are value arrays of the product's pattern and broadcast scalars) as
+// straight-line code for one program: the wave-chunk walk of spmv_wave.hpp
+// with the next chunk's indices and value-leaf entries in flight during the
+// current chunk's gathers, the program evaluated per stored entry, products
+// summed per row in CSR order. Same semantics and summation order as the
+// interpreter kernel (long-row chunks included).
+constexpr int SX_JIT_DECLINED = 1;
+
+struct JitSxArgs {
+    const void* leaf[CM_CWISE_MAX_LEAVES];
+    long long sb[CM_CWISE_MAX_LEAVES];
+    const int* JA;
+    const int* IA;
+    const void* x;
+    void* y;
+    int rows, cols, x_batched, pad;
+};
+
+int jit_csr_expr_source(const CwProgram& P, int dtype, int mode, std::string& key, std::string& body)
+{
+    if (dtype != CM_F32 && dtype != CM_I32) return SX_JIT_DECLINED;
+    char buf[512];
+    snprintf(buf, sizeof(buf), "csrx32|t%d|m%d|", dtype, mode);
+    key = buf;
+    std::string members, loads, tail_loads, ptrs, scalars, args_v, args_s, params_v, params_s;
+    std::string nm_chunk[CM_CWISE_MAX_LEAVES], nm_tail[CM_CWISE_MAX_LEAVES];
+    const char* pc[CM_CWISE_MAX_LEAVES];
+    const char* pt[CM_CWISE_MAX_LEAVES];
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL) return SX_JIT_DECLINED;
+        const char* et = L.dtype == CM_BOOL ? "unsigned char" : "uint32_t";
+        const bool value = L.sr == -1;
+        if (!value && !(L.kind == CM_LEAF_STRIDED && L.sr == 0 && L.sc == 0)) return SX_JIT_DECLINED;
+        snprintf(buf, sizeof(buf), "L%d%c|", L.dtype, value ? 'v' : 's');
+        key += buf;
+        const std::string ls = std::to_string(l);
+        if (value) {
+            members += "    R w" + ls + "[EPT];\n";
+            ptrs += std::string("    const ") + et + "* __restrict__ p" + ls + " = (const " + et + "*)A.leaf[" + ls +
+                    "] + A.sb[" + ls + "] * b;\n";
+            params_v += std::string(", const ") + et + "* __restrict__ p" + ls;
+            args_v += ", p" + ls;
+            loads += "            C.w" + ls + "[j] = ok ? (R)__builtin_nontemporal_load(p" + ls + " + k) : 0u;\n";
+            tail_loads += "                const R t" + ls + " = (R)p" + ls + "[k];\n";
+            nm_chunk[l] = "C.w" + ls + "[j]";
+            nm_tail[l] = "t" + ls;
+        } else {
+            scalars += std::string("    const R c") + ls + " = (R)((const " + et + "*)A.leaf[" + ls + "])[A.sb[" + ls +
+                       "] * b];\n";
+            params_s += ", const R c" + ls;
+            args_s += ", c" + ls;
+            nm_chunk[l] = "c" + ls;
+            nm_tail[l] = "c" + ls;
+        }
+        pc[l] = nm_chunk[l].c_str();
+        pt[l] = nm_tail[l].c_str();
+    }
+    for (int i = 0; i < P.ncode; ++i) {
+        const cm_instr& I = P.code[i];
+        uint64_t fb;
+        memcpy(&fb, &I.fimm, 8);
+        snprintf(buf, sizeof(buf), "%d,%d,%d,%d,%d,%d,%llx,%llx;", I.opcode, I.dtype, I.fn, I.src, I.arg, I.flags,
+                 (unsigned long long)fb, (unsigned long long)I.iimm);
+        key += buf;
+    }
+    std::string prog_chunk, prog_tail;
+    if (!jit::program_body(P, pc, prog_chunk) || !jit::program_body(P, pt, prog_tail)) return SX_JIT_DECLINED;
+    const char* get = dtype == CM_F32 ? "getf" : "geti";
+    const char* tt = dtype == CM_F32 ? "float" : "int";
+    snprintf(buf, sizeof(buf), "typedef %s TT;\nconstexpr int MODE = %d;\nconstexpr int EPT = 8;\n", tt, mode);
+    body = std::string(buf) +
+           "typedef uint32_t R;\n"
+           "struct JitSxArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) + "]; long long sb[" +
+           std::to_string(CM_CWISE_MAX_LEAVES) +
+           "]; const int* JA; const int* IA; const void* x; void* y; int rows, cols, x_batched, pad; };\n"
+           "struct Ch {\n    int r0, nr, s, e, rb, re;\n    int col[EPT];\n" + members + "};\n"
+           "__device__ __forceinline__ void ld(Ch& C, const int* __restrict__ JA, const int* __restrict__ IA, int rows, int r0" +
+           params_v +
+           ")\n{\n"
+           "    const int lane = threadIdx.x & 63;\n"
+           "    C.r0 = r0;\n    C.nr = min(64, rows - r0);\n"
+           "    C.s = JA[r0];\n    C.e = JA[r0 + C.nr];\n"
+           "    C.rb = lane < C.nr ? JA[r0 + lane] : 0;\n    C.re = lane < C.nr ? JA[r0 + lane + 1] : 0;\n"
+           "    if (C.e - C.s <= EPT * 64) {\n"
+           "#pragma unroll\n"
+           "        for (int j = 0; j < EPT; ++j) {\n"
+           "            const int k = C.s + lane + 64 * j;\n"
+           "            const bool ok = k < C.e;\n"
+           "            C.col[j] = ok ? __builtin_nontemporal_load(IA + k) : 0;\n" +
+           loads +
+           "        }\n    }\n}\n"
+           "__device__ __forceinline__ TT comp(const Ch& C, const int* __restrict__ IA, const TT* __restrict__ xb, TT* slice" +
+           params_v + params_s +
+           ")\n{\n"
+           "    const int lane = threadIdx.x & 63;\n"
+           "    TT acc = TT(0);\n"
+           "    if (C.e - C.s <= EPT * 64) {\n"
+           "        TT xv[EPT];\n"
+           "#pragma unroll\n"
+           "        for (int j = 0; j < EPT; ++j) xv[j] = (C.s + lane + 64 * j < C.e) ? xb[C.col[j]] : TT(0);\n"
+           "#pragma unroll\n"
+           "        for (int j = 0; j < EPT; ++j) {\n"
+           "            R acc = 0, t = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;\n"
+           "            (void)t; (void)s0; (void)s1; (void)s2; (void)s3;\n" +
+           prog_chunk +
+           "            const int k = C.s + lane + 64 * j;\n"
+           "            if (k < C.e) slice[k - C.s] = " + get + "(acc) * xv[j];\n"
+           "        }\n"
+           "        __builtin_amdgcn_wave_barrier();\n"
+           "        const int a = C.rb - C.s, bnd = C.re - C.s;\n"
+           "        if (lane < C.nr && a < bnd) {\n"
+           "            acc = slice[a];\n"
+           "            for (int k = a + 1; k < bnd; ++k) acc += slice[k];\n"
+           "        }\n"
+           "        __builtin_amdgcn_wave_barrier();\n"
+           "    } else if (lane < C.nr) {\n"
+           "        // long rows: each lane walks its own row in CSR order\n"
+           "        for (int k = C.rb; k < C.re; ++k) {\n"
+           "            const int c = IA[k];\n"
+           "            TT p;\n"
+           "            {\n" +
+           tail_loads +
+           "                R acc = 0, t = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;\n"
+           "                (void)t; (void)s0; (void)s1; (void)s2; (void)s3;\n" +
+           prog_tail +
+           "                p = " + get + "(acc) * xb[c];\n"
+           "            }\n"
+           "            acc = (k == C.rb) ? p : acc + p;\n"
+           "        }\n"
+           "    }\n"
+           "    return acc;\n"
+           "}\n"
+           "extern \"C\" __global__ __launch_bounds__(256) void cm_jit_csr_expr(const JitSxArgs A)\n{\n"
+           "    __shared__ TT lds[4][EPT * 64];\n"
+           "    TT* slice = lds[threadIdx.x >> 6];\n"
+           "    const int lane = threadIdx.x & 63;\n"
+           "    const long long b = blockIdx.y;\n"
+           "    const int rows = A.rows;\n"
+           "    const TT* __restrict__ xb = (const TT*)A.x + (A.x_batched ? b * A.cols : 0);\n"
+           "    TT* __restrict__ yb = (TT*)A.y + b * rows;\n" +
+           ptrs + scalars +
+           // chunk walk of spmv_wave.hpp: XCD x owns chunks [x*NC/8, (x+1)*NC/8)
+           "    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n"
+           "    const int NC = (rows + 63) >> 6;\n"
+           "    const int xcd = blockIdx.x & 7;\n"
+           "    const int hi = (int)(((long long)NC * (xcd + 1)) >> 3);\n"
+           "    const int step = (gridDim.x >> 3) * 4;\n"
+           "    int cur = (int)(((long long)NC * xcd) >> 3) + (blockIdx.x >> 3) * 4 + wave;\n"
+           "    if (cur >= hi) return;\n"
+           "    Ch cu, nx;\n"
+           "    ld(cu, A.JA, A.IA, rows, cur * 64" + args_v + ");\n"
+           "    for (;;) {\n"
+           "        const int nc = cur + step;\n"
+           "        const bool more = nc < hi;\n"
+           "        if (more) ld(nx, A.JA, A.IA, rows, nc * 64" + args_v + ");\n"
+           "        const TT acc = comp(cu, A.IA, xb, slice" + args_v + args_s + ");\n"
+           "        if (lane < cu.nr) {\n"
+           "            TT* yy = yb + cu.r0 + lane;\n"
+           "            if (MODE == CM_ASSIGN) __builtin_nontemporal_store(acc, yy);\n"
+           "            else *yy = assign1<TT>(MODE, *yy, acc);\n"
+           "        }\n"
+           "        if (!more) break;\n"
+           "        cu = nx;\n"
+           "        cur = nc;\n"
+           "    }\n"
+           "}\n";
+    return CM_OK;
+}
+
+static int jit_csr_expr(cm_ctx_t ctx, const CwProgram& P, int dtype, int rows, int cols, int batches,
+                        const int* JA, const int* IA, const void* x, int xb, void* y, int mode)
+{
+    std::string key, body;
+    if (jit_csr_expr_source(P, dtype, mode, key, body) != CM_OK) return SX_JIT_DECLINED;
+    jit::Kernel k;
+    if (jit::get(key, body, "cm_jit_csr_expr", &k) != 0) return SX_JIT_DECLINED;
+    JitSxArgs A;
+    memset(&A, 0, sizeof(A));
+    for (int l = 0; l < P.nleaves; ++l) {
+        A.leaf[l] = P.leaves[l].ptr;
+        A.sb[l] = P.leaves[l].sb;
+    }
+    A.JA = JA;
+    A.IA = IA;
+    A.x = x;
+    A.y = y;
+    A.rows = rows;
+    A.cols = cols;
+    A.x_batched = xb;
+    const unsigned grid = (unsigned)walk_grid(div_up(div_up(rows, 64), SX_BS / 64), 2048);
+    if (jit::launch(k, grid, SX_BS, ctx->stream, &A, sizeof(A), (unsigned)batches) != 0)
+        return fail(CM_E_HIP, "cm_csr_expr_spmv: launch of the generated kernel failed");
+    count(CM_CNT_KERNEL_LAUNCH);
+    return CM_OK;
+}
 }  // namespace cm
 
 extern "C" {
@@ -323,6 +523,10 @@ int cm_csr_expr_spmv(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm_
         if (L.kind != CM_LEAF_STRIDED || L.sr != 0 || L.sc != 0) values_only = false;
     }
     const int shape = values_only ? 2 : (need_row ? 1 : 0);
+    if (shape == 2 && !wide && cm::jit::want((long long)nnz * batches)) {
+        const int s = cm::jit_csr_expr(ctx, P, dtype, rows, cols, batches, JA, IA, x, x_batched, y, mode);
+        if (s != cm::SX_JIT_DECLINED) return s;
+    }
     return cm::sx_dispatch<cm::CsrExprLaunch>(dtype, wide, nleaves, ctx, P, shape, rows, cols, batches, JA, IA, x,
                                               x_batched, y, mode);
     CM_API_END

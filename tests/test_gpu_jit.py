@@ -223,3 +223,69 @@ def test_auto_mode_threshold_and_cache(gpu, data):
     _eval(cm, [a], code, np.float32, 0, small, jit=1)
     assert cm.counters()["JitLaunch"] == 0
     assert np.array_equal(r1, np.maximum(data["f"].arr, np.float32(0.125)))
+
+
+def _long_row_pattern(g, nlong, width, rng):
+    """2D 5-point pattern plus 'nlong' rows with 'width' extra entries (their
+    chunks exceed the lane budget and take the per-lane row walk)."""
+    from cumat_amd.problems import poisson2d
+    JA, IA, _ = poisson2d(g, 0.0, np.float64)
+    n = g * g
+    long_rows = set(rng.choice(n, nlong, replace=False).tolist())
+    rows = []
+    for r in range(n):
+        cols = list(IA[JA[r]:JA[r + 1]])
+        if r in long_rows:
+            cols = sorted(set(cols) | set(int(c) for c in rng.choice(n, width, replace=False)))
+        rows.append(cols)
+    JA2 = np.zeros(n + 1, np.int32)
+    JA2[1:] = np.cumsum([len(c) for c in rows])
+    return JA2, np.concatenate([np.array(c, np.int32) for c in rows]), n
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int32])
+@pytest.mark.parametrize("nlong", [0, 30])
+def test_csr_expression_product_generated(gpu, dtype, nlong):
+    """(a*A + B - s).sparseView(pattern) * x with batched same-pattern value
+    arrays, a broadcast scalar and a bool value array in a select; modes =,
+    += and -=: the generated product equals the interpreter's bit for bit
+    (random FP32 data, so rounding order matters)."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    rng = np.random.default_rng(31 + nlong)
+    JA, IA, n = _long_row_pattern(180, nlong, 900, rng)
+    nnz, B = IA.size, 2
+    if dtype == np.float32:
+        va = rng.standard_normal(nnz * B).astype(dtype)
+        vb = rng.standard_normal(nnz * B).astype(dtype)
+        x = rng.standard_normal(n * B).astype(dtype)
+        three, one = 3.0, 1.0
+    else:
+        va = rng.integers(-9, 10, nnz * B).astype(dtype)
+        vb = rng.integers(-9, 10, nnz * B).astype(dtype)
+        x = rng.integers(-9, 10, n * B).astype(dtype)
+        three, one = 3, 1
+    pat = cm.SparsityPattern(n, n, JA, IA)
+    A = cm.SparseMatrix(pat, va, batches=B)
+    Bm = cm.SparseMatrix(pat, vb, batches=B)
+    mask = cm.SparseMatrix(pat, rng.integers(0, 2, nnz).astype(np.bool_))
+    xd = cm.DeviceArray.from_numpy(x, rows=n, cols=1, batches=B)
+    e = cm.select(mask.direct(), A.direct() * three + Bm.direct() - one, A.direct())
+    y0 = (rng.standard_normal(n * B) * 10).astype(dtype)
+    outs = {}
+    for jit in (0, 2):
+        old = cm.jit_mode(jit)
+        try:
+            cm.reset_counters()
+            y = cm.sparse_view(e, pat).matvec(xd)
+            assert cm.counters()["JitLaunch"] == (1 if jit else 0)
+            res = [y.numpy()]
+            for mode in (capi.ADD, capi.SUB):
+                yd = cm.DeviceArray.from_numpy(y0.copy(), rows=n, cols=1, batches=B)
+                cm.sparse_view(e, pat).matvec(xd, out=yd, mode=mode)
+                res.append(yd.numpy())
+        finally:
+            cm.jit_mode(old)
+        outs[jit] = res
+    for a, b in zip(outs[0], outs[2]):
+        assert a.tobytes() == b.tobytes()
