@@ -280,6 +280,8 @@ constexpr int JIT_DECLINED = 1;
 
 struct JitCwiseArgs {
     const void* leaf[CM_CWISE_MAX_LEAVES];
+    long long gsc[CM_CWISE_MAX_LEAVES];  // gathered leaves: element stride of the index
+    const int* idx;                       // gathered leaves: leaf[idx[e] * gsc]
     void* dst;
     long long n;
 };
@@ -290,29 +292,59 @@ static const char* jit_raw_load(int dtype)
     return dtype == CM_BOOL ? "(R)((const unsigned char*)A.leaf[%d])[%s]" : "((const uint32_t*)A.leaf[%d])[%s]";
 }
 
-// Source and cache key of the generated kernel cm_jit_cwise for P.
-int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& body)
+// Source and cache key of the generated kernel cm_jit_cwise for P. Leaves in
+// 'gmask' are gathered: element e reads leaf[idx[e] * gsc] (a leaf that
+// depends on the column of a stored CSR entry, idx = the column indices).
+int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& body, unsigned gmask = 0,
+                          const int* idx = nullptr)
 {
     const int dst_dt = P.dst_dtype;
     if (dst_dt != CM_F32 && dst_dt != CM_I32 && dst_dt != CM_BOOL) return JIT_DECLINED;
     const bool dst_al = ((uintptr_t)P.dst & 15) == 0;
+    const bool idx_al = ((uintptr_t)idx & 15) == 0;
     key.clear();
     body.clear();
     char buf[256];
     snprintf(buf, sizeof(buf), "cwflat32|d%d%c|m%d|", dst_dt, dst_al ? 'a' : 'u', P.mode);
     key += buf;
+    if (gmask) {
+        snprintf(buf, sizeof(buf), "i%c|", idx_al ? 'a' : 'u');
+        key += buf;
+    }
     // leaf value expressions and the per-leaf load code
-    std::string decls, loads_full, loads_tail, scalars;
+    std::string decls, loads_full, loads_tail, scalars, gathers;
+    if (gmask) {
+        // the index of a thread's 8 elements, loaded like a direct leaf
+        decls += "        int ix[8];\n";
+        if (idx_al)
+            loads_full += "            { const u4v w = __builtin_nontemporal_load((const u4v*)(A.idx + e0));\n"
+                          "              ix[0] = w.x; ix[1] = w.y; ix[2] = w.z; ix[3] = w.w; }\n"
+                          "            { const u4v w = __builtin_nontemporal_load((const u4v*)(A.idx + e0 + 256));\n"
+                          "              ix[4] = w.x; ix[5] = w.y; ix[6] = w.z; ix[7] = w.w; }\n";
+        else
+            loads_full += "#pragma unroll\n            for (int k = 0; k < 8; ++k) ix[k] = A.idx[EL(e0, k)];\n";
+        loads_tail += "#pragma unroll\n            for (int k = 0; k < 8; ++k) ix[k] = A.idx[min(EL(e0, k), n - 1)];\n";
+    }
     std::string names[CM_CWISE_MAX_LEAVES];
     const char* namep[CM_CWISE_MAX_LEAVES];
     for (int l = 0; l < P.nleaves; ++l) {
         const cm_leaf& L = P.leaves[l];
         if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL) return JIT_DECLINED;
         const bool scalar = (P.leaf_scalar >> l) & 1;
+        const bool gathered = (gmask >> l) & 1;
         const bool al = L.dtype != CM_BOOL && ((uintptr_t)L.ptr & 15) == 0;
-        snprintf(buf, sizeof(buf), "L%d%c%c|", L.dtype, scalar ? 's' : 'd', al ? 'a' : 'u');
+        snprintf(buf, sizeof(buf), "L%d%c%c|", L.dtype, gathered ? 'g' : (scalar ? 's' : 'd'), al ? 'a' : 'u');
         key += buf;
-        if (scalar) {
+        if (gathered) {
+            snprintf(buf, sizeof(buf), "v%d[k]", l);
+            names[l] = buf;
+            decls += "        R v" + std::to_string(l) + "[8];\n";
+            char ld[160], at[64];
+            snprintf(at, sizeof(at), "(long long)ix[k] * A.gsc[%d]", l);
+            snprintf(ld, sizeof(ld), jit_raw_load(L.dtype), l, at);
+            gathers += std::string("#pragma unroll\n        for (int k = 0; k < 8; ++k) v") + std::to_string(l) +
+                       "[k] = " + ld + ";\n";
+        } else if (scalar) {
             snprintf(buf, sizeof(buf), "c%d", l);
             names[l] = buf;
             char ld[160];
@@ -393,8 +425,8 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
     }
     snprintf(buf, sizeof(buf), "constexpr int MODE = %d;\n", P.mode);
     body = std::string("typedef unsigned int u4v __attribute__((ext_vector_type(4)));\n") + buf +
-           "struct JitCwiseArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) +
-           "]; void* dst; long long n; };\n"
+           "struct JitCwiseArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) + "]; long long gsc[" +
+           std::to_string(CM_CWISE_MAX_LEAVES) + "]; const int* idx; void* dst; long long n; };\n"
            "#define EL(e0, k) ((e0) + ((k) >> 2) * 256 + ((k) & 3))\n"
            "extern \"C\" __global__ __launch_bounds__(256) void cm_jit_cwise(const JitCwiseArgs A)\n{\n"
            "    typedef uint32_t R;\n"
@@ -411,6 +443,7 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
            "        } else {\n" +
            loads_tail +
            "        }\n" +
+           gathers +
            "        R res[8];\n"
            "#pragma unroll\n"
            "        for (int k = 0; k < 8; ++k) {\n"
@@ -430,15 +463,19 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
     return CM_OK;
 }
 
-static int jit_cwise_flat(cm_ctx_t ctx, const CwProgram& P)
+static int jit_cwise_flat(cm_ctx_t ctx, const CwProgram& P, unsigned gmask = 0, const int* idx = nullptr)
 {
     std::string key, body;
-    if (jit_cwise_flat_source(P, key, body) != CM_OK) return JIT_DECLINED;
+    if (jit_cwise_flat_source(P, key, body, gmask, idx) != CM_OK) return JIT_DECLINED;
     jit::Kernel k;
     if (jit::get(key, body, "cm_jit_cwise", &k) != 0) return JIT_DECLINED;
     JitCwiseArgs A;
     memset(&A, 0, sizeof(A));
-    for (int l = 0; l < P.nleaves; ++l) A.leaf[l] = P.leaves[l].ptr;
+    for (int l = 0; l < P.nleaves; ++l) {
+        A.leaf[l] = P.leaves[l].ptr;
+        A.gsc[l] = P.leaves[l].sc;
+    }
+    A.idx = idx;
     A.dst = P.dst;
     A.n = P.n;
     // the interpreter's grid: one block per 2048 elements, at most 64 per CU
@@ -672,67 +709,95 @@ int cm_csr_cwise_eval(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const cm
     cm::count(CM_CNT_EVAL_CWISE_SPARSE);
     cm::count(CM_CNT_EVAL_ANY);
     if (rows == 0 || nnz == 0) return CM_OK;
-    // a program whose leaves are the value arrays, scalars and leaves that
-    // depend on the column only (row stride 0; one batch or batch stride 0)
-    // needs no row coordinate: each column leaf is first gathered along IA
-    // into an nnz-long temporary, then the program is a dense program over
-    // the nnz x 1 x batches value array (value leaves become leaves with
-    // row stride 1 and their own batch stride, the same offsets as below),
-    // run by the streaming dense kernels
-    if (nleaves >= 0 && nleaves <= CM_CWISE_MAX_LEAVES && leaves != nullptr && val_dtype != CM_BF16 &&
-        cm::dtype_size(val_dtype) > 0) {
-        bool ok = true;
-        size_t gbytes = 0;
-        for (int l = 0; ok && l < nleaves; ++l) {
-            const cm_leaf& L = leaves[l];
-            if (L.sr == -1) continue;  // a value array
-            const int es = cm::dtype_size(L.dtype);
-            ok = L.kind == CM_LEAF_STRIDED && L.sr == 0 && L.ptr != nullptr && (es == 1 || es == 4 || es == 8);
-            if (ok && L.sc != 0) {
-                ok = L.sb == 0 || batches == 1;
-                gbytes += ((size_t)es * nnz + 255) / 256 * 256;
-            }
-        }
-        if (ok) {
-            void* tmp = nullptr;
-            if (gbytes) CM_TRY(cm::scratch(ctx, 0, gbytes, &tmp));
-            cm_leaf dl[CM_CWISE_MAX_LEAVES];
-            size_t off = 0;
-            const long long cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
-            const unsigned gblocks = (unsigned)std::max<long long>(1, std::min<long long>(cm::div_up(nnz, 256), cus * 16));
-            for (int l = 0; l < nleaves; ++l) {
-                dl[l] = leaves[l];
-                if (dl[l].sr == -1) {
-                    dl[l].sr = 1;
-                    dl[l].sc = 0;
-                } else if (dl[l].sc != 0) {
-                    const int es = cm::dtype_size(dl[l].dtype);
-                    void* g = (char*)tmp + off;
-                    if (es == 4)
-                        cm::csr_gather_kernel<uint32_t><<<gblocks, 256, 0, ctx->stream>>>(
-                            (const uint32_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint32_t*)g);
-                    else if (es == 8)
-                        cm::csr_gather_kernel<uint64_t><<<gblocks, 256, 0, ctx->stream>>>(
-                            (const uint64_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint64_t*)g);
-                    else
-                        cm::csr_gather_kernel<uint8_t><<<gblocks, 256, 0, ctx->stream>>>(
-                            (const uint8_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint8_t*)g);
-                    CM_LAUNCH_CHECK();
-                    dl[l].ptr = g;
-                    dl[l].sr = 1;
-                    dl[l].sc = 0;
-                    dl[l].sb = 0;
-                    off += ((size_t)es * nnz + 255) / 256 * 256;
-                }
-            }
-            return cm::cwise_dense(ctx, dl, nleaves, code, ncode, vals, val_dtype, nnz, 1, batches, 1, nnz, nnz,
-                                   mode);
-        }
-    }
+    CM_REQUIRE(JA && IA && vals, CM_E_ARG, "cm_csr_cwise_eval: NULL pointer");
+    CM_REQUIRE(val_dtype != CM_BF16 && cm::dtype_size(val_dtype) > 0, CM_E_UNSUPPORTED,
+               "cm_csr_cwise_eval: value dtype %d", val_dtype);
+    // the program is checked before anything is launched
+    CM_REQUIRE((leaves || nleaves == 0) && code, CM_E_ARG, "cm_csr_cwise_eval: NULL leaves or program");
     cm::CwProgram P;
     bool wide = false;
     CM_TRY(cm::build_program(leaves, nleaves, code, ncode, P, wide));
     if (val_dtype == CM_F64) wide = true;
+    // a program whose leaves are the value arrays, scalars and leaves that
+    // depend on the column only (row stride 0; one batch or batch stride 0)
+    // needs no row coordinate
+    bool row_free = true;
+    unsigned value_mask = 0, gather_mask = 0, scalar_mask = 0;
+    size_t gbytes = 0;
+    for (int l = 0; row_free && l < nleaves; ++l) {
+        const cm_leaf& L = leaves[l];
+        if (L.sr == -1) {  // a value array
+            value_mask |= 1u << l;
+            continue;
+        }
+        const int es = cm::dtype_size(L.dtype);
+        row_free = L.kind == CM_LEAF_STRIDED && L.sr == 0 && (es == 1 || es == 4 || es == 8);
+        if (row_free && L.sc != 0) {
+            row_free = L.sb == 0 || batches == 1;
+            gather_mask |= 1u << l;
+            gbytes += ((size_t)es * nnz + 255) / 256 * 256;
+        } else if (row_free) {
+            scalar_mask |= 1u << l;
+        }
+    }
+    if (row_free && batches == 1 && !wide && cm::jit::want(nnz)) {
+        // one generated kernel over the stored entries: value arrays read
+        // directly, column leaves gathered through IA, scalars broadcast
+        cm::CwProgram G = P;
+        for (int l = 0; l < nleaves; ++l)
+            if ((value_mask >> l) & 1) G.leaves[l].sr = 1, G.leaves[l].sc = 0, G.leaves[l].sb = 0;
+        G.leaf_direct = value_mask;
+        G.leaf_scalar = scalar_mask;
+        G.dst = vals;
+        G.dst_dtype = val_dtype;
+        G.mode = mode;
+        G.rows = nnz;
+        G.cols = 1;
+        G.batches = 1;
+        G.n = nnz;
+        G.dst_direct = 1;
+        const int s = cm::jit_cwise_flat(ctx, G, gather_mask, IA);
+        if (s != cm::JIT_DECLINED) return s;
+    }
+    if (row_free) {
+        // each column leaf is first gathered along IA into an nnz-long
+        // temporary, then the program is a dense program over the nnz x 1 x
+        // batches value array (value leaves become leaves with row stride 1
+        // and their own batch stride, the same offsets as below), run by the
+        // streaming dense kernels
+        void* tmp = nullptr;
+        if (gbytes) CM_TRY(cm::scratch(ctx, 0, gbytes, &tmp));
+        cm_leaf dl[CM_CWISE_MAX_LEAVES];
+        size_t off = 0;
+        const long long cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
+        const unsigned gblocks = (unsigned)std::max<long long>(1, std::min<long long>(cm::div_up(nnz, 256), cus * 16));
+        for (int l = 0; l < nleaves; ++l) {
+            dl[l] = leaves[l];
+            if ((value_mask >> l) & 1) {
+                dl[l].sr = 1;
+                dl[l].sc = 0;
+            } else if ((gather_mask >> l) & 1) {
+                const int es = cm::dtype_size(dl[l].dtype);
+                void* g = (char*)tmp + off;
+                if (es == 4)
+                    cm::csr_gather_kernel<uint32_t><<<gblocks, 256, 0, ctx->stream>>>(
+                        (const uint32_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint32_t*)g);
+                else if (es == 8)
+                    cm::csr_gather_kernel<uint64_t><<<gblocks, 256, 0, ctx->stream>>>(
+                        (const uint64_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint64_t*)g);
+                else
+                    cm::csr_gather_kernel<uint8_t><<<gblocks, 256, 0, ctx->stream>>>(
+                        (const uint8_t*)dl[l].ptr, dl[l].sc, IA, nnz, (uint8_t*)g);
+                CM_LAUNCH_CHECK();
+                dl[l].ptr = g;
+                dl[l].sr = 1;
+                dl[l].sc = 0;
+                dl[l].sb = 0;
+                off += ((size_t)es * nnz + 255) / 256 * 256;
+            }
+        }
+        return cm::cwise_dense(ctx, dl, nleaves, code, ncode, vals, val_dtype, nnz, 1, batches, 1, nnz, nnz, mode);
+    }
     P.dst = vals;
     P.dst_dtype = val_dtype;
     P.mode = mode;

@@ -289,3 +289,71 @@ def test_csr_expression_product_generated(gpu, dtype, nlong):
         outs[jit] = res
     for a, b in zip(outs[0], outs[2]):
         assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.int32])
+def test_csr_cwise_column_leaves_generated(gpu, dtype):
+    """A = A * r(col) + s(col) (row vectors indexed by each stored entry's
+    column) and a compound mode, by the generated kernel (one launch, the
+    column leaves gathered through IA) and by the interpreter route (gather
+    pass + dense program): the same bits."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    rng = np.random.default_rng(77)
+    JA, IA, n = _long_row_pattern(400, 0, 0, rng)
+    nnz = IA.size
+    if dtype == np.float32:
+        v0 = rng.standard_normal(nnz).astype(dtype)
+        r = rng.standard_normal(n).astype(dtype)
+    else:
+        v0 = rng.integers(-50, 50, nnz).astype(dtype)
+        r = rng.integers(-7, 8, n).astype(dtype)
+    flag = rng.integers(0, 2, n).astype(np.bool_)
+    pat = cm.SparsityPattern(n, n, JA, IA)
+    rd = cm.DeviceArray.from_numpy(r, rows=1, cols=n, batches=1, row_major=True)
+    fd = cm.DeviceArray.from_numpy(flag, rows=1, cols=n, batches=1, row_major=True)
+    outs = {}
+    for jit in (0, 2):
+        S = cm.SparseMatrix(pat, v0.copy())
+        old = cm.jit_mode(jit)
+        try:
+            cm.reset_counters()
+            S.assign(cm.select(fd.expr(), S.direct() * rd.expr(), S.direct()))
+            c = cm.counters()
+            assert c["JitLaunch"] == (1 if jit else 0)
+            if jit:
+                assert c["KernelLaunch"] == 1  # no gather pass
+            S.assign(S.direct() + rd.expr(), mode=capi.ADD)
+            outs[jit] = S.values.numpy()
+        finally:
+            cm.jit_mode(old)
+    assert outs[0].tobytes() == outs[2].tobytes()
+    rows_c = IA
+    exp = np.where(flag[rows_c], v0 * r[rows_c], v0).astype(dtype)
+    exp = (exp + (exp + r[rows_c])).astype(dtype)
+    if dtype == np.int32:
+        assert np.array_equal(outs[2], exp)
+
+
+def test_csr_cwise_rejects_null_pointers_and_bad_programs_before_launch(gpu):
+    from cumat_amd import _capi as capi
+    cm = gpu
+    from cumat_amd.problems import poisson2d
+    JA, IA, V = poisson2d(16)
+    pat = cm.SparsityPattern(256, 256, JA, IA)
+    S = cm.SparseMatrix(pat, V)
+    r = cm.DeviceArray.from_numpy(np.ones(256, np.float32), rows=1, cols=256, batches=1, row_major=True)
+    L = (capi.Leaf * 2)()
+    L[0].ptr, L[0].dtype, L[0].kind, L[0].sr, L[0].sc, L[0].sb = S.values.ptr, capi.F32, 0, -1, 0, 0
+    L[1].ptr, L[1].dtype, L[1].kind, L[1].sr, L[1].sc, L[1].sb = r.ptr, capi.F32, 0, 0, 1, 0
+    good = (capi.Instr * 2)(_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),
+                            _instr(capi.OP_BINARY, capi.F32, capi.BIN_MUL, capi.SRC_LEAF, 1))
+    bad = (capi.Instr * 2)(_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0), _instr(77, capi.F32))
+    Lb = capi.lib()
+    cm.reset_counters()
+    rc = Lb.cm_csr_cwise_eval(None, L, 2, good, 2, 256, 256, IA.size, 1, pat.JA.ptr, None, S.values.ptr, capi.F32, 0)
+    assert rc == capi.CM_E_ARG
+    rc = Lb.cm_csr_cwise_eval(None, L, 2, bad, 2, 256, 256, IA.size, 1, pat.JA.ptr, pat.IA.ptr, S.values.ptr,
+                              capi.F32, 0)
+    assert rc == capi.CM_E_ARG
+    assert cm.counters()["KernelLaunch"] == 0
