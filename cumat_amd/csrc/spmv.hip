@@ -50,6 +50,37 @@ __global__ __launch_bounds__(SPMV_BS) void csr_spmv_kernel(int rows, int cols, i
     }
 }
 
+// Rows of at most MAXW slots: every slot's index and value load is issued
+// at once, then the gathers, then the adds in slot order (the same sums as
+// ell_spmv_kernel; 2048^2 width 5: 36.3 -> 34.6 us).
+template <typename T, int MAXW>
+__global__ void ell_spmv_w_kernel(int rows, int cols, int npr, const int* __restrict__ idx,
+                                  const T* __restrict__ vals, int vals_batched, const T* __restrict__ x,
+                                  int x_batched, T* __restrict__ y, int mode)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int b = blockIdx.y;
+    const T* v = vals + (vals_batched ? (long long)b * rows * npr : 0);
+    const T* xb = x + (x_batched ? (long long)b * cols : 0);
+    int c[MAXW];
+    T a[MAXW], g[MAXW];
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j) {
+        const bool ok = j < npr;  // uniform
+        c[j] = ok ? __builtin_nontemporal_load(idx + r + (long long)rows * j) : -1;
+        a[j] = ok ? __builtin_nontemporal_load(v + r + (long long)rows * j) : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j) g[j] = c[j] >= 0 ? xb[c[j]] : T(0);
+    T acc = T(0);
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j)
+        if (c[j] >= 0) acc += a[j] * g[j];
+    T* yy = y + r + (long long)rows * b;
+    *yy = (mode == CM_ASSIGN) ? acc : assign_mode<T>(mode, *yy, acc);
+}
+
 template <typename T>
 __global__ void ell_spmv_kernel(int rows, int cols, int npr, const int* __restrict__ idx,
                                 const T* __restrict__ vals, int vals_batched,
@@ -168,7 +199,11 @@ int cm_ell_spmv(cm_ctx_t ctx, int dtype, int rows, int cols, int nnz_per_row, in
     cm::count(CM_CNT_EVAL_ANY);
     if (rows == 0) return CM_OK;
     dim3 grid(cm::div_up(rows, 256), batches);
-    if (dtype == CM_F32) {
+    if (dtype == CM_F32 && nnz_per_row <= 8) {
+        cm::ell_spmv_w_kernel<float, 8><<<grid, 256, 0, ctx->stream>>>(rows, cols, nnz_per_row, indices,
+                                                                        (const float*)vals, vals_batched,
+                                                                        (const float*)x, x_batched, (float*)y, mode);
+    } else if (dtype == CM_F32) {
         cm::ell_spmv_kernel<float><<<grid, 256, 0, ctx->stream>>>(rows, cols, nnz_per_row, indices,
                                                                    (const float*)vals, vals_batched,
                                                                    (const float*)x, x_batched, (float*)y, mode);
