@@ -254,6 +254,25 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
     out["dot_2^26"] = {"us": round(ms * 1000, 1), "GBps": round(8 * N / (ms * 1e-3) / 1e9, 1),
                        "frac": round(8 * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     log("dot", out["dot_2^26"])
+    # ---- reductions of unevaluated expressions (one launch, no temporary),
+    # 2^26 elements: squared norm of a difference and the sum of a*b+c
+    red = {}
+    for label, fn, bpe in (("(a-b).squaredNorm()", lambda: cm.squared_norm(vecs[0].expr() - vecs[1].expr(), out=d_out,
+                                                                           ctx=ctx), 8),
+                           ("dot(a-b, c)", lambda: cm.dot(vecs[0].expr() - vecs[1].expr(), vecs[2], out=d_out, ctx=ctx),
+                            12),
+                           ("(a*b+c).sum()", lambda: cm.reduce(vecs[0].expr() * vecs[1].expr() + vecs[2].expr(),
+                                                              "sum", capi.AXIS_ALL, out=d_out, ctx=ctx), 12)):
+        fn()
+        ctx.sync()
+        timer.start(stream)
+        for _ in range(10):
+            fn()
+        ms = timer.stop_ms(stream) / 10
+        red[label] = {"us": round(ms * 1000, 1), "GBps": round(bpe * N / (ms * 1e-3) / 1e9, 1),
+                      "frac": round(bpe * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out["reduce_expr_2^26"] = red
+    log("reduce_expr_2^26", red)
     # ---- general programs (not linear combinations), 2^26 elements: the
     # kernel generated and compiled for each program at run time (the
     # default from 2^18 elements on; the compilation happens in the untimed
@@ -483,7 +502,8 @@ def bench_batched(cm, capi, ctx):
     x = cm.DeviceArray(P["b"].size, dtype=np.float32, ctx=ctx)
     its = cm.DeviceArray(nsys, dtype=np.int32, ctx=ctx)
     errs = cm.DeviceArray(nsys, dtype=np.float32, ctx=ctx)
-    bs, tune_ms = pcg_batched_autotune(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], tol, maxit, ctx=ctx)
+    bs, gpc, tune_ms = pcg_batched_autotune(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], tol, maxit, ctx=ctx,
+                                            with_groups=True)
     pcg_batched(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], x, tol, maxit, its, errs, use_graph=True, ctx=ctx)
     ts = []
     for _ in range(3):
@@ -510,6 +530,7 @@ def bench_batched(cm, capi, ctx):
         loop_t += time.perf_counter() - t0
     loop_sps = len(sample) / loop_t
     return {"systems": nsys, "rows": int(P["b"].size), "nnz": int(P["IA"].size), "block_size": bs,
+            "workgroups_per_cu": gpc,
             "seconds": round(tb, 4), "solves_per_s": round(nsys / tb, 1),
             "plain_loop_solves_per_s": round(loop_sps, 1), "speedup_vs_plain_loop": round(nsys / tb / loop_sps, 2),
             "iterations_total": int(ih.sum()), "iterations_max": int(ih.max()), "max_error": float(eh.max())}

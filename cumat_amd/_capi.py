@@ -117,7 +117,7 @@ _PROTOS = {
     "cm_pcg_batched_solve": ([_VP, _I32, _VP, _I32, _I32, _VP, _VP, _VP, _VP, _VP, C.c_float, _I32,
                               _VP, _VP, _I32, _I32], _I32),
     "cm_pcg_batched_autotune": ([_VP, _I32, _VP, _I32, _I32, _VP, _VP, _VP, _VP, C.c_float, _I32,
-                                 C.POINTER(C.c_int), C.POINTER(C.c_float)], _I32),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)], _I32),
     "cm_pcg_ir_solve": ([_VP, _I32, _I32, _VP, _VP, _VP, _I32, _VP, _VP, _F64, _F64, _I32, _I32,
                          C.POINTER(IrStats)], _I32),
     "cm_edge_assemble": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _VP], _I32),

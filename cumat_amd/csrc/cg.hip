@@ -251,16 +251,19 @@ __global__ __launch_bounds__(BS) void cg_resident_kernel(const ResidentArgs<T> A
 }
 
 // --------------------------------------------------------------- host side
+// grid <= 0: a persistent grid of 'gpc' workgroups per CU (gpc <= 0: as
+// many as the occupancy calculator allows; the workgroups pop systems from
+// the queue, so more than can be resident only wait their turn).
 template <typename T, int BS, int RPT>
-static int launch_resident_one(cm_ctx_t ctx, const ResidentArgs<T>& A, int grid, size_t lds)
+static int launch_resident_one(cm_ctx_t ctx, const ResidentArgs<T>& A, int grid, size_t lds, int gpc)
 {
     auto k = cg_resident_kernel<T, BS, RPT>;
     CM_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (grid <= 0) {
-        // persistent grid: as many workgroups as can be resident at once
         int per_cu = 0;
         CM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, BS, lds));
         if (per_cu < 1) return fail(CM_E_UNSUPPORTED, "resident CG: tile does not fit on a CU");
+        if (gpc > 0) per_cu = gpc;
         grid = std::min(A.nsys, per_cu * (ctx->cu_count > 0 ? ctx->cu_count : 256));
     }
     if (grid <= 0) return CM_OK;  // nothing to solve (attribute set up only)
@@ -271,30 +274,30 @@ static int launch_resident_one(cm_ctx_t ctx, const ResidentArgs<T>& A, int grid,
 
 // Picks the register tile RPT for a block size and the largest system.
 template <typename T, int BS>
-static int launch_resident_bs(cm_ctx_t ctx, const ResidentArgs<T>& A, int nmax, int grid)
+static int launch_resident_bs(cm_ctx_t ctx, const ResidentArgs<T>& A, int nmax, int grid, int gpc)
 {
     const size_t lds = sizeof(T) * (size_t)nmax;
     const int rpt = (nmax + BS - 1) / BS;
-    if (rpt <= 1) return launch_resident_one<T, BS, 1>(ctx, A, grid, lds);
-    if (rpt <= 2) return launch_resident_one<T, BS, 2>(ctx, A, grid, lds);
-    if (rpt <= 4) return launch_resident_one<T, BS, 4>(ctx, A, grid, lds);
-    if (rpt <= 8) return launch_resident_one<T, BS, 8>(ctx, A, grid, lds);
-    if (rpt <= 12) return launch_resident_one<T, BS, 12>(ctx, A, grid, lds);
-    if (rpt <= 16) return launch_resident_one<T, BS, 16>(ctx, A, grid, lds);
+    if (rpt <= 1) return launch_resident_one<T, BS, 1>(ctx, A, grid, lds, gpc);
+    if (rpt <= 2) return launch_resident_one<T, BS, 2>(ctx, A, grid, lds, gpc);
+    if (rpt <= 4) return launch_resident_one<T, BS, 4>(ctx, A, grid, lds, gpc);
+    if (rpt <= 8) return launch_resident_one<T, BS, 8>(ctx, A, grid, lds, gpc);
+    if (rpt <= 12) return launch_resident_one<T, BS, 12>(ctx, A, grid, lds, gpc);
+    if (rpt <= 16) return launch_resident_one<T, BS, 16>(ctx, A, grid, lds, gpc);
     if constexpr (BS <= 512) {
-        if (rpt <= 24) return launch_resident_one<T, BS, 24>(ctx, A, grid, lds);
-        if (rpt <= 32) return launch_resident_one<T, BS, 32>(ctx, A, grid, lds);
+        if (rpt <= 24) return launch_resident_one<T, BS, 24>(ctx, A, grid, lds, gpc);
+        if (rpt <= 32) return launch_resident_one<T, BS, 32>(ctx, A, grid, lds, gpc);
     }
     return fail(CM_E_UNSUPPORTED, "resident CG: %d rows exceed %d threads x tile", nmax, BS);
 }
 
 template <typename T>
-static int launch_resident(cm_ctx_t ctx, const ResidentArgs<T>& A, int nmax, int bs, int grid)
+static int launch_resident(cm_ctx_t ctx, const ResidentArgs<T>& A, int nmax, int bs, int grid, int gpc = 0)
 {
     switch (bs) {
-    case 256: return launch_resident_bs<T, 256>(ctx, A, nmax, grid);
-    case 512: return launch_resident_bs<T, 512>(ctx, A, nmax, grid);
-    case 1024: return launch_resident_bs<T, 1024>(ctx, A, nmax, grid);
+    case 256: return launch_resident_bs<T, 256>(ctx, A, nmax, grid, gpc);
+    case 512: return launch_resident_bs<T, 512>(ctx, A, nmax, grid, gpc);
+    case 1024: return launch_resident_bs<T, 1024>(ctx, A, nmax, grid, gpc);
     default: return fail(CM_E_ARG, "resident CG: block size %d (256, 512 or 1024)", bs);
     }
 }
@@ -388,7 +391,7 @@ static int make_plan(cm_ctx_t ctx, int nsys, const int* row_off_dev, int total_r
 struct GraphKey {
     cm_ctx_t ctx;
     const void* ptrs[9];
-    int nsys, bs, max_iter, nmax, total_rows, total_nnz;
+    int nsys, bs, gpc, max_iter, nmax, total_rows, total_nnz;
     float tol;
     uint64_t offsets_hash;
     bool operator<(const GraphKey& o) const { return memcmp(this, &o, sizeof(GraphKey)) < 0; }
@@ -406,7 +409,8 @@ constexpr size_t TUNE_CACHE_MAX = 64;
 static std::mutex g_graph_mutex;
 static std::map<GraphKey, GraphEntry> g_graphs;
 static unsigned long long g_graph_clock = 0;
-static std::map<std::tuple<int, int, int, int>, int> g_tuned_bs;
+// autotuned launch shape per batch shape: (block size, workgroups per CU)
+static std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_tuned_bs;
 
 // Destroys a cached graph and returns its blocks (caller holds the mutex; the
 // stream is idle after the synchronous solve that last used it).
@@ -434,7 +438,8 @@ void graph_cache_drop(cm_ctx_t ctx)
 
 static int batched_launch(cm_ctx_t ctx, int nsys, const int* row_off, int total_nnz, const int* JA, const int* IA,
                           const float* vals, const float* b, float* x, float tol, int max_iter, int* iters,
-                          float* errors, int bs, const BatchedPlan& plan, void* order_dev, void* queue_dev)
+                          float* errors, int bs, const BatchedPlan& plan, void* order_dev, void* queue_dev,
+                          int gpc = 0)
 {
     CM_TRY(fill_zero(ctx, queue_dev, 256));
     ResidentArgs<float> A{};
@@ -454,7 +459,7 @@ static int batched_launch(cm_ctx_t ctx, int nsys, const int* row_off, int total_
     A.err_out = errors;
     A.queue = (unsigned*)queue_dev;
     A.nnz_total = total_nnz;
-    return launch_resident<float>(ctx, A, plan.nmax, bs, 0);
+    return launch_resident<float>(ctx, A, plan.nmax, bs, 0, gpc);
 }
 
 static int default_bs(int nmax)
@@ -576,12 +581,17 @@ static int pcg_ir(cm_ctx_t ctx, int n, int nnz, const int* JA, const int* IA, co
     CM_TRY(::cm::host_sync(ctx));
     int outer = 0, inner_total = 0;
     double rel = (hbn2 > 0) ? 1.0 : 0.0;
+    double prev = 2.0;
     while (hbn2 > 0) {
         CM_TRY(cm_dot(ctx, CM_F64, n, 1, r, 1, r, 1, rn2));
         CM_HIP(hipMemcpyAsync(&hrn2, rn2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         CM_TRY(::cm::host_sync(ctx));
         rel = std::sqrt(hrn2 / hbn2);
         if (rel <= tol || outer >= max_outer || hrn2 == 0.0) break;
+        // stagnation: an outer step that no longer halves the FP64 residual
+        // has reached the rounding floor of the FP64 residual itself
+        if (rel > 0.5 * prev) break;
+        prev = rel;
         ir_scale_kernel<<<g, 256, 0, ctx->stream>>>(n, (const double*)r, rn2, (float*)r32);
         CM_LAUNCH_CHECK();
         int it = 0;
@@ -628,10 +638,16 @@ int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off, int total_r
     cm::BatchedPlan plan;
     CM_TRY(cm::make_plan(ctx, nsys, row_off, total_rows, plan));
     CM_REQUIRE(plan.nmax <= 16384, CM_E_UNSUPPORTED, "batched CG: system of %d rows (max 16384)", plan.nmax);
+    int gpc = 0;
     if (block_size == 0) {
         std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
         auto it = cm::g_tuned_bs.find({nsys, total_rows, total_nnz, plan.nmax});
-        block_size = (it != cm::g_tuned_bs.end()) ? it->second : cm::default_bs(plan.nmax);
+        if (it != cm::g_tuned_bs.end()) {
+            block_size = it->second.first;
+            gpc = it->second.second;
+        } else {
+            block_size = cm::default_bs(plan.nmax);
+        }
     }
     unsigned bad = 0;
     if (!use_graph) {
@@ -641,7 +657,7 @@ int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off, int total_r
         CM_TRY(tmp.get(256, &q));
         CM_HIP(hipMemcpyAsync(ord, plan.order.data(), sizeof(int) * nsys, hipMemcpyHostToDevice, ctx->stream));
         CM_TRY(cm::batched_launch(ctx, nsys, row_off, total_nnz, JA, IA, vals, b, x, tol, max_iter, iterations,
-                                  errors, block_size, plan, ord, q));
+                                  errors, block_size, plan, ord, q, gpc));
         CM_HIP(hipMemcpyAsync(&bad, (unsigned*)q + 1, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
         CM_TRY(::cm::host_sync(ctx));
     } else {
@@ -654,6 +670,7 @@ int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off, int total_r
         memcpy(key.ptrs, ps, sizeof(ps));
         key.nsys = nsys;
         key.bs = block_size;
+        key.gpc = gpc;
         key.max_iter = max_iter;
         key.nmax = plan.nmax;
         key.total_rows = total_rows;
@@ -687,7 +704,7 @@ int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off, int total_r
             // the kernel attribute must be set outside capture
             if (s == CM_OK)
                 s = cm::batched_launch(ctx, 0, row_off, total_nnz, JA, IA, vals, b, x, tol, max_iter, iterations,
-                                       errors, block_size, plan, ge.order, ge.queue);
+                                       errors, block_size, plan, ge.order, ge.queue, gpc);
             if (s == CM_OK && ::cm::host_sync(ctx) != CM_OK)
                 s = cm::fail(CM_E_HIP, "batched CG: stream synchronization failed");
             if (s == CM_OK) {
@@ -695,7 +712,7 @@ int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off, int total_r
                     s = cm::fail(CM_E_HIP, "batched CG: stream capture failed to start");
                 } else {
                     s = cm::batched_launch(ctx, nsys, row_off, total_nnz, JA, IA, vals, b, x, tol, max_iter,
-                                           iterations, errors, block_size, plan, ge.order, ge.queue);
+                                           iterations, errors, block_size, plan, ge.order, ge.queue, gpc);
                     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
                     if (s == CM_OK && e != hipSuccess)
                         s = cm::fail(CM_E_HIP, "batched CG: capture failed: %s", hipGetErrorString(e));
@@ -726,7 +743,7 @@ int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off, int total_r
 
 int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off, int total_rows, int total_nnz, const int* JA,
                             const int* IA, const float* vals, const float* b, float tol, int max_iter,
-                            int* best_block_size, float* best_ms)
+                            int* best_block_size, int* best_groups_per_cu, float* best_ms)
 {
     CM_API_BEGIN
     CM_TRY(cm::resolve(ctx, &ctx));
@@ -745,17 +762,21 @@ int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off, int tota
     hipEvent_t e0, e1;
     CM_HIP(hipEventCreate(&e0));
     CM_HIP(hipEventCreate(&e1));
-    int best = 0;
+    int best = 0, best_gpc = 0;
     float bestms = 1e30f;
+    // launch shapes: threads per system (the register tile follows from the
+    // largest system) x workgroups per CU of the persistent grid
     const int cands[3] = {256, 512, 1024};
+    const int gpcs[3] = {1, 2, 4};
     int status = CM_OK;
-    for (int bs : cands) {
+    for (int c = 0; c < 9 && status == CM_OK; ++c) {
+        const int bs = cands[c / 3], gpc = gpcs[c % 3];
         if ((plan.nmax + bs - 1) / bs > (bs <= 512 ? 32 : 16)) continue;  // register tile too large
         float ms_min = 1e30f;
         for (int rep = 0; rep < 2 && status == CM_OK; ++rep) {
             if (hipEventRecord(e0, ctx->stream) != hipSuccess) { status = cm::fail(CM_E_HIP, "autotune: event"); break; }
             status = cm::batched_launch(ctx, nsys, row_off, total_nnz, JA, IA, vals, b, (float*)xs, tol, max_iter,
-                                        (int*)it, (float*)er, bs, plan, ord, q);
+                                        (int*)it, (float*)er, bs, plan, ord, q, gpc);
             if (status != CM_OK) break;
             float ms = 0;
             if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -766,7 +787,7 @@ int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off, int tota
             ms_min = std::min(ms_min, ms);
         }
         if (status != CM_OK) break;
-        if (ms_min < bestms) { bestms = ms_min; best = bs; }
+        if (ms_min < bestms) { bestms = ms_min; best = bs; best_gpc = gpc; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -779,9 +800,10 @@ int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off, int tota
     {
         std::lock_guard<std::mutex> lk(cm::g_graph_mutex);
         if (cm::g_tuned_bs.size() >= cm::TUNE_CACHE_MAX) cm::g_tuned_bs.clear();
-        cm::g_tuned_bs[{nsys, total_rows, total_nnz, plan.nmax}] = best;
+        cm::g_tuned_bs[{nsys, total_rows, total_nnz, plan.nmax}] = {best, best_gpc};
     }
     if (best_block_size) *best_block_size = best;
+    if (best_groups_per_cu) *best_groups_per_cu = best_gpc;
     if (best_ms) *best_ms = bestms;
     return CM_OK;
     CM_API_END

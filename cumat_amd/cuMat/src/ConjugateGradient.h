@@ -203,6 +203,7 @@ class BatchedConjugateGradient {
     float tol_ = 1e-6f;
     int maxIter_ = 1000;
     int blockSize_ = 0;
+    int tunedBlockSize_ = 0;
     bool useGraph_ = true;
 
 public:
@@ -217,18 +218,21 @@ public:
     BatchedConjugateGradient& setUseGraph(bool g) { useGraph_ = g; return *this; }
     Index systems() const { return rowOff_.rows() - 1; }
 
-    // picks the block size by timing the candidates on this batch
+    // picks the launch shape (threads per system x workgroups per CU) by
+    // timing the candidates on this batch; later solves use it
     int autotune(const VectorXf& b)
     {
         const auto& P = A_.getSparsityPattern();
-        int bs = 0;
+        int bs = 0, gpc = 0;
         float ms = 0;
         CUMAT_AMD_CALL(cm_pcg_batched_autotune(Context::current().handle(), (int)systems(), rowOff_.data(),
                                                (int)A_.rows(), (int)A_.nnz(), P.JA.data(), P.IA.data(),
-                                               A_.getData().data(), b.data(), tol_, maxIter_, &bs, &ms));
-        blockSize_ = bs;
+                                               A_.getData().data(), b.data(), tol_, maxIter_, &bs, &gpc, &ms));
+        blockSize_ = 0;  // the tuned shape, block size and grid together
+        tunedBlockSize_ = bs;
         return bs;
     }
+    int tunedBlockSize() const { return tunedBlockSize_; }
 
     // x (zero initial guess) and per-system iteration counts / errors
     void solve(const VectorXf& b, VectorXf& x, VectorXi& iterations, VectorXf& errors) const

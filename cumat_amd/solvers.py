@@ -139,12 +139,19 @@ def pcg_batched(row_off: DeviceArray, JA: DeviceArray, IA: DeviceArray, vals: De
               int(block_size), int(use_graph))
 
 
-def pcg_batched_autotune(row_off, JA, IA, vals, b, tol, max_iter, ctx=None):
+def pcg_batched_autotune(row_off, JA, IA, vals, b, tol, max_iter, ctx=None, with_groups=False):
+    """Times the launch shapes of the persistent batched CG on this batch
+    (threads per system x workgroups per CU); a later pcg_batched of the same
+    batch shape with block_size=0 uses the best. Returns (block size, ms), or
+    (block size, workgroups per CU, ms) with with_groups=True."""
     bs = C.c_int(0)
+    gpc = C.c_int(0)
     ms = C.c_float(0)
     capi.call("cm_pcg_batched_autotune", ctx_handle(ctx), row_off.size - 1, row_off.ptr, b.size,
               IA.size, JA.ptr, IA.ptr, vals.ptr, b.ptr, C.c_float(tol), int(max_iter), C.byref(bs),
-              C.byref(ms))
+              C.byref(gpc), C.byref(ms))
+    if with_groups:
+        return bs.value, gpc.value, ms.value
     return bs.value, ms.value
 
 

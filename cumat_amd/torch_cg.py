@@ -107,6 +107,31 @@ class EdgeLaplacian:
                   C.byref(st))
         return x, list(its), list(errs)
 
+    def solve_mixed(self, vals: torch.Tensor, rhs: torch.Tensor, tol: float, inner_tol: float = 1e-5,
+                    storage: str = "f32", max_outer: int = 50):
+        """Per-batch mixed-precision CG with FP64 refinement (cm_pcg_ir_solve):
+        FP64 values and right-hand sides, FP32 (or BF16) inner solves, the
+        residual and the solution kept in FP64 until the FP64 relative
+        residual is at most tol. Returns (x, inner iterations per batch,
+        final relative residual per batch)."""
+        self._check("rhs", rhs, self.n, dtype=torch.float64)
+        self._check("vals", vals, self.nnz, batches=rhs.shape[0], dtype=torch.float64)
+        vals, rhs = vals.contiguous(), rhs.contiguous()
+        B = rhs.shape[0]
+        x = torch.empty_like(rhs)
+        h = _StreamCtx.get(rhs.device)
+        vdt = capi.BF16 if storage == "bf16" else capi.F32
+        its, res = [], []
+        for k in range(B):
+            st = capi.IrStats()
+            capi.call("cm_pcg_ir_solve", h, self.n, self.nnz, _p(self.JA), _p(self.IA),
+                      C.c_void_p(vals.data_ptr() + 8 * k * self.nnz), vdt,
+                      C.c_void_p(rhs.data_ptr() + 8 * k * self.n), C.c_void_p(x.data_ptr() + 8 * k * self.n),
+                      float(tol), float(inner_tol), int(max_outer), -1, C.byref(st))
+            its.append(st.inner_iterations)
+            res.append(st.final_rel_residual)
+        return x, its, res
+
     def edge_grad(self, lam: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
         self._check("x", x, self.n)
         self._check("lam", lam, self.n, batches=x.shape[0], dtype=x.dtype)
@@ -120,18 +145,32 @@ class EdgeLaplacian:
 
 
 class CGSolve(torch.autograd.Function):
-    """x = A(theta)^{-1} b, differentiable in theta and b."""
+    """x = A(theta)^{-1} b, differentiable in theta and b.
+
+    precision "native": Jacobi-PCG in the dtype of theta and b (batched, one
+    launch sequence for all right-hand sides; stops at the relative error
+    tol). "mixed": float64 theta and b, each solve by FP32 (or, with
+    "mixed-bf16", BF16-valued) inner CG inside FP64 refinement to the FP64
+    relative residual tol -- FP64 accuracy of x, lambda and the gradient
+    with the inner iterations in single precision."""
 
     @staticmethod
-    def forward(ctx, theta, b, lap: EdgeLaplacian, tol: float, max_iter: int):
+    def forward(ctx, theta, b, lap: EdgeLaplacian, tol: float, max_iter: int, precision: str = "native"):
         # theta and b must agree in dtype and batch count: the solver reads
         # one (B, nnz) block of values per right-hand side
         lap._check("b", b, lap.n)
         lap._check("theta", theta, lap.nedges, batches=b.shape[0], dtype=b.dtype)
+        if precision not in ("native", "mixed", "mixed-bf16"):
+            raise capi.CumatArgError(capi.CM_E_ARG, f"precision {precision!r} (native, mixed, mixed-bf16)")
+        if precision != "native" and b.dtype != torch.float64:
+            raise capi.CumatArgError(capi.CM_E_ARG, "mixed precision takes float64 theta and b")
         vals = lap.assemble(theta)
-        x, its, errs = lap.solve(vals, b, tol, max_iter)
+        if precision == "native":
+            x, its, errs = lap.solve(vals, b, tol, max_iter)
+        else:
+            x, its, errs = lap.solve_mixed(vals, b, tol, storage="bf16" if precision == "mixed-bf16" else "f32")
         ctx.save_for_backward(vals, x)
-        ctx.lap, ctx.tol, ctx.max_iter = lap, tol, max_iter
+        ctx.lap, ctx.tol, ctx.max_iter, ctx.precision = lap, tol, max_iter, precision
         ctx.forward_stats = (its, errs)
         CGSolve.last_forward_stats = (its, errs)
         return x
@@ -140,12 +179,16 @@ class CGSolve(torch.autograd.Function):
     def backward(ctx, gx):
         vals, x = ctx.saved_tensors
         lap = ctx.lap
-        lam, its, errs = lap.solve(vals, gx, ctx.tol, ctx.max_iter)  # A^T = A
+        if ctx.precision == "native":
+            lam, its, errs = lap.solve(vals, gx, ctx.tol, ctx.max_iter)  # A^T = A
+        else:
+            lam, its, errs = lap.solve_mixed(vals, gx, ctx.tol,
+                                             storage="bf16" if ctx.precision == "mixed-bf16" else "f32")
         CGSolve.last_backward_stats = (its, errs)
         gtheta = lap.edge_grad(lam, x)
-        return gtheta, lam, None, None, None
+        return gtheta, lam, None, None, None, None
 
 
 def cg_solve(theta: torch.Tensor, b: torch.Tensor, lap: EdgeLaplacian, tol: float = 1e-7,
-             max_iter: int = 20000) -> torch.Tensor:
-    return CGSolve.apply(theta, b, lap, tol, max_iter)
+             max_iter: int = 20000, precision: str = "native") -> torch.Tensor:
+    return CGSolve.apply(theta, b, lap, tol, max_iter, precision)

@@ -455,7 +455,9 @@ int cm_pcg_dense_solve(cm_ctx_t ctx, int dtype, int n, int batches,
  * JA[row_off[s]] of one concatenated block-diagonal CSR (column indices are
  * global). Each system follows the plain solver's semantics on its own and
  * stops when it converges; iterations[s] and errors[s] are device arrays.
- * block_size/rows_per_thread == 0 take the autotuned choice.
+ * block_size == 0 takes the shape cm_pcg_batched_autotune chose for this
+ * batch shape (or a default when it was not tuned); otherwise the block
+ * size is block_size and the grid as many workgroups as fit.
  * ====================================================================== */
 int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off,
                          int total_rows, int total_nnz,
@@ -463,13 +465,17 @@ int cm_pcg_batched_solve(cm_ctx_t ctx, int nsys, const int* row_off,
                          const float* b, float* x, float tol, int max_iter,
                          int* iterations, float* errors,
                          int block_size, int use_graph);
-/* Autotuner: times the candidate launch shapes on the given batch and
- * returns the best block size (host int). */
+/* Autotuner: times the candidate launch shapes on the given batch -- threads
+ * per system (256, 512, 1024; the register tile follows from the largest
+ * system) x workgroups per CU of the persistent grid (1, 2, 4) -- and
+ * returns the best (host ints; either pointer may be NULL). A later
+ * cm_pcg_batched_solve of the same batch shape with block_size == 0 uses
+ * that shape. */
 int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off,
                             int total_rows, int total_nnz, const int* JA,
                             const int* IA, const float* vals, const float* b,
                             float tol, int max_iter, int* best_block_size,
-                            float* best_ms);
+                            int* best_groups_per_cu, float* best_ms);
 
 /* ======================================================================
  * Addition 1 (no reference counterpart): mixed-precision CG with FP64
@@ -478,7 +484,10 @@ int cm_pcg_batched_autotune(cm_ctx_t ctx, int nsys, const int* row_off,
  * CG on A e = r with FP32 vectors and FP32 (val_dtype CM_F32) or BF16
  * (CM_BF16) matrix storage. BF16 storage keeps the diagonal in FP32 and
  * folds the rounding of each row's off-diagonals into it, so every row sum
- * of the stored matrix equals the row sum of A.
+ * of the stored matrix equals the row sum of A. The outer loop stops at the
+ * FP64 relative residual tol, after max_outer steps, or when a step no
+ * longer halves the FP64 residual (the rounding floor of the residual
+ * itself; final_rel_residual then reports where it stopped).
  * ====================================================================== */
 typedef struct cm_ir_stats {
     int32_t outer_iterations;

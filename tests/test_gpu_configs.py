@@ -250,3 +250,44 @@ def test_lincomb8_2e26_against_fp64(gpu):
     ratio = np.abs(got - ref) / bound
     _say(f"lincomb 8 x 2^26: max error / bound {ratio.max():.3f}")
     assert ratio.max() <= 1.0
+
+
+def test_adjoint_mixed_precision_512sq(gpu, orc):
+    """The adjoint with FP32 inner solves inside FP64 refinement (precision
+    "mixed"): on two 512^2 systems of the benchmark family the gradient is
+    within 1e-6 relative of the FP64 path's gradient (which the test above
+    pins to central differences and the FP64 oracle) -- the accuracy the
+    FP32 solves at tol 1e-7 cannot reach."""
+    import torch
+    from cumat_amd.problems import grid_laplacian
+    from cumat_amd.torch_cg import CGSolve, EdgeLaplacian, cg_solve
+    g = 512
+    rng = np.random.default_rng(20261206)
+    P = grid_laplacian((g, g), lambda s: np.ones(s))
+    n, E = P["n"], P["nedges"]
+    lap = EdgeLaplacian(P, "cuda")
+    theta = np.exp(0.5 * rng.standard_normal((64, E)))[[0, 37]]
+    b = rng.uniform(-1, 1, (64, n))[[0, 37]]
+    xs = rng.standard_normal((64, n))[[0, 37]]
+    grads, res = {}, []
+    for prec, tol in (("native", 1e-13), ("mixed", 1e-12)):
+        th = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+        bb = torch.tensor(b, dtype=torch.float64, device="cuda")
+        x = cg_solve(th, bb, lap, tol=tol, max_iter=40000, precision=prec)
+        if prec == "mixed":
+            res += CGSolve.last_forward_stats[1]
+        (0.5 * ((x - torch.tensor(xs, dtype=torch.float64, device="cuda")) ** 2).sum()).backward()
+        if prec == "mixed":
+            res += CGSolve.last_backward_stats[1]
+        grads[prec] = th.grad.detach().cpu().numpy()
+    e = max(np.linalg.norm(grads["mixed"][k] - grads["native"][k]) / np.linalg.norm(grads["native"][k])
+            for k in range(2))
+    _say(f"adjoint mixed precision vs FP64 gradient: {e:.2e}; FP64 relative residuals of the refined solves "
+         f"{', '.join(f'{r:.1e}' for r in res)}")
+    # the refinement stops at 1e-12 or where the FP64 residual stops falling
+    # (its own rounding floor, a few 1e-12 for the adjoint right-hand sides)
+    assert max(res) <= 1e-11, res
+    assert e <= 1e-6, e
+    with pytest.raises(Exception):
+        cg_solve(torch.tensor(theta, dtype=torch.float32, device="cuda"),
+                 torch.tensor(b, dtype=torch.float32, device="cuda"), lap, precision="mixed")

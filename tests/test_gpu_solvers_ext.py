@@ -98,8 +98,16 @@ def test_batched_autotune_and_block_sizes(gpu, orc):
     P = _batched_problem(64, 30, 70, seed=9)
     d = {k: cm.DeviceArray.from_numpy(P[k]) for k in ("row_off", "JA", "IA", "vals", "b")}
     from cumat_amd.solvers import pcg_batched_autotune
-    bs, ms = pcg_batched_autotune(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], 1e-6, 2000)
-    assert bs in (256, 512, 1024) and ms > 0
+    bs, gpc, ms = pcg_batched_autotune(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], 1e-6, 2000,
+                                       with_groups=True)
+    assert bs in (256, 512, 1024) and gpc in (1, 2, 4) and ms > 0
+    # block_size=0 solves with the tuned shape (threads per system and
+    # workgroups per CU): every system converges, iterate close to the others
+    xt = cm.DeviceArray(P["b"].size, dtype=np.float32)
+    itst = cm.DeviceArray(64, dtype=np.int32)
+    errst = cm.DeviceArray(64, dtype=np.float32)
+    cm.pcg_batched(d["row_off"], d["JA"], d["IA"], d["vals"], d["b"], xt, 1e-6, 2000, itst, errst, block_size=0)
+    assert (errst.numpy() <= 1e-6).all()
     res = []
     for b_s in (256, 512, 1024):
         x = cm.DeviceArray(P["b"].size, dtype=np.float32)
@@ -110,6 +118,8 @@ def test_batched_autotune_and_block_sizes(gpu, orc):
     # block size changes the reduction tree only: same iterations within 2
     assert np.abs(res[0][1] - res[2][1]).max() <= 2
     np.testing.assert_allclose(res[0][0], res[2][0], atol=1e-4)
+    np.testing.assert_allclose(xt.numpy(), res[2][0], atol=1e-4)
+    assert np.abs(itst.numpy() - res[2][1]).max() <= 2
 
 
 # ------------------------------------------------------------ mixed precision IR
