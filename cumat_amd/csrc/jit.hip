@@ -28,6 +28,9 @@ namespace jit {
 static const char kOpsText[] =
 #include "cwise_ops.inc"
     ;
+static const char kFoldText[] =
+#include "fold_ops.inc"
+    ;
 
 namespace {
 
@@ -88,6 +91,8 @@ std::string prelude()
     };
     for (const auto& c : k) appendf(s, "constexpr int %s = %d;\n", c.name, c.value);
     s += kOpsText;
+    s += "\n";
+    s += kFoldText;
     s += "\nusing namespace cm::ops;\n";
     return s;
 }

@@ -18,6 +18,7 @@
 // long segments fold their per-chunk partials in a second launch, the same
 // order every run (results are bit-reproducible).
 #include "cwise_core.hpp"
+#include "jit.hpp"
 
 #include <limits>
 
@@ -78,7 +79,9 @@ __device__ __forceinline__ Coords rx_coords(const RxShape& S, long long s, long 
     return q;
 }
 
-template <typename Op, typename T, typename S, bool W, int NL>
+// FLAT: every leaf is a broadcast scalar or lies in memory in the virtual
+// order (element j of segment s at s*L + j), so no coordinates are formed.
+template <typename Op, typename T, typename S, bool W, int NL, bool FLAT>
 __global__ __launch_bounds__(RX_BS) void reduce_expr_kernel(const CwProgram P, const RxShape SH, int nchunk,
                                                             long long chunk, T* __restrict__ partials,
                                                             S* __restrict__ out)
@@ -96,18 +99,29 @@ __global__ __launch_bounds__(RX_BS) void reduce_expr_kernel(const CwProgram P, c
         for (int k = 0; k < V; ++k) {
             const long long j = j0 + k;
             const bool ok = j < end;
-            const Coords q = rx_coords(SH, s, ok ? j : beg);
+            if constexpr (FLAT) {
+                const long long e = s * SH.L + (ok ? j : beg);
 #pragma unroll
-            for (int l = 0; l < NL; ++l) {
-                Reg<W> t;
-                t.v[0] = 0;
-                if (l < P.nleaves) {
-                    bool z;
-                    const long long off = leaf_offset(P.leaves[l], q, z);
-                    load_raw<W>(P.leaves[l], z ? 0 : off, t, 0);
-                    if (z) t.v[0] = 0;
+                for (int l = 0; l < NL; ++l) {
+                    Reg<W> t;
+                    t.v[0] = 0;
+                    if (l < P.nleaves) load_raw<W>(P.leaves[l], ((P.leaf_scalar >> l) & 1) ? 0 : e, t, 0);
+                    lv.k[k][l] = t.v[0];
                 }
-                lv.k[k][l] = t.v[0];
+            } else {
+                const Coords q = rx_coords(SH, s, ok ? j : beg);
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    Reg<W> t;
+                    t.v[0] = 0;
+                    if (l < P.nleaves) {
+                        bool z;
+                        const long long off = leaf_offset(P.leaves[l], q, z);
+                        load_raw<W>(P.leaves[l], z ? 0 : off, t, 0);
+                        if (z) t.v[0] = 0;
+                    }
+                    lv.k[k][l] = t.v[0];
+                }
             }
         }
         Reg<W> val;
@@ -135,9 +149,176 @@ __global__ __launch_bounds__(RX_BS) void reduce_expr_combine_kernel(const T* __r
     if (threadIdx.x == 0) out[s] = (S)r;
 }
 
-template <typename Op, typename T, typename S, bool W, int NL>
-static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P, const RxShape& SH, S* out)
+// Whether every leaf of P is a broadcast scalar or lies in the virtual order
+// of SH (marks the scalars in P.leaf_scalar).
+static bool rx_flat(CwProgram& P, const RxShape& SH)
 {
+    const long long dim[3] = {P.rows, P.cols, P.batches};
+    long long es[3] = {0, 0, 0};
+    long long st = 1;
+    for (int i = 0; i < SH.nr; ++i) es[SH.raxis[i]] = st, st *= SH.rdim[i];
+    for (int i = 0; i < SH.nk; ++i) es[SH.kaxis[i]] = st, st *= SH.kdim[i];
+    P.leaf_scalar = 0;
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        if (L.kind != CM_LEAF_STRIDED) return false;
+        const long long ls[3] = {L.sr, L.sc, L.sb};
+        bool scalar = true, flat = true;
+        for (int a = 0; a < 3; ++a) {
+            if (dim[a] <= 1) continue;
+            scalar = scalar && ls[a] == 0;
+            flat = flat && ls[a] == es[a];
+        }
+        if (scalar) P.leaf_scalar |= 1u << l;
+        else if (!flat) return false;
+    }
+    return true;
+}
+
+constexpr int RX_JIT_DECLINED = 1;
+
+// Arguments of the generated reduction kernel cm_jit_reduce.
+struct JitRxArgs {
+    const void* leaf[CM_CWISE_MAX_LEAVES];
+    void* partials;
+    void* out;
+    long long L, chunk;
+    int nchunk, pad;
+};
+
+template <typename Op> struct RxOpName;
+template <> struct RxOpName<OpSum> { static const char* get() { return "cm::OpSum"; } };
+template <> struct RxOpName<OpProd> { static const char* get() { return "cm::OpProd"; } };
+template <> struct RxOpName<OpMin> { static const char* get() { return "cm::OpMin"; } };
+template <> struct RxOpName<OpMax> { static const char* get() { return "cm::OpMax"; } };
+template <> struct RxOpName<OpAll> { static const char* get() { return "cm::OpAll"; } };
+template <> struct RxOpName<OpAny> { static const char* get() { return "cm::OpAny"; } };
+template <> struct RxOpName<OpBand> { static const char* get() { return "cm::OpBand"; } };
+template <> struct RxOpName<OpBor> { static const char* get() { return "cm::OpBor"; } };
+
+// The identity of Op in T as a source literal (the values of RxIdentity).
+template <typename Op, typename T>
+static std::string rx_identity_src()
+{
+    char buf[64];
+    const bool f = std::is_same<T, float>::value;
+    if (std::is_same<Op, OpMin>::value)
+        snprintf(buf, sizeof(buf), f ? "__uint_as_float(0x7f7fffffu)" : "2147483647");
+    else if (std::is_same<Op, OpMax>::value)
+        snprintf(buf, sizeof(buf), f ? "__uint_as_float(0xff7fffffu)" : "(-2147483647 - 1)");
+    else if (std::is_same<Op, OpProd>::value || std::is_same<Op, OpAll>::value)
+        snprintf(buf, sizeof(buf), f ? "1.0f" : "1");
+    else if (std::is_same<Op, OpBand>::value)
+        snprintf(buf, sizeof(buf), "(~0)");
+    else
+        snprintf(buf, sizeof(buf), f ? "0.0f" : "0");
+    return buf;
+}
+
+// Source and key of the generated FLAT reduction (32-bit programs): the
+// kernel of reduce_expr_kernel<Op, T, S, false, NL, true> with the program
+// inlined, 16-byte loads where a thread's 8 elements are whole and aligned,
+// the same per-thread fold order and the same block reduction
+// (fold_ops.hpp), so both give the same bits.
+template <typename Op, typename T, typename S>
+int jit_reduce_flat_source(const CwProgram& P, std::string& key, std::string& body)
+{
+    if (!std::is_same<T, float>::value && !std::is_same<T, int>::value) return RX_JIT_DECLINED;
+    char buf[256];
+    const char* tname = std::is_same<T, float>::value ? "float" : "int";
+    const char* sname = std::is_same<S, float>::value ? "float" : (std::is_same<S, int>::value ? "int" : "unsigned char");
+    snprintf(buf, sizeof(buf), "rxflat32|%s|%s|%s|", RxOpName<Op>::get(), tname, sname);
+    key = buf;
+    std::string decls, vload, sload, scalars, names[CM_CWISE_MAX_LEAVES];
+    const char* namep[CM_CWISE_MAX_LEAVES];
+    bool all_al = true;
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL) return RX_JIT_DECLINED;
+        const bool scalar = (P.leaf_scalar >> l) & 1;
+        const bool al = L.dtype != CM_BOOL && ((uintptr_t)L.ptr & 15) == 0;
+        snprintf(buf, sizeof(buf), "L%d%c%c|", L.dtype, scalar ? 's' : 'd', al ? 'a' : 'u');
+        key += buf;
+        const std::string ls = std::to_string(l);
+        const char* rd = L.dtype == CM_BOOL ? "(R)((const unsigned char*)A.leaf[" : "((const uint32_t*)A.leaf[";
+        if (scalar) {
+            names[l] = "c" + ls;
+            scalars += "    const R c" + ls + " = " + rd + ls + "])[0];\n";
+        } else {
+            names[l] = "v" + ls + "[k]";
+            decls += "        R v" + ls + "[8];\n";
+            if (!al) all_al = false;
+            if (al)
+                vload += "            { const u4v a = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[" + ls +
+                         "] + e0)); const u4v b = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[" + ls +
+                         "] + e0 + 4));\n              v" + ls + "[0] = a.x; v" + ls + "[1] = a.y; v" + ls + "[2] = a.z; v" +
+                         ls + "[3] = a.w; v" + ls + "[4] = b.x; v" + ls + "[5] = b.y; v" + ls + "[6] = b.z; v" + ls +
+                         "[7] = b.w; }\n";
+            sload += "#pragma unroll\n            for (int k = 0; k < 8; ++k) v" + ls + "[k] = " + rd + ls +
+                     "])[s * A.L + (j0 + k < end ? j0 + k : beg)];\n";
+        }
+        namep[l] = names[l].c_str();
+    }
+    for (int i = 0; i < P.ncode; ++i) {
+        const cm_instr& I = P.code[i];
+        uint64_t fb;
+        memcpy(&fb, &I.fimm, 8);
+        snprintf(buf, sizeof(buf), "%d,%d,%d,%d,%d,%d,%llx,%llx;", I.opcode, I.dtype, I.fn, I.src, I.arg, I.flags,
+                 (unsigned long long)fb, (unsigned long long)I.iimm);
+        key += buf;
+    }
+    std::string prog;
+    if (!jit::program_body(P, namep, prog)) return RX_JIT_DECLINED;
+    const char* get = std::is_same<T, float>::value ? "getf" : "geti";
+    body = std::string("typedef unsigned int u4v __attribute__((ext_vector_type(4)));\n") + "typedef " + tname +
+           " TT;\ntypedef " + sname + " SS;\ntypedef " + RxOpName<Op>::get() +
+           " OP;\n"
+           "struct JitRxArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) +
+           "]; void* partials; void* out; long long L, chunk; int nchunk, pad; };\n"
+           "extern \"C\" __global__ __launch_bounds__(256) void cm_jit_reduce(const JitRxArgs A)\n{\n"
+           "    typedef uint32_t R;\n"
+           "    __shared__ TT lds[4];\n"
+           "    const long long s = blockIdx.x / A.nchunk;\n"
+           "    const int c = blockIdx.x % A.nchunk;\n"
+           "    const long long beg = c * A.chunk;\n"
+           "    const long long end = min(A.L, beg + A.chunk);\n" +
+           scalars +
+           "    TT acc = " + rx_identity_src<Op, T>() + ";\n"
+           "    for (long long j0 = beg + (long long)threadIdx.x * 8; j0 < end; j0 += 256 * 8) {\n" + decls +
+           "        const long long e0 = s * A.L + j0;\n" +
+           (all_al ? "        if (j0 + 8 <= end && (e0 & 3) == 0) {\n" + vload + "        } else {\n"
+                   : std::string("        {\n")) +
+           sload +
+           "        }\n"
+           "#pragma unroll\n"
+           "        for (int k = 0; k < 8; ++k) {\n"
+           "            R acc_r = 0, t = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;\n"
+           "            (void)t; (void)s0; (void)s1; (void)s2; (void)s3;\n"
+           "#define acc acc_r\n" +
+           prog +
+           "#undef acc\n"
+           "            if (j0 + k < end) acc = OP::apply(acc, (TT)" + get + "(acc_r));\n"
+           "        }\n"
+           "    }\n"
+           "    acc = cm::block_reduce<OP, 256>(acc, lds);\n"
+           "    if (threadIdx.x == 0) {\n"
+           "        if (A.nchunk == 1) ((SS*)A.out)[s] = (SS)acc;\n"
+           "        else ((TT*)A.partials)[blockIdx.x] = acc;\n"
+           "    }\n"
+           "}\n";
+    return CM_OK;
+}
+
+// instances with external linkage, so the generator can be exercised on a
+// host without a GPU (source generation and compilation only)
+template int jit_reduce_flat_source<OpSum, float, float>(const CwProgram&, std::string&, std::string&);
+template int jit_reduce_flat_source<OpAll, int, unsigned char>(const CwProgram&, std::string&, std::string&);
+
+template <typename Op, typename T, typename S, bool W, int NL>
+static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P0, const RxShape& SH, S* out)
+{
+    CwProgram P = P0;
+    const bool flat = rx_flat(P, SH);
     constexpr int V = CwVec<W>::v;
     const long long cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
     // chunks of at least one pass of the block; about 8 blocks per CU overall
@@ -153,9 +334,36 @@ static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P, const RxShape& S
     CM_REQUIRE(blocks < (1ll << 31), CM_E_DIM, "cm_reduce_expr: reduction too large");
     void* part = nullptr;
     if (nchunk > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * (size_t)blocks, &part));
-    reduce_expr_kernel<Op, T, S, W, NL><<<(unsigned)blocks, RX_BS, 0, ctx->stream>>>(P, SH, (int)nchunk, chunk,
-                                                                                     (T*)part, out);
-    CM_LAUNCH_CHECK();
+    bool done = false;
+    if constexpr (!W) {
+        if (flat && jit::want(SH.L * SH.nseg)) {
+            std::string key, body;
+            jit::Kernel k;
+            if (jit_reduce_flat_source<Op, T, S>(P, key, body) == CM_OK && jit::get(key, body, "cm_jit_reduce", &k) == 0) {
+                JitRxArgs A;
+                memset(&A, 0, sizeof(A));
+                for (int l = 0; l < P.nleaves; ++l) A.leaf[l] = P.leaves[l].ptr;
+                A.partials = part;
+                A.out = out;
+                A.L = SH.L;
+                A.chunk = chunk;
+                A.nchunk = (int)nchunk;
+                if (jit::launch(k, (unsigned)blocks, RX_BS, ctx->stream, &A, sizeof(A)) != 0)
+                    return fail(CM_E_HIP, "cm_reduce_expr: launch of the generated kernel failed");
+                count(CM_CNT_KERNEL_LAUNCH);
+                done = true;
+            }
+        }
+    }
+    if (!done) {
+        if (flat)
+            reduce_expr_kernel<Op, T, S, W, NL, true><<<(unsigned)blocks, RX_BS, 0, ctx->stream>>>(
+                P, SH, (int)nchunk, chunk, (T*)part, out);
+        else
+            reduce_expr_kernel<Op, T, S, W, NL, false><<<(unsigned)blocks, RX_BS, 0, ctx->stream>>>(
+                P, SH, (int)nchunk, chunk, (T*)part, out);
+        CM_LAUNCH_CHECK();
+    }
     if (nchunk > 1) {
         reduce_expr_combine_kernel<Op, T, S><<<(unsigned)SH.nseg, RX_BS, 0, ctx->stream>>>((const T*)part,
                                                                                           (int)nchunk, out);

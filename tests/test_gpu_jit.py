@@ -357,3 +357,102 @@ def test_csr_cwise_rejects_null_pointers_and_bad_programs_before_launch(gpu):
                               capi.F32, 0)
     assert rc == capi.CM_E_ARG
     assert cm.counters()["KernelLaunch"] == 0
+
+
+def _rx(cm, leaves, code, dtype, op, axis, shape, out_np, jit):
+    """cm_reduce_expr of a raw program over a rows x cols x batches shape;
+    leaves are (array, (sr, sc, sb)) with element strides."""
+    from cumat_amd import _capi as capi
+    from cumat_amd.device import np_to_dtype
+    R, Cc, B = shape
+    L = (capi.Leaf * max(1, len(leaves)))()
+    for i, (lf, st) in enumerate(leaves):
+        L[i].ptr, L[i].dtype, L[i].kind = lf.ptr, np_to_dtype(lf.arr.dtype), capi.LEAF_STRIDED
+        L[i].sr, L[i].sc, L[i].sb = st
+    prog = (capi.Instr * len(code))(*code)
+    oR = 1 if axis & 1 else R
+    oC = 1 if axis & 2 else Cc
+    oB = 1 if axis & 4 else B
+    out = cm.DeviceArray(oR * oC * oB, dtype=out_np)
+    old = cm.jit_mode(jit)
+    try:
+        capi.call("cm_reduce_expr", None, L, len(leaves), prog, len(code), dtype, op, axis, R, Cc, B, 0, out.ptr)
+        return out.numpy()
+    finally:
+        cm.jit_mode(old)
+
+
+@pytest.mark.parametrize("op", ["sum", "prod", "min", "max"])
+@pytest.mark.parametrize("axis", [7, 3])
+def test_reduce_expression_generated_f32(gpu, data, op, axis):
+    """(a*b + c) reduced over all elements or per batch (the virtual order is
+    the memory order: the generated reduction) -- the same bits as the
+    interpreter; one launch of the generated kernel (plus the combine)."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    ops = {"sum": capi.RED_SUM, "prod": capi.RED_PROD, "min": capi.RED_MIN, "max": capi.RED_MAX}
+    B = 4
+    R = (N - 3) // B
+    shape = (R, 1, B)
+    st = (1, R, R)
+    g = data["g"] if op != "prod" else _Leaf(gpu, (1.0 + data["f"].arr * 1e-4).astype(np.float32))
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_MUL, capi.SRC_LEAF, 1),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_ADD, capi.SRC_LEAF, 2)]
+    leaves = [(data["f"], st), (g, st), (data["g"], st)]
+    cm.reset_counters()
+    ref = _rx(cm, leaves, code, capi.F32, ops[op], axis, shape, np.float32, jit=0)
+    assert cm.counters()["JitLaunch"] == 0
+    got = _rx(cm, leaves, code, capi.F32, ops[op], axis, shape, np.float32, jit=2)
+    assert cm.counters()["JitLaunch"] == 1, "the generated reduction did not run"
+    assert ref.tobytes() == got.tobytes()
+
+
+@pytest.mark.parametrize("op", ["sum", "min", "max", "all", "any", "band", "bor"])
+def test_reduce_expression_generated_int_and_bool(gpu, data, op):
+    """int32 (i % j + scalar) and bool (i > j) reductions, an unaligned leaf
+    and a broadcast scalar leaf."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    ops = {"sum": capi.RED_SUM, "min": capi.RED_MIN, "max": capi.RED_MAX, "all": capi.RED_ALL,
+           "any": capi.RED_ANY, "band": capi.RED_BAND, "bor": capi.RED_BOR}
+    n = N - 5
+    shape = (n, 1, 1)
+    st = (1, n, n)
+    iu = _Leaf(cm, data["i"].arr[:n], offset=1)
+    sc = _Leaf(cm, np.array([7], np.int32), scalar=True)
+    code = [_instr(capi.OP_LOAD, capi.I32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.I32, capi.BIN_MOD, capi.SRC_LEAF, 1),
+            _instr(capi.OP_BINARY, capi.I32, capi.BIN_ADD, capi.SRC_LEAF, 2)]
+    leaves = [(iu, st), (data["j"], st), (sc, (0, 0, 0))]
+    ref = _rx(cm, leaves, code, capi.I32, ops[op], 7, shape, np.int32, jit=0)
+    cm.reset_counters()
+    got = _rx(cm, leaves, code, capi.I32, ops[op], 7, shape, np.int32, jit=2)
+    assert cm.counters()["JitLaunch"] == 1
+    assert ref.tobytes() == got.tobytes()
+    if op in ("sum", "all", "any"):
+        codeb = [_instr(capi.OP_LOAD, capi.I32, src=capi.SRC_LEAF, arg=0),
+                 _instr(capi.OP_BINARY, capi.I32, capi.BIN_GT, capi.SRC_LEAF, 1)]
+        refb = _rx(cm, leaves[:2], codeb, capi.BOOL, ops[op], 7, shape, np.bool_, jit=0)
+        gotb = _rx(cm, leaves[:2], codeb, capi.BOOL, ops[op], 7, shape, np.bool_, jit=2)
+        assert refb.tobytes() == gotb.tobytes()
+    if op == "sum":
+        i, j = data["i"].arr[:n].astype(np.int64), data["j"].arr[:n].astype(np.int64)
+        m = np.where(j != 0, np.fmod(i, np.where(j == 0, 1, j)), 0) + 7
+        exp = np.array([int(m.sum()) & 0xffffffff], np.uint32).view(np.int32)[0]  # int32 sums wrap
+        assert int(got[0]) == int(exp)
+
+
+def test_reduce_expression_non_flat_stays_on_interpreter(gpu, data):
+    """A row-major leaf reduced along rows walks a virtual order that is not
+    its memory order: the interpreter runs it (no generated kernel)."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    R, Cc = 1024, 1000
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0), _instr(capi.OP_UNARY, capi.F32, capi.UN_ABS)]
+    leaves = [(data["f"], (Cc, 1, R * Cc))]
+    cm.reset_counters()
+    got = _rx(cm, leaves, code, capi.F32, capi.RED_SUM, 1, (R, Cc, 1), np.float32, jit=2)
+    assert cm.counters()["JitLaunch"] == 0
+    a = np.abs(data["f"].arr[:R * Cc].reshape(R, Cc)).astype(np.float64).sum(axis=0)
+    np.testing.assert_allclose(got, a, rtol=1e-5)
