@@ -93,11 +93,20 @@ __global__ __launch_bounds__(RX_BS) void reduce_expr_kernel(const CwProgram P, c
     const long long beg = c * chunk;
     const long long end = min(SH.L, beg + chunk);
     T acc = RxIdentity<Op, T>::v();
-    for (long long j0 = beg + (long long)threadIdx.x * V; j0 < end; j0 += (long long)RX_BS * V) {
+    // element of slot k in the pass starting at p0: 32-bit FLAT reductions
+    // give a thread two groups of 4 that lie 256 elements apart (one 16-byte
+    // load of a wavefront then covers 1 KB contiguously); otherwise V
+    // consecutive elements
+    constexpr bool SPREAD = FLAT && !W;
+    const long long t0 = SPREAD ? (long long)(threadIdx.x >> 6) * 512 + (threadIdx.x & 63) * 4
+                                : (long long)threadIdx.x * V;
+    for (long long p0 = beg; p0 < end; p0 += (long long)RX_BS * V) {
+        const long long j0 = p0 + t0;
+        if (!SPREAD && j0 >= end) break;
         Bank<W, NL> lv;
 #pragma unroll
         for (int k = 0; k < V; ++k) {
-            const long long j = j0 + k;
+            const long long j = SPREAD ? j0 + (k >> 2) * 256 + (k & 3) : j0 + k;
             const bool ok = j < end;
             if constexpr (FLAT) {
                 const long long e = s * SH.L + (ok ? j : beg);
@@ -127,8 +136,10 @@ __global__ __launch_bounds__(RX_BS) void reduce_expr_kernel(const CwProgram P, c
         Reg<W> val;
         run_program<W, NL>(P, lv, val);
 #pragma unroll
-        for (int k = 0; k < V; ++k)
-            if (j0 + k < end) acc = Op::apply(acc, rx_get<T>(val, k));
+        for (int k = 0; k < V; ++k) {
+            const long long j = SPREAD ? j0 + (k >> 2) * 256 + (k & 3) : j0 + k;
+            if (j < end) acc = Op::apply(acc, rx_get<T>(val, k));
+        }
     }
     acc = block_reduce<Op, RX_BS>(acc, lds);
     if (threadIdx.x == 0) {
@@ -217,9 +228,9 @@ static std::string rx_identity_src()
 
 // Source and key of the generated FLAT reduction (32-bit programs): the
 // kernel of reduce_expr_kernel<Op, T, S, false, NL, true> with the program
-// inlined, 16-byte loads where a thread's 8 elements are whole and aligned,
-// the same per-thread fold order and the same block reduction
-// (fold_ops.hpp), so both give the same bits.
+// inlined, 16-byte loads where a thread's two groups of 4 elements are whole
+// and aligned, the same elements per thread, the same per-thread fold order
+// and the same block reduction (fold_ops.hpp), so both give the same bits.
 template <typename Op, typename T, typename S>
 int jit_reduce_flat_source(const CwProgram& P, std::string& key, std::string& body)
 {
@@ -251,11 +262,11 @@ int jit_reduce_flat_source(const CwProgram& P, std::string& key, std::string& bo
             if (al)
                 vload += "            { const u4v a = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[" + ls +
                          "] + e0)); const u4v b = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[" + ls +
-                         "] + e0 + 4));\n              v" + ls + "[0] = a.x; v" + ls + "[1] = a.y; v" + ls + "[2] = a.z; v" +
+                         "] + e0 + 256));\n              v" + ls + "[0] = a.x; v" + ls + "[1] = a.y; v" + ls + "[2] = a.z; v" +
                          ls + "[3] = a.w; v" + ls + "[4] = b.x; v" + ls + "[5] = b.y; v" + ls + "[6] = b.z; v" + ls +
                          "[7] = b.w; }\n";
             sload += "#pragma unroll\n            for (int k = 0; k < 8; ++k) v" + ls + "[k] = " + rd + ls +
-                     "])[s * A.L + (j0 + k < end ? j0 + k : beg)];\n";
+                     "])[s * A.L + (EL(j0, k) < end ? EL(j0, k) : beg)];\n";
         }
         namep[l] = names[l].c_str();
     }
@@ -275,6 +286,7 @@ int jit_reduce_flat_source(const CwProgram& P, std::string& key, std::string& bo
            " OP;\n"
            "struct JitRxArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) +
            "]; void* partials; void* out; long long L, chunk; int nchunk, pad; };\n"
+           "#define EL(j0, k) ((j0) + ((k) >> 2) * 256 + ((k) & 3))\n"
            "extern \"C\" __global__ __launch_bounds__(256) void cm_jit_reduce(const JitRxArgs A)\n{\n"
            "    typedef uint32_t R;\n"
            "    __shared__ TT lds[4];\n"
@@ -284,9 +296,11 @@ int jit_reduce_flat_source(const CwProgram& P, std::string& key, std::string& bo
            "    const long long end = min(A.L, beg + A.chunk);\n" +
            scalars +
            "    TT acc = " + rx_identity_src<Op, T>() + ";\n"
-           "    for (long long j0 = beg + (long long)threadIdx.x * 8; j0 < end; j0 += 256 * 8) {\n" + decls +
+           "    const long long t0 = (long long)(threadIdx.x >> 6) * 512 + (threadIdx.x & 63) * 4;\n"
+           "    for (long long p0 = beg; p0 < end; p0 += 256 * 8) {\n"
+           "        const long long j0 = p0 + t0;\n" + decls +
            "        const long long e0 = s * A.L + j0;\n" +
-           (all_al ? "        if (j0 + 8 <= end && (e0 & 3) == 0) {\n" + vload + "        } else {\n"
+           (all_al ? "        if (j0 + 259 < end && (e0 & 3) == 0) {\n" + vload + "        } else {\n"
                    : std::string("        {\n")) +
            sload +
            "        }\n"
@@ -297,7 +311,7 @@ int jit_reduce_flat_source(const CwProgram& P, std::string& key, std::string& bo
            "#define acc acc_r\n" +
            prog +
            "#undef acc\n"
-           "            if (j0 + k < end) acc = OP::apply(acc, (TT)" + get + "(acc_r));\n"
+           "            if (EL(j0, k) < end) acc = OP::apply(acc, (TT)" + get + "(acc_r));\n"
            "        }\n"
            "    }\n"
            "    acc = cm::block_reduce<OP, 256>(acc, lds);\n"
