@@ -115,27 +115,9 @@ def main():
     xd = cm.DeviceArray(n, dtype=np.float32, ctx=ctx)
     cg = cm.ConjugateGradient(A, path=args.cg_path).setTolerance(0.0)
 
-    # ------------------------------------------------------------ timed CG
-    if args.kernels_only:
-        args.steps = args.warmup = 3  # a few CG iterations, so every kernel is counted mid-solve
-    cg.setMaxIterations(args.warmup)
-    cg.solve(bd, xd, ctx=ctx)
-    ctx.sync()
-    cg.setMaxIterations(args.steps)
-    ctx.sync()
-    t0 = time.perf_counter()
-    cg.solve(bd, xd, ctx=ctx)
-    ctx.sync()
-    t1 = time.perf_counter()
-    assert cg.iterations() == args.steps, cg.iterations()
-    sec = t1 - t0
-    iters_per_s = args.steps / sec
-    ms_per_step = 1000.0 * sec / args.steps
-    cg_bytes = 8 * nnz + 56 * n
-    log(f"CG 2048^2: {iters_per_s:.1f} it/s, {ms_per_step * 1000:.1f} us/it, "
-        f"{cg_bytes / (ms_per_step * 1e-3) / 1e9:.0f} GB/s (fused-minimum bytes)")
-
     # ------------------------------------------------------------ SpMV roofline
+    # (measured first: the CG below then runs on a GPU that has been busy,
+    # as in continuous use, rather than straight out of idle)
     copies = []
     for k in range(4):
         Ak = cm.SparseMatrix.from_csr(n, n, JA, IA, V, ctx=ctx)
@@ -158,6 +140,28 @@ def main():
     spmv_bytes = 8 * nnz + 12 * n
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     log(f"SpMV 2048^2: {spmv_ms * 1000:.1f} us/launch, {spmv_gbs:.0f} GB/s ({spmv_gbs / HBM_PEAK_GBS:.1%})")
+    del copies
+
+    # ------------------------------------------------------------ timed CG
+    if args.kernels_only:
+        args.steps = args.warmup = 3  # a few CG iterations, so every kernel is counted mid-solve
+    cg.setMaxIterations(args.warmup)
+    cg.solve(bd, xd, ctx=ctx)
+    ctx.sync()
+    cg.setMaxIterations(args.steps)
+    ctx.sync()
+    t0 = time.perf_counter()
+    cg.solve(bd, xd, ctx=ctx)
+    ctx.sync()
+    t1 = time.perf_counter()
+    assert cg.iterations() == args.steps, cg.iterations()
+    sec = t1 - t0
+    iters_per_s = args.steps / sec
+    ms_per_step = 1000.0 * sec / args.steps
+    cg_bytes = 8 * nnz + 56 * n
+    log(f"CG 2048^2: {iters_per_s:.1f} it/s, {ms_per_step * 1000:.1f} us/it, "
+        f"{cg_bytes / (ms_per_step * 1e-3) / 1e9:.0f} GB/s (fused-minimum bytes)")
+
     if args.kernels_only:
         # calibration for the HBM byte counters: dst = src over 2^26 floats
         # (one cwise launch, 256 MiB read + 256 MiB written)
