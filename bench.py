@@ -14,8 +14,8 @@ Also reported (bounded, after the timed region): the linear combination of 8
 vectors of 2^26 floats, dot of 2^26, general cwise programs over 2^26,
 row/column/batch sums of 1024x1024x64, the 16384^2 dense product (both
 layouts), ELLPACK SpMV and cwise evaluation into CSR values on the 2048^2
-matrix, the 128^2 plain CG solve (iters/s, time to tol 1e-6) and small
-systems, the batched, mixed-precision and adjoint solvers, the reference's
+matrix, the block-scalar (3x3 blocks) SpMV and CG on a 96^3 grid, the 128^2
+plain CG solve (iters/s, time to tol 1e-6) and small systems, the batched, mixed-precision and adjoint solvers, the reference's
 published benchmark workloads, and the CPU baseline: the reference's
 vendored Eigen (oracle/_ref) on the host cores.
 
@@ -457,7 +457,69 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
         except Exception as ex:  # reported, the headline line is still printed
             out[name] = {"error": repr(ex)}
             log(name, "failed:", repr(ex))
+    out["block_cg_3x3_96^3"] = bench_block_cg(cm, ctx, stream, timer)
+    log("block_cg_3x3_96^3", out["block_cg_3x3_96^3"])
     return out
+
+
+def bench_block_cg(cm, ctx, stream, timer):
+    """Block-scalar path (the reference's SparseMatrix<real3x3> CG,
+    tests/TestBlockedConjugateGradient.cu): 3D 7-point block Poisson with
+    3x3 blocks (three float4 rows, 48 bytes each) on a 96^3 grid. Block SpMV
+    bytes = 52*nnz (blocks + column indices) + 4*(rows+1) + 32*rows (x, y);
+    one CG iteration moves those (with p for x and Ap for y) plus 160*rows
+    for the two vector kernels (float4 entries: r, Ap, M^-1 read and r
+    written; x, r, M^-1, p read and x, p written)."""
+    from cumat_amd.problems import block_poisson
+    g = 96
+    JA, IA, V = block_poisson((g, g, g), 3, seed=20261203)
+    rows, nnz = g ** 3, int(IA.size)
+    A = cm.BlockSparseMatrix.from_bcsr(rows, rows, JA, IA, V, ctx=ctx)
+    del V
+    rng = np.random.default_rng(5)
+    x = cm.DeviceArray.from_numpy(rng.uniform(-1, 1, 4 * rows).astype(np.float32), ctx=ctx)
+    y = cm.DeviceArray(4 * rows, dtype=np.float32, ctx=ctx)
+    for _ in range(3):
+        A.matvec(x, out=y, ctx=ctx)
+    ctx.sync()
+    group = []
+    for _ in range(10):
+        timer.start(stream)
+        for _ in range(5):
+            A.matvec(x, out=y, ctx=ctx)
+        group.append(timer.stop_ms(stream) / 5)
+    ms = float(np.median(group))
+    sp_bytes = 52 * nnz + 4 * (rows + 1) + 32 * rows
+    res = {"grid": f"{g}^3", "block_rows": rows, "blocks": nnz,
+           "spmv_us": round(ms * 1000, 1), "spmv_GBps": round(sp_bytes / (ms * 1e-3) / 1e9, 1),
+           "spmv_frac": round(sp_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    b = cm.DeviceArray.from_numpy(rng.uniform(-1, 1, 4 * rows).astype(np.float32), ctx=ctx)
+    cg = cm.ConjugateGradient(A).setTolerance(0.0)
+    cg.setMaxIterations(5)
+    cg.solve(b, y, ctx=ctx)
+    ctx.sync()
+    K = 50
+    cg.setMaxIterations(K)
+    t0 = time.perf_counter()
+    cg.solve(b, y, ctx=ctx)
+    ctx.sync()
+    sec = time.perf_counter() - t0
+    it_bytes = sp_bytes + 160 * rows
+    res.update({"cg_iterations": cg.iterations(), "cg_us_per_iteration": round(sec / K * 1e6, 1),
+                "cg_iters_per_s": round(K / sec, 1),
+                "cg_GBps": round(it_bytes / (sec / K) / 1e9, 1),
+                "cg_frac": round(it_bytes / (sec / K) / 1e9 / HBM_PEAK_GBS, 4)})
+    # convergence at the reference test's tolerance, against the FP64 residual
+    cg.setTolerance(1e-5).setMaxIterations(4000)
+    t0 = time.perf_counter()
+    xs = cg.solve(b, ctx=ctx)
+    ctx.sync()
+    res.update({"solve_tol1e-5_ms": round((time.perf_counter() - t0) * 1000, 2),
+                "solve_iterations": cg.iterations(), "solve_error": cg.error()})
+    ax = A.matvec(xs, ctx=ctx).numpy().astype(np.float64).reshape(rows, 4)[:, :3]
+    bh = b.numpy().astype(np.float64).reshape(rows, 4)[:, :3]
+    res["true_rel_residual_fp64"] = float(np.linalg.norm(bh - ax) / np.linalg.norm(bh))
+    return res
 
 
 def bench_config1(cm, capi, ctx):

@@ -18,7 +18,8 @@ from ._capi import CumatArgError, CumatError  # noqa: F401
 from .device import DeviceArray, Context, default_context, synchronize  # noqa: F401
 from .expr import (Expr, as_expr, assign, cast, select, where_scalar,  # noqa: F401
                    lower, evaluate)
-from .sparse import SparseMatrix, SparsityPattern, ELLMatrix, SparseExpressionView, sparse_view  # noqa: F401
+from .sparse import (SparseMatrix, SparsityPattern, ELLMatrix, BlockSparseMatrix, SparseExpressionView,  # noqa: F401
+                     sparse_view)
 from .reductions import reduce, dot, squared_norm, norm  # noqa: F401
 from .solvers import ConjugateGradient, DiagonalPreconditioner, pcg_batched, pcg_ir  # noqa: F401
 from . import dense  # noqa: F401

@@ -106,6 +106,10 @@ _PROTOS = {
     "cm_ell_expr_spmv": ([_VP, C.POINTER(Leaf), _I32, C.POINTER(Instr), _I32, _I32, _I32, _I32, _I32,
                           _I32, _VP, _VP, _I32, _VP, _I32], _I32),
     "cm_csr_inverse_diagonal": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP], _I32),
+    "cm_bcsr_spmv": ([_VP, _I32, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP, _I32, _VP, _I32], _I32),
+    "cm_bcsr_inverse_diagonal": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP], _I32),
+    "cm_pcg_bcsr_solve": ([_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP, _I32, _VP, _VP, _VP, _I32, _F64, _I32,
+                           C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(CgStats)], _I32),
     "cm_dense_gemv": ([_VP, _I32, _I32, _I32, _I32, _VP, _I32, _I32, _VP, _I32, _VP, _I32], _I32),
     "cm_dense_inverse_diagonal": ([_VP, _I32, _I32, _I32, _VP, _I32, _VP], _I32),
     "cm_pcg_dense_solve": ([_VP, _I32, _I32, _I32, _VP, _I32, _I32, _VP, _VP, _VP, _I32, _F64, _I32,

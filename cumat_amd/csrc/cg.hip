@@ -904,6 +904,67 @@ int cm_pcg_dense_solve(cm_ctx_t ctx, int dtype, int n, int batches, const void* 
     CM_API_END
 }
 
+int cm_pcg_bcsr_solve(cm_ctx_t ctx, int block_dim, int rows, int nnz, int batches, const int* JA, const int* IA,
+                      const float* vals, int vals_batched, const float* invdiag, const float* b, float* x,
+                      int use_guess, double tol, int max_iter, int* iterations_out, double* error_out,
+                      cm_cg_stats* stats)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(block_dim >= 2 && block_dim <= 4, CM_E_UNSUPPORTED, "cm_pcg_bcsr_solve: block dimension %d (2..4)",
+               block_dim);
+    CM_REQUIRE(rows >= 0 && nnz >= 0 && batches >= 1 && batches <= cm::CG_MAXB, CM_E_DIM,
+               "cm_pcg_bcsr_solve: bad sizes (batches 1..%d)", cm::CG_MAXB);
+    CM_REQUIRE(rows <= (1 << 29), CM_E_DIM, "cm_pcg_bcsr_solve: %d block rows (max 2^29)", rows);
+    CM_REQUIRE(tol >= 0, CM_E_ARG, "cm_pcg_bcsr_solve: negative tolerance");
+    if (max_iter < 0) max_iter = 2 * rows;  // IterativeSolverBase::maxIterations: 2 * cols (in blocks)
+    if (rows == 0) {
+        if (iterations_out) *iterations_out = 0;
+        if (error_out) *error_out = 0;
+        return CM_OK;
+    }
+    CM_REQUIRE(JA && b && x && (nnz == 0 || (IA && vals)), CM_E_ARG, "cm_pcg_bcsr_solve: NULL pointer");
+    CM_REQUIRE((((uintptr_t)vals | (uintptr_t)invdiag | (uintptr_t)b | (uintptr_t)x) & 15) == 0, CM_E_ARG,
+               "cm_pcg_bcsr_solve: blocks and vectors must be 16-byte aligned (float4 entries)");
+    cm::Temps tmp(ctx);
+    const long long N = (long long)rows * batches;  // float4 entries of b and x
+    const int nbi = vals_batched ? batches : 1;
+    // the right-hand side with components >= D cleared (the reference's
+    // squaredNorm and Jacobi entries never see them), then the Jacobi entries
+    float* bz;
+    CM_TRY(tmp.get(16 * (size_t)N + (invdiag ? 0 : 16 * (size_t)rows * nbi), (void**)&bz));
+    const dim3 mg(cm::div_up(N, 256));
+    if (block_dim == 2)
+        cm::bcsr_mask_kernel<2><<<mg, 256, 0, ctx->stream>>>(N, (const cm::bc_f4*)b, (cm::bc_f4*)bz);
+    else if (block_dim == 3)
+        cm::bcsr_mask_kernel<3><<<mg, 256, 0, ctx->stream>>>(N, (const cm::bc_f4*)b, (cm::bc_f4*)bz);
+    else
+        cm::bcsr_mask_kernel<4><<<mg, 256, 0, ctx->stream>>>(N, (const cm::bc_f4*)b, (cm::bc_f4*)bz);
+    CM_LAUNCH_CHECK();
+    const long long vstride = vals_batched ? (long long)nnz * block_dim : 0;
+    if (!invdiag) {
+        float* id = bz + 4 * N;
+        cm::launch_bcsr_invdiag(ctx->stream, block_dim, rows, nbi, JA, IA, (const cm::bc_f4*)vals, vstride,
+                                (cm::bc_f4*)id);
+        CM_LAUNCH_CHECK();
+        invdiag = id;
+    }
+    cm::BlockOp op;
+    op.JA = JA;
+    op.IA = IA;
+    op.vals = (const cm::bc_f4*)vals;
+    op.vstride = vstride;
+    op.rows = rows;
+    op.D = block_dim;
+    cm::CgBatchOpts opt;
+    opt.vals_batched = vals_batched;  // the Jacobi entries then have one row of 4*rows per batch
+    opt.block = &op;
+    return cm::pcg_multikernel<float, float>(ctx, 4 * rows, 0, batches, nullptr, nullptr, nullptr, invdiag, bz, x,
+                                             use_guess, tol, max_iter, iterations_out, error_out, stats, nullptr,
+                                             opt);
+    CM_API_END
+}
+
 int cm_pcg_solve_ex(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int* JA, const int* IA,
                     const void* vals, int vals_batched, const void* invdiag, const void* b, void* x, int use_guess,
                     double tol, int max_iter, int per_batch, int* iterations_each, double* error_each,

@@ -54,6 +54,7 @@
 #include <map>
 #include <type_traits>
 
+#include "bcsr.hpp"
 #include "dense.hpp"
 #include "spmv_core.hpp"
 #include "spmv_wave.hpp"
@@ -551,6 +552,43 @@ __global__ __launch_bounds__(CG_BS) void cg_dense_kernel(int n, const T* __restr
     if (threadIdx.x == 0) part[(long long)bb * gridDim.x + blockIdx.x] = s;
 }
 
+// K1 for a block-CSR operator of D x D blocks (bcsr.hpp): the vectors of
+// the solve hold one float4 per block row (n = 4 * rows floats per batch);
+// tmp = A p and the per-block partial p.tmp, same grid rule, partial layout
+// and iteration counter as cg_spmv_kernel
+template <int D>
+__global__ __launch_bounds__(CG_BS) void cg_bcsr_kernel(int rows, const int* __restrict__ JA,
+                                                        const int* __restrict__ IA, const bc_f4* __restrict__ vals,
+                                                        long long vstride, const float* __restrict__ p,
+                                                        float* __restrict__ tmp, float* __restrict__ part,
+                                                        CgState<float>* st)
+{
+    const int bb = blockIdx.y;
+    cg_k1_snapshot(st, bb);
+    if (batch_inactive(st, bb)) return;
+    constexpr int WPB = CG_BS / 64;
+    __shared__ bc_f4 lds[WPB][BC_WIN];
+    __shared__ float red[WPB];
+    bc_f4* slice = lds[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const long long off = (long long)bb * rows;
+    const bc_f4* pb = reinterpret_cast<const bc_f4*>(p) + off;
+    bc_f4* tb = reinterpret_cast<bc_f4*>(tmp) + off;
+    const bc_f4* vb = vals + bb * vstride;
+    float s = 0.0f;
+    for (ChunkWalk w(div_up_dev(rows, 64), WPB); w.valid(); w.next()) {
+        const int r0 = w.cur * 64, nr = min(64, rows - r0);
+        const bc_f4 ap = bcsr_chunk_product<D, false>(JA, IA, vb, r0, nr, [pb](int c) { return pb[c]; }, slice);
+        if (lane < nr) {
+            const bc_f4 pv = pb[r0 + lane];
+            tb[r0 + lane] = ap;
+            s += pv.x * ap.x + pv.y * ap.y + pv.z * ap.z + pv.w * ap.w;
+        }
+    }
+    s = block_reduce<OpSum, CG_BS>(s, red);
+    if (threadIdx.x == 0) part[(long long)bb * gridDim.x + blockIdx.x] = s;
+}
+
 // p_{i} = M^-1 r_i + beta p_{i-1} with one explicit rounding pattern, so that
 // the value a lane writes for its own row and the value other lanes
 // recompute for that column are bit-identical
@@ -1000,13 +1038,37 @@ struct DenseOp {
     bool row_major = false;
 };
 
+// a block-CSR operator (FP32 blocks of D float4 rows, bcsr.hpp)
+struct BlockOp {
+    const int* JA = nullptr;
+    const int* IA = nullptr;
+    const bc_f4* vals = nullptr;
+    long long vstride = 0;  // float4 entries between the batches' values (0: shared)
+    int rows = 0;           // block rows; the solve's vectors hold 4 * rows floats per batch
+    int D = 3;
+};
+
+inline void launch_cg_bcsr(hipStream_t stream, const BlockOp& op, int B, int G1, const float* p, float* t,
+                           float* part1, CgState<float>* st)
+{
+    const dim3 grid(G1, B);
+    if (op.D == 2)
+        cg_bcsr_kernel<2><<<grid, CG_BS, 0, stream>>>(op.rows, op.JA, op.IA, op.vals, op.vstride, p, t, part1, st);
+    else if (op.D == 3)
+        cg_bcsr_kernel<3><<<grid, CG_BS, 0, stream>>>(op.rows, op.JA, op.IA, op.vals, op.vstride, p, t, part1, st);
+    else
+        cg_bcsr_kernel<4><<<grid, CG_BS, 0, stream>>>(op.rows, op.JA, op.IA, op.vals, op.vstride, p, t, part1, st);
+}
+
 template <typename VT, typename T>
 static void launch_cg_iteration(cm_ctx_t ctx, int n, int B, int G1, int G2, const int* JA, const int* IA,
                                 const VT* vals, long long vstride, const T* invd, int istride, T* p, T* t, T* x, T* r,
                                 T* part1, T* part2, CgState<T>* st, const T* diag, const DenseOp<T>* dense,
-                                const T* r0, CgMirror* mirror, int x0_zero)
+                                const T* r0, CgMirror* mirror, int x0_zero, const BlockOp* block = nullptr)
 {
-    if (dense && dense->row_major)
+    if (block) {
+        if constexpr (std::is_same<T, float>::value) launch_cg_bcsr(ctx->stream, *block, B, G1, p, t, part1, st);
+    } else if (dense && dense->row_major)
         cg_dense_kernel<T, true><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, dense->A, dense->bstride, p, t, part1, st);
     else if (dense)
         cg_dense_kernel<T, false><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, dense->A, dense->bstride, p, t, part1,
@@ -1045,6 +1107,7 @@ struct CgBatchOpts {
     // recomputed from r, M^-1, p_prev) instead of three
     bool fused = false;
     const void* dense = nullptr;  // DenseOp<T>: a dense operator replaces the CSR matrix
+    const BlockOp* block = nullptr;  // a block-CSR operator replaces the CSR matrix (FP32)
 };
 
 template <typename VT, typename T>
@@ -1058,6 +1121,9 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     const DenseOp<T>* dense = (const DenseOp<T>*)opt.dense;
     CM_REQUIRE(!dense || (invd && !opt.fused), CM_E_ARG,
                "CG: a dense operator needs an inverse diagonal and the three-launch form");
+    const BlockOp* block = opt.block;
+    CM_REQUIRE(!block || (invd && !opt.fused && !dense && std::is_same<T, float>::value && n == 4 * block->rows),
+               CM_E_ARG, "CG: a block operator needs FP32, an inverse diagonal and the three-launch form");
     CM_REQUIRE(B >= 1 && B <= CG_MAXB, CM_E_UNSUPPORTED, "CG: %d right-hand sides (max %d)", B, CG_MAXB);
     Temps tmp(ctx);
     const long long N = (long long)n * B;
@@ -1066,7 +1132,8 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
     // K1 walks 64-row chunks with 4 wavefronts per block; the setup walks
     // 512-row blocks with the same grid.
     const int gcap = std::max(8, CG_GMAX / B);
-    const int G1 = walk_grid(div_up(div_up(n, 64), CG_BS / 64), gcap);
+    // (a block operator walks 64-block-row chunks of its rows)
+    const int G1 = walk_grid(div_up(div_up(block ? block->rows : n, 64), CG_BS / 64), gcap);
     const int G2 = walk_grid(div_up(n, (long long)CG_BS * CG_VPT), std::min(gcap, CG_G2MAX));
     // reduce_pairs_batched folds at most CG_PPT + 1 pairs per thread
     CM_REQUIRE(B == 1 || G2 <= (CG_PPT + 1) * (CG_BS / B), CM_E_UNSUPPORTED, "CG: grid %d too large for %d batches",
@@ -1145,6 +1212,15 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
             ax_pre = t;
             launches += 1;
         }
+        if (block) {
+            // t = A x_0 (float4 entries), read by the setup kernel
+            if constexpr (std::is_same<T, float>::value)
+                launch_bcsr_spmv(ctx->stream, block->D, block->rows, B, block->JA, block->IA, block->vals,
+                                 block->vstride, (const bc_f4*)x, block->rows, (bc_f4*)t, CM_ASSIGN);
+            CM_LAUNCH_CHECK();
+            ax_pre = t;
+            launches += 1;
+        }
         cg_setup_kernel<VT, T><<<dim3(G1, B), CG_BS, 0, ctx->stream>>>(n, JA, IA, vals, vstride, invd, istride, b,
                                                                        x, r, p, 1, part1, diag, ax_pre);
     } else {
@@ -1180,7 +1256,7 @@ static int pcg_multikernel(cm_ctx_t ctx, int n, int nnz, int B, const int* JA, c
         for (int k = 0; k < todo; ++k, ++it) {
             if (!fused) {
                 launch_cg_iteration<VT, T>(ctx, n, B, G1, G2, JA, IA, vals, vstride, invd, istride, p, t, x, r, part1,
-                                           part2, st, diag, dense, r0, mirror, x0_zero);
+                                           part2, st, diag, dense, r0, mirror, x0_zero, block);
                 continue;
             }
             T* pcur = pbuf[it & 1];

@@ -6,6 +6,7 @@
 //            (cuMat/src/SparseProductEvaluation.h:194-324)
 //   Jacobi:  replaces ExtractDiagonalOp over SparseMatrix::coeff +
 //            cwiseInverseCheck (cuMat/src/ConjugateGradient.h:228-230)
+#include "bcsr.hpp"
 #include "spmv_core.hpp"
 #include "spmv_wave.hpp"
 
@@ -242,6 +243,53 @@ int cm_csr_inverse_diagonal(cm_ctx_t ctx, int dtype, int rows, int nnz, int batc
             rows, nnz, JA, IA, (const double*)vals, vals_batched, (double*)invdiag);
     else
         return cm::fail(CM_E_UNSUPPORTED, "cm_csr_inverse_diagonal: unsupported dtype %d", dtype);
+    CM_LAUNCH_CHECK();
+    return CM_OK;
+    CM_API_END
+}
+
+int cm_bcsr_spmv(cm_ctx_t ctx, int block_dim, int rows, int cols, int nnz, int batches, const int* JA,
+                 const int* IA, const float* vals, int vals_batched, const float* x, int x_batched, float* y,
+                 int mode)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(block_dim >= 2 && block_dim <= 4, CM_E_UNSUPPORTED, "cm_bcsr_spmv: block dimension %d (2..4)",
+               block_dim);
+    CM_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && batches >= 1, CM_E_DIM,
+               "cm_bcsr_spmv: bad sizes rows=%d cols=%d nnz=%d batches=%d", rows, cols, nnz, batches);
+    CM_REQUIRE(mode >= CM_ASSIGN && mode <= CM_DIV, CM_E_ARG, "cm_bcsr_spmv: bad mode %d", mode);
+    cm::count(CM_CNT_EVAL_MATMUL_SPARSE);
+    cm::count(CM_CNT_EVAL_ANY);
+    if (rows == 0) return CM_OK;
+    CM_REQUIRE(JA && y && (nnz == 0 || (IA && vals && x)), CM_E_ARG, "cm_bcsr_spmv: NULL pointer");
+    CM_REQUIRE((((uintptr_t)vals | (uintptr_t)x | (uintptr_t)y) & 15) == 0, CM_E_ARG,
+               "cm_bcsr_spmv: blocks and vectors must be 16-byte aligned (float4 entries)");
+    cm::launch_bcsr_spmv(ctx->stream, block_dim, rows, batches, JA, IA, (const cm::bc_f4*)vals,
+                         vals_batched ? (long long)nnz * block_dim : 0, (const cm::bc_f4*)x,
+                         x_batched ? (long long)cols : 0, (cm::bc_f4*)y, mode);
+    CM_LAUNCH_CHECK();
+    return CM_OK;
+    CM_API_END
+}
+
+int cm_bcsr_inverse_diagonal(cm_ctx_t ctx, int block_dim, int rows, int nnz, int batches, const int* JA,
+                             const int* IA, const float* vals, int vals_batched, float* invdiag)
+{
+    CM_API_BEGIN
+    CM_TRY(cm::resolve(ctx, &ctx));
+    CM_REQUIRE(block_dim >= 2 && block_dim <= 4, CM_E_UNSUPPORTED,
+               "cm_bcsr_inverse_diagonal: block dimension %d (2..4)", block_dim);
+    CM_REQUIRE(rows >= 0 && nnz >= 0 && batches >= 1, CM_E_DIM, "cm_bcsr_inverse_diagonal: bad sizes");
+    cm::count(CM_CNT_EVAL_CWISE);
+    cm::count(CM_CNT_EVAL_ANY);
+    if (rows == 0) return CM_OK;
+    CM_REQUIRE(JA && invdiag && (nnz == 0 || (IA && vals)), CM_E_ARG, "cm_bcsr_inverse_diagonal: NULL pointer");
+    CM_REQUIRE((((uintptr_t)vals | (uintptr_t)invdiag) & 15) == 0, CM_E_ARG,
+               "cm_bcsr_inverse_diagonal: blocks and entries must be 16-byte aligned");
+    cm::launch_bcsr_invdiag(ctx->stream, block_dim, rows, vals_batched ? batches : 1, JA, IA,
+                            (const cm::bc_f4*)vals, vals_batched ? (long long)nnz * block_dim : 0,
+                            (cm::bc_f4*)invdiag);
     CM_LAUNCH_CHECK();
     return CM_OK;
     CM_API_END

@@ -10,6 +10,10 @@
   endpoints and the CSR-entry -> edge gather map used by the adjoint kernels.
 * batched_poisson: block-diagonal concatenation of uneven shifted Poisson
   systems (persistent batched CG workload).
+* block_poisson: a block-CSR matrix of D x D blocks (D float4 rows each), the
+  Kronecker product of a 2D or 3D Poisson pattern with an SPD coupling block
+  plus a random SPD perturbation of the diagonal blocks (block-scalar CG
+  workload).
 
 All random inputs come from numpy's PCG64 generator seeded explicitly.
 """
@@ -163,3 +167,45 @@ def batched_poisson(nsys: int, side_lo: int, side_hi: int, seed: int, dtype=np.f
     b = rng.uniform(-1.0, 1.0, int(row_off[-1])).astype(dtype)
     return {"JA": JA, "IA": IA, "vals": V, "row_off": row_off.astype(np.int32), "b": b,
             "sides": sides, "sigma": sig}
+
+
+def block_poisson(dims, D: int = 3, seed: int = 0, coupling: float = 0.5):
+    """Block-CSR (JA, IA, vals) over a 2D or 3D grid (C order, Dirichlet):
+    block (i, j) = L_ij * K for the 5- / 7-point Laplacian L and the SPD
+    D x D block K = I + coupling * (tridiagonal -1/2, 1, -1/2), plus on each
+    diagonal block a seeded positive semi-definite term (0.1/D) R R^T with R
+    uniform in [-1, 1], so the matrix stays SPD. vals is float32 (nnz, D, 4):
+    D float4 rows, w = 0."""
+    dims = tuple(int(d) for d in dims)
+    n = int(np.prod(dims))
+    idx = np.arange(n, dtype=np.int64).reshape(dims)
+    coords = np.unravel_index(np.arange(n), dims)
+    strides = [int(np.prod(dims[a + 1:])) for a in range(len(dims))]
+    offs, valid = [], []
+    for a in reversed(range(len(dims))):  # most distant neighbour first: ascending columns
+        offs.append(-strides[a])
+        valid.append(coords[a] > 0)
+    offs.append(0)
+    valid.append(np.ones(n, bool))
+    for a in range(len(dims)):
+        offs.append(strides[a])
+        valid.append(coords[a] < dims[a] - 1)
+    order = np.argsort(offs, kind="stable")
+    offs = [offs[k] for k in order]
+    valid = np.stack([valid[k] for k in order], axis=1)
+    row = idx.reshape(-1)
+    cols = row[:, None] + np.array(offs)[None, :]
+    JA = np.zeros(n + 1, np.int32)
+    JA[1:] = np.cumsum(valid.sum(1))
+    IA = cols[valid].astype(np.int32)
+    K = np.eye(D) + coupling * (np.eye(D) - 0.5 * np.eye(D, k=1) - 0.5 * np.eye(D, k=-1))
+    lap = np.where(np.array(offs) == 0, 2.0 * len(dims), -1.0)
+    scal = np.broadcast_to(lap, valid.shape)[valid]
+    vals = np.zeros((IA.size, D, 4), np.float64)
+    vals[:, :, :D] = scal[:, None, None] * K[None, :, :]
+    rng = np.random.default_rng(seed)
+    diag_pos = np.nonzero(IA == np.repeat(row, valid.sum(1)))[0]
+    R = rng.uniform(-1.0, 1.0, (n, D, D))
+    vals[diag_pos, :, :D] += (0.1 / D) * (R @ np.transpose(R, (0, 2, 1)))  # PSD: the sum stays SPD
+    return JA, IA, vals.astype(np.float32)
+

@@ -12,7 +12,7 @@ import numpy as np
 from . import _capi as capi
 from . import dense
 from .device import DeviceArray, ctx_handle
-from .sparse import SparseMatrix
+from .sparse import BlockSparseMatrix, SparseMatrix
 
 
 def _scalar(matrix):
@@ -45,8 +45,11 @@ class ConjugateGradient:
     tolerance defaults to the scalar epsilon, maxIterations to 2*cols,
     error() = sqrt(max_b |r|^2/|b|^2), iterations() excludes the converging
     step, and all right-hand-side batches iterate until all converge.
-    The matrix is a CSR SparseMatrix or a dense square DeviceArray
-    (ConjugateGradient<MatrixXf>, with the dense product A * p)."""
+    The matrix is a CSR SparseMatrix, a dense square DeviceArray
+    (ConjugateGradient<MatrixXf>, with the dense product A * p) or a
+    BlockSparseMatrix (ConjugateGradient<SparseMatrix<real3x3>>, the
+    reference's block-scalar CG, tests/TestBlockedConjugateGradient.cu:244-275;
+    vectors of one float4 per block row)."""
 
     def __init__(self, matrix: SparseMatrix | None = None, preconditioner=None, path: int = capi.CG_AUTO):
         self._tol = None
@@ -103,6 +106,8 @@ class ConjugateGradient:
 
     def _solve(self, b, x, use_guess, ctx):
         A = self.matrix
+        if isinstance(A, BlockSparseMatrix):
+            return self._solve_block(b, x, use_guess, ctx)
         if b.rows != A.cols or b.cols != 1:
             raise capi.CumatArgError(capi.CM_E_DIM, "rhs has the wrong size")
         if x is None:
@@ -123,6 +128,26 @@ class ConjugateGradient:
                   A.values.ptr, self.preconditioner.entries.ptr, b.ptr, x.ptr, int(use_guess),
                   float(self.tolerance()), int(self.maxIterations()), int(self.path), C.byref(it),
                   C.byref(err), C.byref(st))
+        self._iters, self._error = it.value, err.value
+        self.last_stats = {"path": st.path_used, "launches": st.launches, "seconds": st.seconds}
+        return x
+
+
+    def _solve_block(self, b, x, use_guess, ctx):
+        A = self.matrix
+        if b.dtype != capi.F32 or b.rows != 4 * A.cols or b.cols != 1:
+            raise capi.CumatArgError(capi.CM_E_DIM, "rhs must be float32 with one float4 per block row")
+        if x is None:
+            x = DeviceArray(b.rows, 1, b.batches, dtype=np.float32, ctx=b.ctx)
+        if x.dtype != capi.F32 or x.rows != b.rows or x.batches != b.batches:
+            raise capi.CumatArgError(capi.CM_E_DIM, "solution has the wrong size")
+        it = C.c_int(0)
+        err = C.c_double(0)
+        st = capi.CgStats()
+        p = A.pattern
+        capi.call("cm_pcg_bcsr_solve", ctx_handle(ctx), A.block_dim, p.rows, p.nnz, b.batches, p.JA.ptr,
+                  p.IA.ptr, A.values.ptr, 0, self.preconditioner.entries.ptr, b.ptr, x.ptr, int(use_guess),
+                  float(self.tolerance()), int(self.maxIterations()), C.byref(it), C.byref(err), C.byref(st))
         self._iters, self._error = it.value, err.value
         self.last_stats = {"path": st.path_used, "launches": st.launches, "seconds": st.seconds}
         return x

@@ -1,6 +1,7 @@
 """Sparse matrices: CSR (SparsityPattern<CSR> + SparseMatrix<Scalar,Batches,CSR>,
-cuMat/src/SparseMatrixBase.h:89-147, cuMat/src/SparseMatrix.h:48-414) and
-ELLPACK (cuMat/src/SparseMatrixBase.h:149-206)."""
+cuMat/src/SparseMatrixBase.h:89-147, cuMat/src/SparseMatrix.h:48-414),
+ELLPACK (cuMat/src/SparseMatrixBase.h:149-206) and block-CSR over block
+scalars (SparseMatrix<real3x3>, tests/TestBlockedConjugateGradient.cu:12-239)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -342,3 +343,87 @@ class ELLMatrix:
 
     def sparse_view(self, pattern=None):
         return SparseExpressionView(self._leaf(), self if pattern is None else pattern)
+
+
+class BlockSparseMatrix:
+    """CSR matrix of block scalars: the reference's SparseMatrix<real3x3>
+    with its plug-in types (tests/TestBlockedConjugateGradient.cu:12-239).
+    A block is block_dim (2..4) float4 rows (real3x3: three rows, w = 0);
+    rows, cols and nnz count blocks. Values are (nnz, D, 4) float32, or
+    (batches, nnz, D, 4) for a batched matrix. A vector is a float32
+    DeviceArray of 4 * rows entries (one float4 per block row), batched like
+    any DeviceArray."""
+
+    def __init__(self, pattern: SparsityPattern, block_dim: int, values):
+        D = int(block_dim)
+        if not 2 <= D <= 4:
+            raise capi.CumatArgError(capi.CM_E_ARG, f"block dimension {D} (2..4)")
+        v = np.asarray(values, np.float32)
+        if v.ndim == 3:
+            v = v[None]
+        if v.ndim != 4 or v.shape[1:] != (pattern.nnz, D, 4):
+            raise capi.CumatArgError(capi.CM_E_DIM, f"block values must be (nnz, {D}, 4) or "
+                                     f"(batches, nnz, {D}, 4), got {np.shape(values)}")
+        self.pattern = pattern
+        self.block_dim = D
+        self.batches = int(v.shape[0])
+        per = max(pattern.nnz * D * 4, 4)
+        flat = np.zeros(per * self.batches, np.float32)
+        for k in range(self.batches):
+            flat[k * per:k * per + v[k].size] = v[k].reshape(-1)
+        self.values = DeviceArray.from_numpy(flat, rows=per, cols=1, batches=self.batches, ctx=pattern.ctx)
+        self.dtype = self.values.dtype
+
+    @classmethod
+    def from_bcsr(cls, rows, cols, JA, IA, vals, ctx=None):
+        v = np.asarray(vals, np.float32)
+        pat = SparsityPattern(rows, cols, JA, IA, ctx=ctx)
+        return cls(pat, v.shape[-2], v)
+
+    @property
+    def rows(self):
+        return self.pattern.rows
+
+    @property
+    def cols(self):
+        return self.pattern.cols
+
+    @property
+    def nnz(self):
+        return self.pattern.nnz
+
+    def _check_vec(self, v: DeviceArray, entries: int, what: str):
+        if v.dtype != capi.F32:
+            raise capi.CumatArgError(capi.CM_E_ARG, f"block SpMV: {what} must be float32")
+        if v.rows != 4 * entries or v.cols != 1:
+            raise capi.CumatArgError(capi.CM_E_DIM, f"block SpMV: {what} is {v.rows}x{v.cols}, "
+                                     f"need {4 * entries}x1 (one float4 per block)")
+
+    def matvec(self, x: DeviceArray, out: DeviceArray | None = None, mode: int = capi.ASSIGN, ctx=None):
+        """y (mode)= A * x with the block product functor, rows summed in CSR
+        order (cuMat/src/SparseProductEvaluation.h:43-139)."""
+        self._check_vec(x, self.cols, "x")
+        batches = max(self.batches, x.batches)
+        if self.batches not in (1, batches) or x.batches not in (1, batches):
+            raise capi.CumatArgError(capi.CM_E_DIM, "block SpMV: batch counts do not broadcast")
+        if out is None:
+            out = DeviceArray(4 * self.rows, 1, batches, dtype=np.float32, ctx=x.ctx)
+        self._check_vec(out, self.rows, "the output")
+        if out.batches != batches:
+            raise capi.CumatArgError(capi.CM_E_DIM, "block SpMV: output has the wrong batch count")
+        p = self.pattern
+        capi.call("cm_bcsr_spmv", ctx_handle(ctx), self.block_dim, p.rows, p.cols, p.nnz, batches, p.JA.ptr,
+                  p.IA.ptr, self.values.ptr, int(self.batches > 1), x.ptr, int(x.batches > 1), out.ptr, mode)
+        return out
+
+    __matmul__ = matmul = matvec
+
+    def inverse_diagonal(self, ctx=None) -> DeviceArray:
+        """A.diagonal().cwiseInverseCheck() over block scalars: one float4 per
+        block row (components >= block_dim are 0)."""
+        p = self.pattern
+        out = DeviceArray(4 * p.rows, 1, self.batches, dtype=np.float32, ctx=ctx or p.ctx)
+        capi.call("cm_bcsr_inverse_diagonal", ctx_handle(ctx or p.ctx), self.block_dim, p.rows, p.nnz,
+                  self.batches, p.JA.ptr, p.IA.ptr, self.values.ptr, int(self.batches > 1), out.ptr)
+        return out
+

@@ -313,6 +313,33 @@ int cm_ell_spmv(cm_ctx_t ctx, int dtype, int rows, int cols, int nnz_per_row,
                 int vals_batched, const void* x, int x_batched, void* y,
                 int mode);
 
+/* Block-CSR SpMV over block scalars: ProductAssignment of
+ * SparseMatrix<real3x3> * Matrix<real3> (cuMat/src/SparseProductEvaluation.h:
+ * 43-139) with the block plug-ins of tests/TestBlockedConjugateGradient.cu
+ * (ProductElementFunctor :144-157, dot :17-20, types :12-64). A block is
+ * block_dim (2..4) float4 rows (16 * block_dim bytes; real3x3 is three),
+ * a vector entry one float4; rows, cols and nnz count blocks. For block
+ * row i: y_i (mode)= sum over k of A_k x_{IA[k]} in CSR order, component q
+ * of A_k x_j = a_q.x*x.x + a_q.y*x.y + a_q.z*x.z + a_q.w*x.w (not fused),
+ * components >= block_dim = 0. Batches as cm_csr_spmv (value stride nnz
+ * blocks, vector stride cols entries); mode CM_ASSIGN .. CM_DIV per
+ * component; an empty block row assigns 0. vals, x and y 16-byte aligned. */
+int cm_bcsr_spmv(cm_ctx_t ctx, int block_dim, int rows, int cols, int nnz,
+                 int batches, const int* JA, const int* IA,
+                 const float* vals, int vals_batched,
+                 const float* x, int x_batched, float* y, int mode);
+
+/* Jacobi entries of a block matrix, A.diagonal().cwiseInverseCheck() with
+ * the block diagonal extraction and checked inverse
+ * (cuMat/src/ConjugateGradient.h:218-246; tests/TestBlockedConjugateGradient.cu:
+ * 174-210): one float4 per block row, component q < block_dim = 1/d_q of the
+ * row's diagonal block (1 where d_q == 0 or no diagonal block is stored),
+ * the rest 0; one row of entries per batch when vals_batched. */
+int cm_bcsr_inverse_diagonal(cm_ctx_t ctx, int block_dim, int rows, int nnz,
+                             int batches, const int* JA, const int* IA,
+                             const float* vals, int vals_batched,
+                             float* invdiag);
+
 /* Product whose sparse matrix is a component-wise expression, evaluated
  * lazily inside the product's value loads: ONE launch, no value array
  * written. Replaces ProductAssignment over SparseExpressionOp<CSR>
@@ -447,6 +474,25 @@ int cm_pcg_dense_solve(cm_ctx_t ctx, int dtype, int n, int batches,
                        int use_guess, double tol, int max_iter,
                        int* iterations_out, double* error_out,
                        cm_cg_stats* stats);
+
+/* Jacobi-PCG over block scalars: ConjugateGradient<SparseMatrix<real3x3>>
+ * (cuMat/src/ConjugateGradient.h:47-61,66-207 with the plug-ins of
+ * tests/TestBlockedConjugateGradient.cu:111-239; the reference's test
+ * :244-275). Blocks and vectors as cm_bcsr_spmv; b and x hold 'rows'
+ * float4 entries per batch. Vector operations act per component, dot
+ * products sum all four components, norms the first block_dim (components
+ * >= block_dim of b are ignored; those of x keep the guess's values, 0
+ * without a guess), and the Jacobi entries are cm_bcsr_inverse_diagonal's
+ * (invdiag may be NULL: the solver derives them). Semantics and outputs as
+ * cm_pcg_solve (multi-kernel path, batches <= 64, all batches iterate until
+ * all converge; max_iter < 0: 2 * rows); vals_batched as cm_pcg_solve_ex. */
+int cm_pcg_bcsr_solve(cm_ctx_t ctx, int block_dim, int rows, int nnz,
+                      int batches, const int* JA, const int* IA,
+                      const float* vals, int vals_batched,
+                      const float* invdiag, const float* b, float* x,
+                      int use_guess, double tol, int max_iter,
+                      int* iterations_out, double* error_out,
+                      cm_cg_stats* stats);
 
 /* ======================================================================
  * Addition 2 (no reference counterpart): persistent batched CG over many

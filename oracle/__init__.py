@@ -63,6 +63,8 @@ def lib():
         L.orc_cwise_unary.argtypes = [i32, i32, i64, vp, vp]
         L.orc_assign.argtypes = [i32, i32, i64, vp, vp]
         L.orc_csr_inverse_diagonal_f64.argtypes = [i32, vp, vp, vp, vp]
+        L.orc_bcsr_spmv_f32.argtypes = [i32, i32, i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, i32]
+        L.orc_bcsr_inverse_diagonal_f32.argtypes = [i32, i32, i32, i32, vp, vp, vp, i32, vp]
         for name in ("orc_pcg_f32", "orc_pcg_f64"):
             getattr(L, name).argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, i32, f64, i32,
                                          C.POINTER(C.c_double)]
@@ -152,6 +154,67 @@ def ell_spmv_f32(indices_colmajor, vals, x, rows, cols, nnz_per_row, batches=1,
                  vals_batched=False, x_batched=False, y=None, mode=0):
     return ell_spmv(indices_colmajor, np.asarray(vals, np.float32), x, rows, cols, nnz_per_row, batches,
                     vals_batched, x_batched, y, mode)
+
+
+def bcsr_spmv(D, JA, IA, vals, x, rows, cols, batches=1, vals_batched=False, x_batched=False,
+              y=None, mode=0):
+    """Block-CSR product: vals (nnz, D, 4) float32 blocks of D float4 rows,
+    x (cols*4 per batch) float32; returns rows*4 per batch."""
+    if y is None:
+        y = np.zeros(rows * 4 * batches, np.float32)
+    JA = np.ascontiguousarray(JA, np.int32)
+    IA = np.ascontiguousarray(IA, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    lib().orc_bcsr_spmv_f32(D, rows, cols, len(IA), batches, _p(JA), _p(IA), _p(vals), int(vals_batched),
+                            _p(x), int(x_batched), _p(y), mode)
+    return y
+
+
+def bcsr_inverse_diagonal(D, JA, IA, vals, rows, batches=1, vals_batched=False):
+    out = np.zeros(rows * 4 * (batches if vals_batched else 1), np.float32)
+    JA = np.ascontiguousarray(JA, np.int32)
+    IA = np.ascontiguousarray(IA, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    lib().orc_bcsr_inverse_diagonal_f32(D, rows, len(IA), batches, _p(JA), _p(IA), _p(vals),
+                                        int(vals_batched), _p(out))
+    return out
+
+
+def bcsr_expand(D, JA, IA, vals, rows, dtype=np.float64):
+    """The scalar CSR matrix of size 4*rows that a block matrix acts as on
+    float4 vectors: row 4i+q (q < D) holds row q of each block (i, j) at
+    columns 4j..4j+3, in block order; rows 4i+q with q >= D are empty."""
+    JA = np.asarray(JA, np.int64)
+    IA = np.asarray(IA, np.int64)
+    nnz = IA.size
+    V = np.asarray(vals, np.float64).reshape(nnz, D, 4)
+    brow = np.repeat(np.arange(rows, dtype=np.int64), np.diff(JA))
+    q = np.arange(D, dtype=np.int64)
+    c = np.arange(4, dtype=np.int64)
+    srow = np.broadcast_to(4 * brow[:, None, None] + q[None, :, None], (nnz, D, 4)).reshape(-1)
+    scol = np.broadcast_to(4 * IA[:, None, None] + c[None, None, :], (nnz, D, 4)).reshape(-1)
+    order = np.argsort(srow, kind="stable")  # (block, q, c) order -> by row, blocks then c
+    counts = np.bincount(srow, minlength=4 * rows)
+    sJA = np.zeros(4 * rows + 1, np.int64)
+    sJA[1:] = np.cumsum(counts)
+    return (sJA.astype(np.int32), scol[order].astype(np.int32), V.reshape(-1)[order].astype(dtype))
+
+
+def bcsr_pcg(D, JA, IA, vals, b, rows, batches=1, x0=None, tol=1e-6, max_iter=1000, dtype=np.float64):
+    """Jacobi-PCG over block scalars (cuMat/src/ConjugateGradient.h:123-206
+    with the plug-ins of tests/TestBlockedConjugateGradient.cu:111-239):
+    every vector operation is per component, dot sums the four components,
+    squaredNorm the first D (the rest stay 0), and the Jacobi entries are
+    the diagonal blocks' diagonals. That is the scalar PCG of the expanded
+    matrix (bcsr_expand) on b with components >= D zeroed: the empty rows'
+    Jacobi entry 1 then only ever multiplies 0. Returns (x, iterations,
+    error)."""
+    sJA, sIA, sV = bcsr_expand(D, JA, IA, vals, rows, dtype)
+    bz = np.array(b, dtype).reshape(batches, rows, 4)
+    bz[:, :, D:] = 0
+    return pcg(sJA, sIA, sV, bz.reshape(-1), 4 * rows, batches=batches,
+               x0=None if x0 is None else np.asarray(x0, dtype), tol=tol, max_iter=max_iter)
 
 
 def reduce(a, op, axis, rows, cols, batches, in_row_major, out_row_major):

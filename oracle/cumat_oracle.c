@@ -166,6 +166,67 @@ void orc_csr_spmv_i32(int rows, int cols, int nnz, int batches,
     }
 }
 
+void orc_bcsr_spmv_f32(int D, int rows, int cols, int nnz, int batches,
+                       const int32_t* JA, const int32_t* IA,
+                       const float* vals, int vals_batched,
+                       const float* x, int x_batched, float* y, int mode)
+{
+    /* cuMat/src/SparseProductEvaluation.h:43-88 over block scalars: the
+     * first block's product of a row, then += in CSR order, each product the
+     * block ProductElementFunctor (tests/TestBlockedConjugateGradient.cu:
+     * 144-157: component q = dot(row q of the block, x_j)) with dot of
+     * tests/TestBlockedConjugateGradient.cu:17-20. A block is D float4 rows
+     * (4*D floats), a vector entry one float4; components >= D of a product
+     * are 0 (make_real3 sets w = 0). An empty block row assigns 0. */
+    for (int b = 0; b < batches; ++b) {
+        const float* v = vals + (vals_batched ? (int64_t)b * nnz * 4 * D : 0);
+        const float* xb = x + (x_batched ? (int64_t)b * cols * 4 : 0);
+        for (int r = 0; r < rows; ++r) {
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int k = JA[r]; k < JA[r + 1]; ++k) {
+                const float* xc = xb + (int64_t)IA[k] * 4;
+                for (int q = 0; q < 4; ++q) {
+                    float t = 0.0f;
+                    if (q < D) {
+                        const float* a = v + ((int64_t)k * D + q) * 4;
+                        t = a[0] * xc[0] + a[1] * xc[1] + a[2] * xc[2] + a[3] * xc[3];
+                    }
+                    acc[q] = (k == JA[r]) ? t : acc[q] + t;
+                }
+            }
+            for (int q = 0; q < 4; ++q) {
+                float* yy = &y[((int64_t)rows * b + r) * 4 + q];
+                *yy = assign_f32(mode, *yy, acc[q]);
+            }
+        }
+    }
+}
+
+void orc_bcsr_inverse_diagonal_f32(int D, int rows, int nnz, int batches, const int32_t* JA, const int32_t* IA,
+                                   const float* vals, int vals_batched, float* out)
+{
+    /* A.diagonal().cwiseInverseCheck() over block scalars
+     * (cuMat/src/ConjugateGradient.h:218-246 with the ExtractDiagonalFunctor
+     * and cwiseInverseCheck of tests/TestBlockedConjugateGradient.cu:174-210):
+     * the diagonal block's diagonal (a zero block where none is stored),
+     * each component 1/d or 1 for d == 0; components >= D are 0. */
+    for (int b = 0; b < (vals_batched ? batches : 1); ++b) {
+        const float* v = vals + (vals_batched ? (int64_t)b * nnz * 4 * D : 0);
+        for (int r = 0; r < rows; ++r) {
+            int kd = -1;
+            for (int k = JA[r]; k < JA[r + 1]; ++k) if (IA[k] == r) { kd = k; break; }
+            for (int q = 0; q < 4; ++q) {
+                float o = 0.0f;
+                if (q < D) {
+                    float d = (kd < 0) ? 0.0f : v[((int64_t)kd * D + q) * 4 + q];
+                    o = (d == 0.0f) ? 1.0f : 1.0f / d;
+                }
+                out[((int64_t)rows * b + r) * 4 + q] = o;
+            }
+        }
+    }
+}
+
 void orc_ell_spmv_f32(int rows, int cols, int nnz_per_row, int batches,
                       const int32_t* indices, const float* vals, int vals_batched,
                       const float* x, int x_batched, float* y, int mode)

@@ -9,6 +9,8 @@
  * committed Eigen runs (tests/golden/eigen_cg.json), and closed forms:
  *   orc_csr_spmv_*    4x5 fixture (plain, batched, +=/-=), Dirichlet A*1
  *   orc_ell_spmv_*    ELLPACK fixture in every dtype, equal to the CSR form
+ *   orc_bcsr_*        the dense expansion (numpy), blocks holding one scalar
+ *                     equal to orc_csr_spmv_f32, the blocked CG fixture
  *   orc_reduce        integer fixtures of every op / axis / layout, sum 1..N
  *   orc_dot           batched dot and squared-norm fixtures
  *   orc_cwise_*, orc_assign  compound-assignment fixtures, C / and %
@@ -47,6 +49,15 @@ void orc_csr_spmv_i32(int rows, int cols, int nnz, int batches,
                       const int32_t* JA, const int32_t* IA,
                       const int32_t* vals, int vals_batched,
                       const int32_t* x, int x_batched, int32_t* y, int mode);
+/* Block CSR with D x D float blocks (D = 2..4) stored as D float4 rows and
+ * float4 vector entries, the reference's block-scalar plug-in types
+ * (tests/TestBlockedConjugateGradient.cu:12-239); rows/cols count blocks. */
+void orc_bcsr_spmv_f32(int D, int rows, int cols, int nnz, int batches,
+                       const int32_t* JA, const int32_t* IA,
+                       const float* vals, int vals_batched,
+                       const float* x, int x_batched, float* y, int mode);
+void orc_bcsr_inverse_diagonal_f32(int D, int rows, int nnz, int batches, const int32_t* JA, const int32_t* IA,
+                                   const float* vals, int vals_batched, float* out);
 /* ELLPACK (cuMat/src/SparseProductEvaluation.h:194-226) */
 void orc_ell_spmv_f32(int rows, int cols, int nnz_per_row, int batches,
                       const int32_t* indices, const float* vals, int vals_batched,

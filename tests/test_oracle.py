@@ -284,3 +284,134 @@ def test_ell_matches_csr_on_poisson(orc):
                       x, n, n, 7)
     yc = orc.csr_spmv(P["JA"], P["IA"], P["vals"], x, n, n)
     assert np.array_equal(ye, yc)
+
+
+# ------------------------------------------------------------------ block CSR
+def _bcsr_dense(D, JA, IA, V, rows, cols):
+    """Dense (4 rows) x (4 cols) float64 matrix a block matrix acts as."""
+    A = np.zeros((4 * rows, 4 * cols))
+    V = np.asarray(V, np.float64).reshape(len(IA), D, 4)
+    for i in range(rows):
+        for k in range(JA[i], JA[i + 1]):
+            A[4 * i:4 * i + D, 4 * IA[k]:4 * IA[k] + 4] += V[k]
+    return A
+
+
+def _rand_bcsr(rng, rows, cols, D, max_per_row, integer=True):
+    JA = [0]
+    IA = []
+    for _ in range(rows):
+        k = int(rng.integers(0, max_per_row + 1))
+        IA.extend(sorted(rng.choice(cols, size=min(k, cols), replace=False)))
+        JA.append(len(IA))
+    nnz = len(IA)
+    V = (rng.integers(-4, 5, (nnz, D, 4)) if integer else rng.standard_normal((nnz, D, 4))).astype(np.float32)
+    return np.array(JA, np.int32), np.array(IA, np.int32), V
+
+
+@pytest.mark.parametrize("D", [2, 3, 4])
+def test_bcsr_spmv_dense_expansion(orc, D):
+    rng = np.random.default_rng(10 + D)
+    rows, cols = 37, 29
+    JA, IA, V = _rand_bcsr(rng, rows, cols, D, 9)
+    x = rng.integers(-5, 6, (3, cols * 4)).astype(np.float32)
+    A = _bcsr_dense(D, JA, IA, V, rows, cols)
+    y = orc.bcsr_spmv(D, JA, IA, V, x[0], rows, cols)
+    assert np.array_equal(y, (A @ x[0]).astype(np.float32))
+    assert np.all(y.reshape(rows, 4)[:, D:] == 0)
+    # batched values and vectors (scales 1, 2, -1 on the values)
+    V3 = np.concatenate([V * s for s in (1, 2, -1)])
+    y3 = orc.bcsr_spmv(D, JA, IA, V3, x.reshape(-1), rows, cols, batches=3, vals_batched=True, x_batched=True)
+    exp = np.concatenate([s * (A @ x[b]) for b, s in enumerate((1, 2, -1))]).astype(np.float32)
+    assert np.array_equal(y3, exp)
+    # compound modes
+    y0 = rng.integers(-5, 6, rows * 4).astype(np.float32)
+    for mode, f in ((1, lambda a, b: a + b), (2, lambda a, b: a - b)):
+        yy = orc.bcsr_spmv(D, JA, IA, V, x[0], rows, cols, y=y0.copy(), mode=mode)
+        assert np.array_equal(yy, f(y0, (A @ x[0]).astype(np.float32)))
+
+
+def test_bcsr_spmv_equals_scalar_csr_when_blocks_hold_one_scalar(orc):
+    # blocks whose only entry is (0, 0) and vectors whose only component is
+    # x: component x of the block product is the scalar CSR product, bit for
+    # bit with random floats (same products, same CSR-order sums)
+    rng = np.random.default_rng(7)
+    rows, cols = 300, 280
+    JA, IA, _ = _rand_bcsr(rng, rows, cols, 3, 12)
+    s = rng.standard_normal(len(IA)).astype(np.float32)
+    V = np.zeros((len(IA), 3, 4), np.float32)
+    V[:, 0, 0] = s
+    xs = rng.standard_normal(cols).astype(np.float32)
+    x4 = np.zeros((cols, 4), np.float32)
+    x4[:, 0] = xs
+    y = orc.bcsr_spmv(3, JA, IA, V, x4.reshape(-1), rows, cols).reshape(rows, 4)
+    assert np.array_equal(y[:, 0], orc.csr_spmv(JA, IA, s, xs, rows, cols))
+    assert np.all(y[:, 1:] == 0)
+
+
+def test_bcsr_inverse_diagonal(orc):
+    from cumat_amd.problems import block_poisson
+    JA, IA, V = block_poisson((5, 6), 3, seed=3)
+    rows = 30
+    V = V.copy()
+    # row 4: zero the (1, 1) entry of its diagonal block; row 7: no diagonal block
+    k4 = [k for k in range(JA[4], JA[5]) if IA[k] == 4][0]
+    V[k4, 1, 1] = 0
+    k7 = [k for k in range(JA[7], JA[8]) if IA[k] == 7][0]
+    IA = IA.copy()
+    IA[k7] = IA[k7 - 1] if k7 > JA[7] else IA[k7 + 1]
+    out = orc.bcsr_inverse_diagonal(3, JA, IA, V, rows).reshape(rows, 4)
+    for i in range(rows):
+        ks = [k for k in range(JA[i], JA[i + 1]) if IA[k] == i]
+        d = np.diag(V[ks[0], :, :3]) if ks else np.zeros(3, np.float32)
+        exp = np.where(d == 0, np.float32(1), np.float32(1) / np.where(d == 0, np.float32(1), d))
+        assert np.array_equal(out[i, :3], exp.astype(np.float32)), i
+        assert out[i, 3] == 0
+    assert out[4, 1] == 1 and np.all(out[7, :3] == 1)
+
+
+def test_bcsr_pcg_reference_fixture(orc):
+    g = GOLD["blocked_cg_3x3"]
+    D, rows = g["block_dim"], g["rows"]
+    V = np.array(g["blocks"], np.float32)
+    b = np.array(g["b"], np.float32).reshape(-1)
+    x, it, err = orc.bcsr_pcg(D, g["JA"], g["IA"], V, b, rows, tol=g["tolerance"],
+                              max_iter=g["max_iterations"], dtype=np.float32)
+    assert err < g["tolerance"] and 0 < it < g["max_iterations"]
+    A = _bcsr_dense(D, np.array(g["JA"]), np.array(g["IA"]), V, rows, rows)
+    sel = [4 * i + q for i in range(rows) for q in range(D)]
+    xe = np.linalg.solve(A[np.ix_(sel, sel)], b[sel].astype(np.float64))
+    assert np.allclose(x[sel], xe, rtol=0, atol=1e-4 * np.abs(xe).max())
+    assert np.all(x.reshape(rows, 4)[:, 3] == 0)
+
+
+def test_bcsr_pcg_closed_forms(orc):
+    from cumat_amd.problems import block_poisson
+    rng = np.random.default_rng(5)
+    # block-diagonal matrix of diagonal blocks: Jacobi-PCG is exact at once
+    rows = 40
+    JA = np.arange(rows + 1, dtype=np.int32)
+    IA = np.arange(rows, dtype=np.int32)
+    V = np.zeros((rows, 3, 4), np.float64)
+    d = rng.uniform(1, 3, (rows, 3))
+    for q in range(3):
+        V[:, q, q] = d[:, q]
+    b = rng.standard_normal(rows * 4)
+    x, it, err = orc.bcsr_pcg(3, JA, IA, V, b, rows, tol=1e-12)
+    assert it == 0 and err < 1e-14
+    assert np.allclose(x.reshape(rows, 4)[:, :3], b.reshape(rows, 4)[:, :3] / d, rtol=1e-14)
+    # a general SPD block system: A x = b to the FP64 tolerance, and the w
+    # component of b does not enter (the reference's squaredNorm and
+    # preconditioner ignore it)
+    JA, IA, V = block_poisson((6, 7), 3, seed=9)
+    rows = 42
+    b = rng.standard_normal(rows * 4)
+    x, it, err = orc.bcsr_pcg(3, JA, IA, V, b, rows, tol=1e-12, max_iter=500)
+    A = _bcsr_dense(3, JA, IA, V, rows, rows)
+    sel = [4 * i + q for i in range(rows) for q in range(3)]
+    r = b[sel] - A[np.ix_(sel, sel)] @ x[sel]
+    assert np.linalg.norm(r) <= 1e-11 * np.linalg.norm(b[sel]) and 0 < it <= 3 * rows
+    b2 = b.reshape(rows, 4).copy()
+    b2[:, 3] = 0
+    x2, it2, _ = orc.bcsr_pcg(3, JA, IA, V, b2.reshape(-1), rows, tol=1e-12, max_iter=500)
+    assert it2 == it and np.array_equal(x2, x)
