@@ -11,6 +11,10 @@
 // matrix. The whole loop runs on the device (one cm_pcg_solve call); the host
 // only checks convergence every few iterations.
 //
+// Block scalars (BlockScalar.h: SparseMatrix<BlockRows<3>> with Float4
+// vectors, or the user's own types declared through BlockScalarTraits) run
+// the block-scalar CG of the reference's blocked test.
+//
 // Two extensions without a cuMat counterpart sit beside it:
 // BatchedConjugateGradient (many independent uneven systems, one persistent
 // launch) and MixedPrecisionConjugateGradient (FP32 / BF16 inner CG inside
@@ -23,11 +27,13 @@
 CUMAT_NAMESPACE_BEGIN
 
 // invdiag = 1 / diag(A), 1 where the diagonal is zero (cwiseInverseCheck)
+// (block scalars: the diagonal blocks' diagonals, one vector entry per
+// block row, with the zero guard per component)
 template <typename _MatrixType>
 class DiagonalPreconditioner {
 public:
     typedef typename _MatrixType::Scalar Scalar;
-    typedef Matrix<Scalar, Dynamic, 1, 1, ColumnMajor> Vector;
+    typedef Matrix<typename internal::vector_scalar<Scalar>::type, Dynamic, 1, 1, ColumnMajor> Vector;
 
 private:
     Vector entries_;
@@ -36,7 +42,12 @@ public:
     DiagonalPreconditioner() = default;
     explicit DiagonalPreconditioner(const _MatrixType& A) : entries_(A.rows())
     {
-        if constexpr (internal::is_sparse_matrix<_MatrixType>::value) {
+        if constexpr (internal::is_block_scalar<Scalar>::value) {
+            const auto& P = A.getSparsityPattern();
+            CUMAT_AMD_CALL(cm_bcsr_inverse_diagonal(Context::current().handle(), BlockScalarTraits<Scalar>::rows,
+                                                    (int)A.rows(), (int)A.nnz(), 1, P.JA.data(), P.IA.data(),
+                                                    (const float*)A.getData().data(), 0, (float*)entries_.data()));
+        } else if constexpr (internal::is_sparse_matrix<_MatrixType>::value) {
             static_assert((int)_MatrixType::SFlags == (int)SparseFlags::CSR, "the CG solver takes CSR matrices");
             const auto& P = A.getSparsityPattern();
             CUMAT_AMD_CALL(cm_csr_inverse_diagonal(Context::current().handle(), internal::DType<Scalar>::value,
@@ -91,14 +102,16 @@ public:
     typedef _MatrixType MatrixType;
     typedef _Preconditioner Preconditioner;
     typedef typename _MatrixType::Scalar Scalar;
-    typedef Scalar RealScalar;
-    typedef Matrix<Scalar, Dynamic, 1, Dynamic, ColumnMajor> Vector;
+    // tolerances and errors in the elemental type (float for block scalars)
+    typedef typename internal::elemental_scalar<Scalar>::type RealScalar;
+    typedef Matrix<typename internal::vector_scalar<Scalar>::type, Dynamic, 1, Dynamic, ColumnMajor> Vector;
+    static constexpr bool IsBlock = internal::is_block_scalar<Scalar>::value;
 
 private:
     MatrixType matrix_;
     Preconditioner preconditioner_;
     bool initialized_ = false;
-    RealScalar tolerance_ = std::numeric_limits<Scalar>::epsilon();
+    RealScalar tolerance_ = std::numeric_limits<RealScalar>::epsilon();
     Index maxIterations_ = -1;
     mutable Index iterations_ = 0;
     mutable RealScalar error_ = 0;
@@ -160,7 +173,15 @@ public:
         auto rhs = internal::materialize(b.derived());
         CUMAT_ASSERT_DIMENSION(guess.rows() == rhs.rows() && guess.cols() == 1);
         Vector x(rhs.rows(), 1, rhs.batches());
-        x.inplace() = guess.derived();  // broadcasts a batch-1 guess
+        if constexpr (IsBlock) {
+            // block vector entries are copied as bytes (one guess per batch)
+            auto g = internal::materialize(guess.derived());
+            CUMAT_ASSERT_DIMENSION(g.batches() == rhs.batches());
+            CUMAT_AMD_CALL(cm_memcpy_d2d(Context::current().handle(), x.data(), g.data(),
+                                         sizeof(typename Vector::Scalar) * x.size()));
+        } else {
+            x.inplace() = guess.derived();  // broadcasts a batch-1 guess
+        }
         run(rhs, x, true);
         return x;
     }
@@ -173,7 +194,17 @@ private:
         CUMAT_ASSERT_DIMENSION(rhs.rows() == matrix_.cols() && rhs.cols() == 1);
         int it = 0;
         double err = 0;
-        if constexpr (IsSparse) {
+        if constexpr (IsBlock) {
+            // the block-scalar CG of the reference's blocked test
+            // (tests/TestBlockedConjugateGradient.cu:244-275)
+            const auto& P = matrix_.getSparsityPattern();
+            CUMAT_AMD_CALL(cm_pcg_bcsr_solve(Context::current().handle(), BlockScalarTraits<Scalar>::rows,
+                                             (int)matrix_.rows(), (int)matrix_.nnz(), (int)rhs.batches(),
+                                             P.JA.data(), P.IA.data(), (const float*)matrix_.getData().data(), 0,
+                                             (const float*)preconditioner_.entries().data(),
+                                             (const float*)rhs.data(), (float*)x.data(), use_guess ? 1 : 0,
+                                             (double)tolerance_, (int)maxIterations(), &it, &err, &stats_));
+        } else if constexpr (IsSparse) {
             const auto& P = matrix_.getSparsityPattern();
             CUMAT_AMD_CALL(cm_pcg_solve(Context::current().handle(), internal::DType<Scalar>::value,
                                         (int)matrix_.rows(), (int)matrix_.nnz(), (int)rhs.batches(), P.JA.data(),

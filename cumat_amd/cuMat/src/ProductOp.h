@@ -12,6 +12,7 @@
 // with more than one column) are not on this library's path and are
 // rejected.
 
+#include "BlockScalar.h"
 #include "ReductionOps.h"
 
 CUMAT_NAMESPACE_BEGIN
@@ -22,7 +23,9 @@ class ProductOp {
     _Right x_;
 
 public:
-    typedef typename _Left::Scalar Scalar;
+    // the result's entry type: the vector entry type of a block-scalar
+    // matrix (BlockScalar.h), else the common scalar type
+    typedef typename internal::vector_scalar<typename _Left::Scalar>::type Scalar;
     static_assert(std::is_same<Scalar, typename _Right::Scalar>::value,
                   "No implicit casting between the factors of a product");
 

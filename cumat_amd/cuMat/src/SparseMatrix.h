@@ -406,10 +406,19 @@ struct is_sparse_matrix<SparseMatrix<S, B, F>> : std::true_type {};
 // when assigned to a dense matrix): one SpMV launch
 // (ProductAssignment over CSR / ELLPACK, cuMat/src/SparseProductEvaluation.h:43-139,194-324)
 template <typename S, int B, int F, typename X>
-void productEvalTo(const SparseMatrix<S, B, F>& A, const X& x, S* y, Index batches, AssignmentMode mode)
+void productEvalTo(const SparseMatrix<S, B, F>& A, const X& x, typename internal::vector_scalar<S>::type* y,
+                   Index batches, AssignmentMode mode)
 {
     const auto& P = A.getSparsityPattern();
-    if constexpr (F == SparseFlags::CSR) {
+    if constexpr (internal::is_block_scalar<S>::value) {
+        // block scalars: the block product of each stored block with x's
+        // entry, rows summed in CSR order (cm_bcsr_spmv)
+        static_assert(F == SparseFlags::CSR, "block scalars are stored in CSR");
+        CUMAT_AMD_CALL(cm_bcsr_spmv(Context::current().handle(), BlockScalarTraits<S>::rows, (int)P.rows,
+                                    (int)P.cols, (int)P.nnz, (int)batches, P.JA.data(), P.IA.data(),
+                                    (const float*)A.getData().data(), A.batches() > 1 ? 1 : 0,
+                                    (const float*)x.data(), x.batches() > 1 ? 1 : 0, (float*)y, (int)mode));
+    } else if constexpr (F == SparseFlags::CSR) {
         const int dt = internal::DType<S>::value;
         CUMAT_AMD_CALL(cm_csr_spmv(Context::current().handle(), dt, dt, (int)P.rows, (int)P.cols, (int)P.nnz,
                                    (int)batches, P.JA.data(), P.IA.data(), A.getData().data(),

@@ -383,3 +383,215 @@ TEST_CASE("dense A * x: row-major and column-major against the host, += and -=")
 }
 
 HARNESS_MAIN()
+
+// ---------------------------------------------------------------- block scalars
+// The reference's blocked CG (tests/TestBlockedConjugateGradient.cu:244-275)
+// with the built-in block types: a 3x3 block is three float4 rows, a vector
+// entry one float4 (w = 0), as the reference test's real3x3 / real3.
+typedef BlockRows<3> Block3;
+typedef SparseMatrix<Block3, 1, SparseFlags::CSR> SMatrix3x3;
+typedef Matrix<Float4, Dynamic, 1, 1, ColumnMajor> Vector3X;
+typedef Matrix<Float4, Dynamic, 1, Dynamic, ColumnMajor> Vector3XB;
+
+static Float4 r3(float x, float y, float z) { return Float4{x, y, z, 0.0f}; }
+static Block3 blk(Float4 a, Float4 b, Float4 c)
+{
+    Block3 m;
+    m.r[0] = a;
+    m.r[1] = b;
+    m.r[2] = c;
+    return m;
+}
+
+struct HostBsr {
+    int n = 0;
+    std::vector<int> ja{0}, ia;
+    std::vector<Block3> va;
+};
+
+static SMatrix3x3 toDeviceBlocks(const HostBsr& h)
+{
+    SparsityPattern<SparseFlags::CSR> P;
+    P.rows = P.cols = h.n;
+    P.nnz = (Index)h.ia.size();
+    P.JA = SparsityPattern<SparseFlags::CSR>::IndexVector::fromHost(h.ja);
+    P.IA = SparsityPattern<SparseFlags::CSR>::IndexVector::fromHost(h.ia);
+    SMatrix3x3 A(P);
+    A.getData().copyFromHost(h.va.data());
+    return A;
+}
+
+// the block product on the host in float, in the device's order: per block
+// row the first block's product, then += in CSR order; component q the
+// left-to-right four-term dot of row q with x_j
+static std::vector<Float4> hostBlockProduct(const HostBsr& h, const std::vector<Float4>& x)
+{
+    std::vector<Float4> y(h.n, Float4{0, 0, 0, 0});
+    for (int i = 0; i < h.n; ++i)
+        for (int k = h.ja[i]; k < h.ja[i + 1]; ++k) {
+            const Float4& xv = x[h.ia[k]];
+            float p[4] = {0, 0, 0, 0};
+            for (int q = 0; q < 3; ++q) {
+                const Float4& a = h.va[k].r[q];
+                p[q] = a.x * xv.x + a.y * xv.y + a.z * xv.z + a.w * xv.w;
+            }
+            Float4& o = y[i];
+            if (k == h.ja[i]) {
+                o = Float4{p[0], p[1], p[2], p[3]};
+            } else {
+                o.x += p[0];
+                o.y += p[1];
+                o.z += p[2];
+                o.w += p[3];
+            }
+        }
+    return y;
+}
+
+// |b - A x| / |b| over the first three components, in double
+static double blockRelResidual(const HostBsr& h, const std::vector<Float4>& b, const std::vector<Float4>& x)
+{
+    double rr = 0, bb = 0;
+    for (int i = 0; i < h.n; ++i) {
+        double ax[3] = {0, 0, 0};
+        for (int k = h.ja[i]; k < h.ja[i + 1]; ++k) {
+            const Float4& xv = x[h.ia[k]];
+            for (int q = 0; q < 3; ++q) {
+                const Float4& a = h.va[k].r[q];
+                ax[q] += (double)a.x * xv.x + (double)a.y * xv.y + (double)a.z * xv.z + (double)a.w * xv.w;
+            }
+        }
+        const double bv[3] = {b[i].x, b[i].y, b[i].z};
+        for (int q = 0; q < 3; ++q) {
+            rr += (bv[q] - ax[q]) * (bv[q] - ax[q]);
+            bb += bv[q] * bv[q];
+        }
+    }
+    return std::sqrt(rr / bb);
+}
+
+// 2D 5-point block Poisson: block (i, j) = L_ij * K with K = tridiag(-0.25, 1.5, -0.25),
+// diagonal blocks + 0.05 I
+static HostBsr blockPoisson(int g)
+{
+    HostBsr h;
+    h.n = g * g;
+    auto K = [](float s, float d) {
+        return blk(r3(s * 1.5f + d, -0.25f * s, 0), r3(-0.25f * s, s * 1.5f + d, -0.25f * s),
+                   r3(0, -0.25f * s, s * 1.5f + d));
+    };
+    for (int x = 0; x < g; ++x)
+        for (int y = 0; y < g; ++y) {
+            int row = y + x * g;
+            if (x > 0) { h.ia.push_back(row - g); h.va.push_back(K(-1, 0)); }
+            if (y > 0) { h.ia.push_back(row - 1); h.va.push_back(K(-1, 0)); }
+            h.ia.push_back(row); h.va.push_back(K(4, 0.05f));
+            if (y < g - 1) { h.ia.push_back(row + 1); h.va.push_back(K(-1, 0)); }
+            if (x < g - 1) { h.ia.push_back(row + g); h.va.push_back(K(-1, 0)); }
+            h.ja.push_back((int)h.ia.size());
+        }
+    return h;
+}
+
+static std::vector<Float4> randomBlockVector(int n, unsigned seed)
+{
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<float> u(-1, 1);
+    std::vector<Float4> v(n);
+    for (auto& t : v) {
+        const float a = u(gen), b = u(gen), c = u(gen);
+        t = r3(a, b, c);
+    }
+    return v;
+}
+
+TEST_CASE("Blocked CG (TestBlockedConjugateGradient.cu:244-275)")
+{
+    SparsityPattern<SparseFlags::CSR> pattern;
+    pattern.rows = 3;
+    pattern.cols = 3;
+    pattern.nnz = 7;
+    pattern.JA = SparsityPattern<SparseFlags::CSR>::IndexVector::fromHost(std::vector<int>{0, 2, 5, 7});
+    pattern.IA = SparsityPattern<SparseFlags::CSR>::IndexVector::fromHost(std::vector<int>{0, 1, 0, 1, 2, 1, 2});
+    SMatrix3x3 mat(pattern);
+    std::vector<Block3> data1 = {
+        blk(r3(2, -1, 0), r3(-1, 2, -1), r3(0, -1, 2)), blk(r3(0, 0, 0), r3(0, 0, 0), r3(-1, 0, 0)),
+        blk(r3(0, 0, -1), r3(0, 0, 0), r3(0, 0, 0)),   blk(r3(2, -1, 0), r3(-1, 2, -1), r3(0, -1, 2)),
+        blk(r3(0, 0, 0), r3(0, 0, 0), r3(-1, 0, 0)),   blk(r3(0, 0, -1), r3(0, 0, 0), r3(0, 0, 0)),
+        blk(r3(2, -1, 0), r3(-1, 2, -1), r3(0, -1, 1))};
+    mat.getData().copyFromHost(data1.data());
+
+    Vector3X b(3);
+    std::vector<Float4> data2 = {r3(0.1f, 0, 0), r3(0, 0, 0), r3(0, 0, -0.1f)};
+    b.copyFromHost(data2.data());
+
+    ConjugateGradient<SMatrix3x3> cg(mat);
+    cg.setMaxIterations(30);
+    float tol = 1e-5f;
+    cg.setTolerance(tol);
+    Vector3X x = cg.solve(b);
+    REQUIRE(cg.error() < cg.tolerance());  // converged
+    REQUIRE(cg.iterations() < cg.maxIterations());
+    REQUIRE(cg.iterations() > 0);  // but something was done
+
+    HostBsr h;
+    h.n = 3;
+    h.ja = {0, 2, 5, 7};
+    h.ia = {0, 1, 0, 1, 2, 1, 2};
+    h.va = data1;
+    std::vector<Float4> xh(3);
+    x.copyToHost(xh.data());
+    REQUIRE(blockRelResidual(h, data2, xh) < 1e-4);
+    for (const auto& v : xh) REQUIRE(v.w == 0.0f);
+}
+
+TEST_CASE("Block SpMV on a 2D block Poisson matrix: the host's bits, y += A x")
+{
+    const HostBsr h = blockPoisson(40);
+    const SMatrix3x3 A = toDeviceBlocks(h);
+    const auto xh = randomBlockVector(h.n, 3);
+    Vector3X x(h.n);
+    x.copyFromHost(xh.data());
+    Vector3X y = A * x;
+    std::vector<Float4> yh(h.n);
+    y.copyToHost(yh.data());
+    const auto ref = hostBlockProduct(h, xh);
+    REQUIRE(std::memcmp(yh.data(), ref.data(), sizeof(Float4) * h.n) == 0);
+    y += A * x;  // the compound mode of the product
+    y.copyToHost(yh.data());
+    bool doubled = true;
+    for (int i = 0; i < h.n; ++i)
+        doubled = doubled && yh[i].x == ref[i].x + ref[i].x && yh[i].y == ref[i].y + ref[i].y &&
+                  yh[i].z == ref[i].z + ref[i].z && yh[i].w == 0.0f;
+    REQUIRE(doubled);
+}
+
+TEST_CASE("Blocked CG on a 2D block Poisson matrix: solve, guess, batched right-hand sides")
+{
+    const HostBsr h = blockPoisson(40);
+    const SMatrix3x3 A = toDeviceBlocks(h);
+    const auto bh = randomBlockVector(h.n, 5);
+    Vector3X b(h.n);
+    b.copyFromHost(bh.data());
+    ConjugateGradient<SMatrix3x3> cg(A);
+    cg.setTolerance(1e-6f);
+    Vector3X x = cg.solve(b);
+    REQUIRE(cg.error() < 1e-6f && cg.iterations() > 0);
+    std::vector<Float4> xh(h.n);
+    x.copyToHost(xh.data());
+    REQUIRE(blockRelResidual(h, bh, xh) < 1e-5);
+    // the solution as the guess: no more than a couple of iterations
+    Vector3X x2 = cg.solveWithGuess(b, x);
+    REQUIRE(cg.iterations() <= 2 && cg.error() < 1e-6f);
+    // two right-hand sides: both reach the tolerance
+    const auto b2 = randomBlockVector(h.n, 6);
+    std::vector<Float4> both(bh);
+    both.insert(both.end(), b2.begin(), b2.end());
+    Vector3XB bb(h.n, 1, 2);
+    bb.copyFromHost(both.data());
+    auto xb = cg.solve(bb);
+    std::vector<Float4> xbh(2 * (size_t)h.n);
+    xb.copyToHost(xbh.data());
+    REQUIRE(blockRelResidual(h, bh, std::vector<Float4>(xbh.begin(), xbh.begin() + h.n)) < 1e-5);
+    REQUIRE(blockRelResidual(h, b2, std::vector<Float4>(xbh.begin() + h.n, xbh.end())) < 1e-5);
+}
