@@ -956,6 +956,7 @@ int cm_pcg_bcsr_solve(cm_ctx_t ctx, int block_dim, int rows, int nnz, int batche
     op.vstride = vstride;
     op.rows = rows;
     op.D = block_dim;
+    op.nnz = nnz;
     cm::CgBatchOpts opt;
     opt.vals_batched = vals_batched;  // the Jacobi entries then have one row of 4*rows per batch
     opt.block = &op;
