@@ -27,6 +27,7 @@
 #include "cg_multi.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <map>
@@ -915,7 +916,8 @@ int cm_pcg_bcsr_solve(cm_ctx_t ctx, int block_dim, int rows, int nnz, int batche
                block_dim);
     CM_REQUIRE(rows >= 0 && nnz >= 0 && batches >= 1 && batches <= cm::CG_MAXB, CM_E_DIM,
                "cm_pcg_bcsr_solve: bad sizes (batches 1..%d)", cm::CG_MAXB);
-    CM_REQUIRE(rows <= (1 << 29), CM_E_DIM, "cm_pcg_bcsr_solve: %d block rows (max 2^29)", rows);
+    CM_REQUIRE(rows <= INT_MAX / 4, CM_E_DIM, "cm_pcg_bcsr_solve: %d block rows (max %d: 4 floats per row)", rows,
+               INT_MAX / 4);
     CM_REQUIRE(tol >= 0, CM_E_ARG, "cm_pcg_bcsr_solve: negative tolerance");
     if (max_iter < 0) max_iter = 2 * rows;  // IterativeSolverBase::maxIterations: 2 * cols (in blocks)
     if (rows == 0) {

@@ -256,8 +256,8 @@ int cm_bcsr_spmv(cm_ctx_t ctx, int block_dim, int rows, int cols, int nnz, int b
     CM_TRY(cm::resolve(ctx, &ctx));
     CM_REQUIRE(block_dim >= 2 && block_dim <= 4, CM_E_UNSUPPORTED, "cm_bcsr_spmv: block dimension %d (2..4)",
                block_dim);
-    CM_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && batches >= 1, CM_E_DIM,
-               "cm_bcsr_spmv: bad sizes rows=%d cols=%d nnz=%d batches=%d", rows, cols, nnz, batches);
+    CM_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && batches >= 1 && batches <= 65535, CM_E_DIM,
+               "cm_bcsr_spmv: bad sizes rows=%d cols=%d nnz=%d batches=%d (1..65535)", rows, cols, nnz, batches);
     CM_REQUIRE(mode >= CM_ASSIGN && mode <= CM_DIV, CM_E_ARG, "cm_bcsr_spmv: bad mode %d", mode);
     cm::count(CM_CNT_EVAL_MATMUL_SPARSE);
     cm::count(CM_CNT_EVAL_ANY);
@@ -280,7 +280,8 @@ int cm_bcsr_inverse_diagonal(cm_ctx_t ctx, int block_dim, int rows, int nnz, int
     CM_TRY(cm::resolve(ctx, &ctx));
     CM_REQUIRE(block_dim >= 2 && block_dim <= 4, CM_E_UNSUPPORTED,
                "cm_bcsr_inverse_diagonal: block dimension %d (2..4)", block_dim);
-    CM_REQUIRE(rows >= 0 && nnz >= 0 && batches >= 1, CM_E_DIM, "cm_bcsr_inverse_diagonal: bad sizes");
+    CM_REQUIRE(rows >= 0 && nnz >= 0 && batches >= 1 && batches <= 65535, CM_E_DIM,
+               "cm_bcsr_inverse_diagonal: bad sizes");
     cm::count(CM_CNT_EVAL_CWISE);
     cm::count(CM_CNT_EVAL_ANY);
     if (rows == 0) return CM_OK;
