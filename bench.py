@@ -303,6 +303,25 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
         out[key] = res
         log(key, res)
     del vecs, outv, ia_, ib_, io_
+    # ---- FP64 program a*b+c over 2^25 doubles (64-bit registers), generated
+    # kernel against the interpreter
+    N64 = 1 << 25
+    da, db, dc = (cm.DeviceArray.from_numpy(rng.standard_normal(N64), ctx=ctx) for _ in range(3))
+    do_ = cm.DeviceArray(N64, dtype=np.float64, ctx=ctx)
+    e64 = da.expr() * db.expr() + dc.expr()
+    for key, jm in (("cwise_f64_generated_2^25", 2), ("cwise_f64_interpreter_2^25", 0)):
+        old_mode = cm.jit_mode(jm)
+        cm.assign(do_, e64, ctx=ctx)
+        ctx.sync()
+        timer.start(stream)
+        for _ in range(10):
+            cm.assign(do_, e64, ctx=ctx)
+        ms = timer.stop_ms(stream) / 10
+        cm.jit_mode(old_mode)
+        out[key] = {"a*b+c": {"us": round(ms * 1000, 1), "GBps": round(32 * N64 / (ms * 1e-3) / 1e9, 1),
+                              "frac": round(32 * N64 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        log(key, out[key])
+    del da, db, dc, do_
     # ---- axis sums of 1024 x 1024 x 64 (two rotating tensors of 256 MiB)
     R, Cc, B = 1024, 1024, 64
     ts = [cm.DeviceArray.from_numpy(rng.standard_normal(R * Cc * B, dtype=np.float32), rows=R, cols=Cc, batches=B,

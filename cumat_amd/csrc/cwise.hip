@@ -288,24 +288,33 @@ struct JitCwiseArgs {
 
 static const char* jit_raw_load(int dtype)
 {
-    // raw register bits of one stored element (the interpreter's load_raw)
-    return dtype == CM_BOOL ? "(R)((const unsigned char*)A.leaf[%d])[%s]" : "((const uint32_t*)A.leaf[%d])[%s]";
+    // raw register bits of one stored element (the interpreter's load_raw;
+    // 32-bit values are zero-extended in a 64-bit register)
+    if (dtype == CM_BOOL) return "(R)((const unsigned char*)A.leaf[%d])[%s]";
+    if (dtype == CM_F64) return "(R)((const uint64_t*)A.leaf[%d])[%s]";
+    return "(R)((const uint32_t*)A.leaf[%d])[%s]";
 }
 
 // Source and cache key of the generated kernel cm_jit_cwise for P. Leaves in
 // 'gmask' are gathered: element e reads leaf[idx[e] * gsc] (a leaf that
 // depends on the column of a stored CSR entry, idx = the column indices).
+// wide: a program with 64-bit values (FP64 leaves, instructions or
+// destination), evaluated in 64-bit registers as the interpreter's wide
+// path does; FP64 leaves and destinations then move two elements per
+// 16-byte access. Gathered leaves are 32-bit programs only.
 int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& body, unsigned gmask = 0,
-                          const int* idx = nullptr)
+                          const int* idx = nullptr, bool wide = false)
 {
     const int dst_dt = P.dst_dtype;
-    if (dst_dt != CM_F32 && dst_dt != CM_I32 && dst_dt != CM_BOOL) return JIT_DECLINED;
+    if (dst_dt != CM_F32 && dst_dt != CM_I32 && dst_dt != CM_BOOL && !(wide && dst_dt == CM_F64))
+        return JIT_DECLINED;
+    if (wide && gmask) return JIT_DECLINED;
     const bool dst_al = ((uintptr_t)P.dst & 15) == 0;
     const bool idx_al = ((uintptr_t)idx & 15) == 0;
     key.clear();
     body.clear();
     char buf[256];
-    snprintf(buf, sizeof(buf), "cwflat32|d%d%c|m%d|", dst_dt, dst_al ? 'a' : 'u', P.mode);
+    snprintf(buf, sizeof(buf), "%s|d%d%c|m%d|", wide ? "cwflat64" : "cwflat32", dst_dt, dst_al ? 'a' : 'u', P.mode);
     key += buf;
     if (gmask) {
         snprintf(buf, sizeof(buf), "i%c|", idx_al ? 'a' : 'u');
@@ -329,7 +338,8 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
     const char* namep[CM_CWISE_MAX_LEAVES];
     for (int l = 0; l < P.nleaves; ++l) {
         const cm_leaf& L = P.leaves[l];
-        if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL) return JIT_DECLINED;
+        if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL && !(wide && L.dtype == CM_F64))
+            return JIT_DECLINED;
         const bool scalar = (P.leaf_scalar >> l) & 1;
         const bool gathered = (gmask >> l) & 1;
         const bool al = L.dtype != CM_BOOL && ((uintptr_t)L.ptr & 15) == 0;
@@ -355,7 +365,17 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
             names[l] = buf;
             decls += "        R v" + std::to_string(l) + "[8];\n";
             char ld[160];
-            if (al) {
+            if (al && L.dtype == CM_F64) {
+                // two doubles per 16-byte non-temporal load, both groups
+                for (int h = 0; h < 4; ++h) {
+                    const int off = (h >> 1) * 256 + (h & 1) * 2;
+                    snprintf(buf, sizeof(buf),
+                             "            w = __builtin_nontemporal_load((const u2v*)((const uint64_t*)A.leaf[%d] + e0 + %d));\n"
+                             "            v%d[%d] = w.x; v%d[%d] = w.y;\n",
+                             l, off, l, 2 * h, l, 2 * h + 1);
+                    loads_full += buf;
+                }
+            } else if (al) {
                 // streamed once: 16-byte non-temporal loads of both groups
                 snprintf(buf, sizeof(buf),
                          "            u = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[%d] + e0));\n"
@@ -388,7 +408,10 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
     if (!jit::program_body(P, namep, prog)) return JIT_DECLINED;
     // store of element k at e (the interpreter's store_elem)
     std::string store_one;
-    if (dst_dt == CM_F32) {
+    if (dst_dt == CM_F64) {
+        store_one = "{ double* p = (double*)A.dst + e; const double v = getd(res[k]); "
+                    "*p = (MODE == CM_ASSIGN) ? v : assign1<double>(MODE, *p, v); }";
+    } else if (dst_dt == CM_F32) {
         store_one = "{ float* p = (float*)A.dst + e; const float v = getf(res[k]); "
                     "*p = (MODE == CM_ASSIGN) ? v : assign1<float>(MODE, *p, v); }";
     } else if (dst_dt == CM_I32) {
@@ -399,10 +422,29 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
                     "if (MODE == CM_AND) v = v && *p; else if (MODE == CM_OR) v = v || *p; *p = (unsigned char)v; }";
     }
     std::string store_vec;
-    if (dst_al && dst_dt != CM_BOOL) {
+    if (dst_al && dst_dt == CM_F64) {
+        store_vec =
+            "        if (full) {\n"
+            "#pragma unroll\n"
+            "            for (int h = 0; h < 4; ++h) {\n"
+            "                u2v* d = (u2v*)((uint64_t*)A.dst + e0 + (h >> 1) * 256 + (h & 1) * 2);\n"
+            "                u2v u;\n"
+            "                if (MODE == CM_ASSIGN) {\n"
+            "                    u.x = res[2 * h]; u.y = res[2 * h + 1];\n"
+            "                } else {\n"
+            "                    const u2v old = *d;\n"
+            "#pragma unroll\n"
+            "                    for (int j = 0; j < 2; ++j) u[j] = (unsigned long long)__double_as_longlong(\n"
+            "                        assign1<double>(MODE, __longlong_as_double((long long)old[j]), getd(res[2 * h + j])));\n"
+            "                }\n"
+            "                __builtin_nontemporal_store(u, d);\n"
+            "            }\n"
+            "            continue;\n"
+            "        }\n";
+    } else if (dst_al && dst_dt != CM_BOOL) {
         const char* comb = dst_dt == CM_F32
-                               ? "__float_as_uint(assign1<float>(MODE, __uint_as_float(old[j]), __uint_as_float(res[4 * q + j])))"
-                               : "(uint32_t)assign1<int>(MODE, (int)old[j], (int)res[4 * q + j])";
+                               ? "__float_as_uint(assign1<float>(MODE, __uint_as_float(old[j]), getf(res[4 * q + j])))"
+                               : "(uint32_t)assign1<int>(MODE, (int)old[j], geti(res[4 * q + j]))";
         store_vec = std::string(
                         "        if (full) {\n"
                         "#pragma unroll\n"
@@ -410,7 +452,8 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
                         "                u4v* d = (u4v*)((uint32_t*)A.dst + e0 + 256 * q);\n"
                         "                u4v u;\n"
                         "                if (MODE == CM_ASSIGN) {\n"
-                        "                    u.x = res[4 * q]; u.y = res[4 * q + 1]; u.z = res[4 * q + 2]; u.w = res[4 * q + 3];\n"
+                        "                    u.x = (uint32_t)res[4 * q]; u.y = (uint32_t)res[4 * q + 1];\n"
+                        "                    u.z = (uint32_t)res[4 * q + 2]; u.w = (uint32_t)res[4 * q + 3];\n"
                         "                } else {\n"
                         "                    const u4v old = *d;\n"
                         "#pragma unroll\n"
@@ -424,12 +467,13 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
                     "        }\n";
     }
     snprintf(buf, sizeof(buf), "constexpr int MODE = %d;\n", P.mode);
-    body = std::string("typedef unsigned int u4v __attribute__((ext_vector_type(4)));\n") + buf +
+    body = std::string("typedef unsigned int u4v __attribute__((ext_vector_type(4)));\n"
+                       "typedef unsigned long long u2v __attribute__((ext_vector_type(2)));\n") + buf +
            "struct JitCwiseArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) + "]; long long gsc[" +
            std::to_string(CM_CWISE_MAX_LEAVES) + "]; const int* idx; void* dst; long long n; };\n"
            "#define EL(e0, k) ((e0) + ((k) >> 2) * 256 + ((k) & 3))\n"
            "extern \"C\" __global__ __launch_bounds__(256) void cm_jit_cwise(const JitCwiseArgs A)\n{\n"
-           "    typedef uint32_t R;\n"
+           + std::string(wide ? "    typedef uint64_t R;\n" : "    typedef uint32_t R;\n") +
            "    const long long n = A.n;\n"
            "    const long long stride = (long long)gridDim.x * (256 * 8);\n" +
            scalars +
@@ -438,7 +482,8 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
            "        const bool full = EL(e0, 7) < n;\n" +
            decls +
            "        if (full) {\n"
-           "            u4v u;\n            (void)u;\n" +
+           "            u4v u;\n            (void)u;\n"
+           "            u2v w;\n            (void)w;\n" +
            loads_full +
            "        } else {\n" +
            loads_tail +
@@ -463,10 +508,11 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
     return CM_OK;
 }
 
-static int jit_cwise_flat(cm_ctx_t ctx, const CwProgram& P, unsigned gmask = 0, const int* idx = nullptr)
+static int jit_cwise_flat(cm_ctx_t ctx, const CwProgram& P, unsigned gmask = 0, const int* idx = nullptr,
+                          bool wide = false)
 {
     std::string key, body;
-    if (jit_cwise_flat_source(P, key, body, gmask, idx) != CM_OK) return JIT_DECLINED;
+    if (jit_cwise_flat_source(P, key, body, gmask, idx, wide) != CM_OK) return JIT_DECLINED;
     jit::Kernel k;
     if (jit::get(key, body, "cm_jit_cwise", &k) != 0) return JIT_DECLINED;
     JitCwiseArgs A;
@@ -666,8 +712,8 @@ static int cwise_dense(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const c
         int k = 0;
         if (cm::match_lincomb(P, A, k)) return cm::lincomb_run(ctx, A, k, (float*)dst, n);
     }
-    if (!wide && flat && P.dst_direct && cm::jit::want(n)) {
-        const int s = cm::jit_cwise_flat(ctx, P);
+    if (flat && P.dst_direct && cm::jit::want(n)) {
+        const int s = cm::jit_cwise_flat(ctx, P, 0, nullptr, wide);
         if (s != cm::JIT_DECLINED) return s;
     }
     if (wide) return cm::launch_dense_w<true>(ctx, P, flat);

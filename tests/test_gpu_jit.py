@@ -1,6 +1,6 @@
 """Run-time generated component-wise kernels (cm_jit_mode) against the
-program interpreter, bit for bit: every binary and unary functor in FP32 and
-int32, casts between all 32-bit dtypes, select, the FMA term, every
+program interpreter, bit for bit: every binary and unary functor in FP32,
+FP64 and int32, casts between all dtypes, select, the FMA term, every
 assignment mode, bool destinations, broadcast scalars, unaligned leaves and
 lengths that end inside a thread's group of elements. Both paths evaluate
 the instruction semantics of cwise_ops.hpp; the oracle pins the interpreter
@@ -201,6 +201,108 @@ def test_short_lengths(gpu, data, n):
     a = _Leaf(cm, data["f"].arr[:n])
     code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0), _instr(capi.OP_UNARY, capi.F32, capi.UN_EXP)]
     _same(cm, [a], code, np.float32, n=n)
+
+
+# ---------------------------------------------------------------- 64-bit programs
+@pytest.fixture(scope="module")
+def data64(gpu):
+    cm = gpu
+    rng = np.random.default_rng(2028)
+    d = rng.standard_normal(N) * 3
+    e = rng.uniform(0.1, 4, N)
+    e[::89] = 0.0
+    return {k: _Leaf(cm, v) for k, v in dict(d=d, e=e).items()}
+
+
+@pytest.mark.parametrize("fn", list(range(19)))
+def test_binary_f64(gpu, data64, fn):
+    """FP64 programs run in 64-bit registers in both paths; the generated
+    kernel moves two doubles per 16-byte access."""
+    from cumat_amd import _capi as capi
+    if fn in (capi.BIN_MOD, capi.BIN_AND, capi.BIN_OR, capi.BIN_XOR, capi.BIN_LAND, capi.BIN_LOR):
+        pytest.skip("not an FP64 functor")
+    cmp = fn >= capi.BIN_EQ
+    code = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F64, fn, capi.SRC_LEAF, 1)]
+    _same(gpu, [data64["d"], data64["e"]], code, np.bool_ if cmp else np.float64)
+    code = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F64, fn, capi.SRC_IMM, 0, flags=1, fimm=1.75)]
+    _same(gpu, [data64["e"]], code, np.bool_ if cmp else np.float64)
+
+
+@pytest.mark.parametrize("fn", list(range(27)))
+def test_unary_f64(gpu, data64, fn):
+    from cumat_amd import _capi as capi
+    if fn == capi.UN_BITNOT:
+        pytest.skip("not an FP64 functor")
+    code = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0), _instr(capi.OP_UNARY, capi.F64, fn)]
+    _same(gpu, [data64["e"] if fn in (capi.UN_ACOS, capi.UN_ASIN) else data64["d"]], code, np.float64)
+
+
+def test_mixed_width_programs(gpu, data, data64):
+    """32-bit leaves widened into an FP64 program (casts, FMA, select on a
+    bool leaf, a broadcast FP64 scalar) and FP64 programs stored to FP32,
+    int32 and bool destinations."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    s = _Leaf(cm, np.array([0.37]), scalar=True)
+    code = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),       # double(f32 f)
+            _instr(capi.OP_BINARY, capi.F64, capi.BIN_MUL, capi.SRC_LEAF, 3),  # * scalar
+            _instr(capi.OP_UNARY, capi.F64, capi.UN_EXP),
+            _instr(capi.OP_FMA, capi.F64, arg=2, fimm=0.5),                    # + d * 0.5
+            _instr(capi.OP_PUSH, capi.F64, arg=1),
+            _instr(capi.OP_LOAD, capi.BOOL, src=capi.SRC_LEAF, arg=1),
+            _instr(capi.OP_SELECT, capi.F64, src=capi.SRC_IMM, arg=1, fimm=2.5)]
+    leaves = [data["f"], data["b"], data64["d"], s]
+    _same(cm, leaves, code, np.float64)
+    _same(cm, leaves, code + [_instr(capi.OP_CAST, capi.F32, arg=capi.F64)], np.float32, mode=capi.ADD,
+          init=np.ones(N, np.float32))
+    _same(cm, leaves, code + [_instr(capi.OP_CAST, capi.I32, arg=capi.F64)], np.int32)
+    code_i = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),        # double(int32 i)
+              _instr(capi.OP_BINARY, capi.F64, capi.BIN_DIV, capi.SRC_LEAF, 1),
+              _instr(capi.OP_BINARY, capi.F64, capi.BIN_GT, capi.SRC_IMM, 0, fimm=-3.0)]
+    _same(cm, [data["i"], data64["e"]], code_i, np.bool_)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+def test_compound_modes_f64(gpu, data64, mode):
+    from cumat_amd import _capi as capi
+    code = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F64, capi.BIN_MUL, capi.SRC_LEAF, 1),
+            _instr(capi.OP_BINARY, capi.F64, capi.BIN_ADD, capi.SRC_LEAF, 0)]
+    init = np.random.default_rng(mode).uniform(0.5, 2, N)
+    _same(gpu, [data64["d"], data64["e"]], code, np.float64, mode=mode, init=init)
+
+
+@pytest.mark.parametrize("offset", [1, 3])
+def test_unaligned_f64_and_short_lengths(gpu, data64, offset):
+    from cumat_amd import _capi as capi
+    from cumat_amd.device import np_to_dtype
+    cm = gpu
+    a = _Leaf(cm, data64["d"].arr, offset=offset)
+    code = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F64, capi.BIN_SUB, capi.SRC_LEAF, 1),
+            _instr(capi.OP_UNARY, capi.F64, capi.UN_TANH)]
+    _same(cm, [a, data64["e"]], code, np.float64)
+    for n in (1, 5, 2049):
+        _same(cm, [_Leaf(cm, data64["d"].arr[:n])], code[:1] + code[2:], np.float64, n=n)
+    # a destination off the 16-byte grid
+    n = N - 5
+    L = (capi.Leaf * 1)()
+    L[0].ptr, L[0].dtype, L[0].kind = data64["d"].ptr, capi.F64, capi.LEAF_STRIDED
+    L[0].sr, L[0].sc, L[0].sb = 1, n, n
+    prog = (capi.Instr * 2)(code[0], code[2])
+    outs = []
+    for jit in (0, 2):
+        dst = cm.DeviceArray.from_numpy(np.zeros(n + offset))
+        old = cm.jit_mode(jit)
+        try:
+            capi.call("cm_cwise_eval", None, L, 1, prog, 2, dst.ptr + 8 * offset, np_to_dtype(np.float64),
+                      n, 1, 1, 1, n, n, 0)
+        finally:
+            cm.jit_mode(old)
+        outs.append(dst.numpy())
+    assert outs[0].tobytes() == outs[1].tobytes()
 
 
 def test_auto_mode_threshold_and_cache(gpu, data):
