@@ -827,9 +827,15 @@ int cm_pcg_solve(cm_ctx_t ctx, int dtype, int n, int nnz, int batches, const int
     CM_REQUIRE(JA && IA && vals && b && x, CM_E_ARG, "cm_pcg_solve: NULL pointer");
     const bool resident_ok = batches == 1 && n <= 16384;
     // measured on MI355X (FP32 2D Poisson, iterations/s): the one-workgroup
-    // solve wins up to 64^2 = 4096 rows (138 k vs 97 k), the multi-kernel
-    // path from 96^2 = 9216 rows (96 k vs 69 k); the crossover is ~6000 rows
-    if (path == CM_CG_AUTO) path = (resident_ok && n <= 6144) ? CM_CG_RESIDENT : CM_CG_MULTIKERNEL;
+    // solve wins up to 64^2 = 4096 rows (138 k vs 97 k), the launches from
+    // 96^2 = 9216 rows (96 k vs 69 k); the crossover is ~6000 rows. Up to
+    // 2^17 unknowns (n x batches) an iteration is launch-bound and the
+    // two-launch form wins despite its tripled gathers (us per iteration,
+    // three / two launches: 128^2 12.6 / 10.2, 256^2 13.0 / 10.2, 512^2
+    // 13.8 / 13.7, 1024^2 26.9 / 33.1)
+    if (path == CM_CG_AUTO)
+        path = (resident_ok && n <= 6144) ? CM_CG_RESIDENT
+                                          : ((long long)n * batches <= (1 << 17) ? CM_CG_FUSED : CM_CG_MULTIKERNEL);
     if (path == CM_CG_RESIDENT) {
         CM_REQUIRE(resident_ok, CM_E_UNSUPPORTED, "resident CG needs one rhs and n <= 16384");
         if (dtype == CM_F32)

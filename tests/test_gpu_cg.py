@@ -162,10 +162,11 @@ def test_large_poisson_2048_residual(gpu):
 
 
 def test_auto_path_choice(gpu):
-    """AUTO: one workgroup for a small single system, the multi-kernel
-    launches for larger ones and for several right-hand sides."""
+    """AUTO: one workgroup for a small single system, the two-launch form up
+    to 2^17 unknowns (also for several right-hand sides), the three-launch
+    form beyond."""
     cm = gpu
-    for g, expect in ((32, 2), (64, 2), (96, 1), (200, 1)):
+    for g, expect in ((32, 2), (64, 2), (96, 3), (200, 3), (362, 3), (363, 1), (600, 1)):
         A, JA, IA, V, n = _poisson(cm, g)
         cg = cm.ConjugateGradient(A).setTolerance(1e-4)
         cg.solve(cm.DeviceArray.from_numpy(np.ones(n, np.float32)))
@@ -173,7 +174,11 @@ def test_auto_path_choice(gpu):
     A, JA, IA, V, n = _poisson(cm, 16)
     cg = cm.ConjugateGradient(A).setTolerance(1e-4)
     cg.solve(cm.DeviceArray.from_numpy(np.ones((2, n, 1), np.float32)))
-    assert cg.last_stats["path"] == 1
+    assert cg.last_stats["path"] == 3
+    A, JA, IA, V, n = _poisson(cm, 300)
+    cg = cm.ConjugateGradient(A).setTolerance(1e-4)
+    cg.solve(cm.DeviceArray.from_numpy(np.ones((2, n, 1), np.float32)))
+    assert cg.last_stats["path"] == 1  # 2 x 90,000 unknowns
 
 
 def test_fused_and_three_launch_agree(gpu):
