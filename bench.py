@@ -322,6 +322,30 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
                               "frac": round(32 * N64 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
         log(key, out[key])
     del da, db, dc, do_
+    # ---- broadcasts over a 4096 x 4096 x 4 FP32 tensor (strided leaves read
+    # at the element's coordinates), generated kernel against the interpreter
+    Rb, Cb, Bb = 4096, 4096, 4
+    nb = Rb * Cb * Bb
+    TA = cm.DeviceArray.from_numpy(rng.standard_normal(nb, dtype=np.float32), rows=Rb, cols=Cb, batches=Bb, ctx=ctx)
+    TV = cm.DeviceArray.from_numpy(rng.standard_normal(Rb, dtype=np.float32), rows=Rb, cols=1, batches=1, ctx=ctx)
+    TW = cm.DeviceArray.from_numpy(rng.standard_normal(Cb, dtype=np.float32), rows=1, cols=Cb, batches=1, ctx=ctx)
+    TO = cm.DeviceArray(Rb, Cb, Bb, dtype=np.float32, ctx=ctx)
+    for key, jm in (("broadcast_generated_4096x4096x4", 2), ("broadcast_interpreter_4096x4096x4", 0)):
+        old_mode = cm.jit_mode(jm)
+        res = {}
+        for label, e in (("A*colvec", TA.expr() * TV.expr()), ("A*rowvec+colvec", TA.expr() * TW.expr() + TV.expr())):
+            cm.assign(TO, e, ctx=ctx)
+            ctx.sync()
+            timer.start(stream)
+            for _ in range(10):
+                cm.assign(TO, e, ctx=ctx)
+            ms = timer.stop_ms(stream) / 10
+            res[label] = {"us": round(ms * 1000, 1), "GBps": round(8 * nb / (ms * 1e-3) / 1e9, 1),
+                          "frac": round(8 * nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        cm.jit_mode(old_mode)
+        out[key] = res
+        log(key, res)
+    del TA, TV, TW, TO
     # ---- axis sums of 1024 x 1024 x 64 (two rotating tensors of 256 MiB)
     R, Cc, B = 1024, 1024, 64
     ts = [cm.DeviceArray.from_numpy(rng.standard_normal(R * Cc * B, dtype=np.float32), rows=R, cols=Cc, batches=B,

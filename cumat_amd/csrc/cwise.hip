@@ -336,15 +336,47 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
     }
     std::string names[CM_CWISE_MAX_LEAVES];
     const char* namep[CM_CWISE_MAX_LEAVES];
+    // strided leaves (dense route only): neither a broadcast scalar nor in
+    // the destination's layout; element e reads the leaf at the coordinates
+    // (r, c, b) of e, with the shape and strides compiled in as constants
+    std::string strided_loads;
+    bool any_strided = false;
     for (int l = 0; l < P.nleaves; ++l) {
         const cm_leaf& L = P.leaves[l];
         if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL && !(wide && L.dtype == CM_F64))
             return JIT_DECLINED;
         const bool scalar = (P.leaf_scalar >> l) & 1;
         const bool gathered = (gmask >> l) & 1;
+        const bool strided = !gmask && !scalar && !((P.leaf_direct >> l) & 1);
         const bool al = L.dtype != CM_BOOL && ((uintptr_t)L.ptr & 15) == 0;
-        snprintf(buf, sizeof(buf), "L%d%c%c|", L.dtype, gathered ? 'g' : (scalar ? 's' : 'd'), al ? 'a' : 'u');
+        snprintf(buf, sizeof(buf), "L%d%c%c|", L.dtype, gathered ? 'g' : (scalar ? 's' : (strided ? 'x' : 'd')),
+                 al ? 'a' : 'u');
         key += buf;
+        if (strided) {
+            if (L.kind != CM_LEAF_STRIDED || !P.dst_direct) return JIT_DECLINED;
+            any_strided = true;
+            snprintf(buf, sizeof(buf), "S%lld,%lld,%lld|", (long long)L.sr, (long long)L.sc, (long long)L.sb);
+            key += buf;
+            snprintf(buf, sizeof(buf), "v%d[k]", l);
+            names[l] = buf;
+            decls += "        R v" + std::to_string(l) + "[8];\n";
+            std::string off;
+            const long long st[3] = {L.sr, L.sc, L.sb};
+            const char* cv[3] = {"cr_", "cc_", "cb_"};
+            for (int a = 0; a < 3; ++a)
+                if (st[a] != 0) {
+                    char t[96];
+                    snprintf(t, sizeof(t), "%s(long long)%s[k] * %lldll", off.empty() ? "" : " + ", cv[a], st[a]);
+                    off += t;
+                }
+            if (off.empty()) off = "0";
+            char ld[256];
+            snprintf(ld, sizeof(ld), jit_raw_load(L.dtype), l, off.c_str());
+            strided_loads += std::string("#pragma unroll\n        for (int k = 0; k < 8; ++k) v") +
+                             std::to_string(l) + "[k] = " + ld + ";\n";
+            namep[l] = names[l].c_str();
+            continue;
+        }
         if (gathered) {
             snprintf(buf, sizeof(buf), "v%d[k]", l);
             names[l] = buf;
@@ -406,6 +438,29 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
     }
     std::string prog;
     if (!jit::program_body(P, namep, prog)) return JIT_DECLINED;
+    // coordinates of a thread's 8 elements (the interpreter's decompose(),
+    // with the shape as constants: divisions become multiplications)
+    std::string coords, shape;
+    if (any_strided) {
+        const bool narrow = P.n <= 0xffffffffll;
+        snprintf(buf, sizeof(buf), "sh%lld,%lld,%lld,%d|", (long long)P.rows, (long long)P.cols, (long long)P.batches,
+                 P.row_order);
+        key += buf;
+        snprintf(buf, sizeof(buf),
+                 "typedef %s IX;\nconstexpr IX SH_R = %lld, SH_C = %lld, SH_RC = %lld;\n",
+                 narrow ? "unsigned" : "unsigned long long", (long long)P.rows, (long long)P.cols,
+                 (long long)P.rows * P.cols);
+        shape = buf;
+        coords = "        IX cr_[8], cc_[8], cb_[8];\n"
+                 "#pragma unroll\n"
+                 "        for (int k = 0; k < 8; ++k) {\n"
+                 "            const IX e_ = (IX)min(EL(e0, k), n - 1);\n"
+                 "            cb_[k] = e_ / SH_RC;\n"
+                 "            const IX w_ = e_ - cb_[k] * SH_RC;\n";
+        coords += P.row_order ? "            cr_[k] = w_ / SH_C;\n            cc_[k] = w_ - cr_[k] * SH_C;\n"
+                              : "            cc_[k] = w_ / SH_R;\n            cr_[k] = w_ - cc_[k] * SH_R;\n";
+        coords += "        }\n";
+    }
     // store of element k at e (the interpreter's store_elem)
     std::string store_one;
     if (dst_dt == CM_F64) {
@@ -468,7 +523,7 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
     }
     snprintf(buf, sizeof(buf), "constexpr int MODE = %d;\n", P.mode);
     body = std::string("typedef unsigned int u4v __attribute__((ext_vector_type(4)));\n"
-                       "typedef unsigned long long u2v __attribute__((ext_vector_type(2)));\n") + buf +
+                       "typedef unsigned long long u2v __attribute__((ext_vector_type(2)));\n") + shape + buf +
            "struct JitCwiseArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) + "]; long long gsc[" +
            std::to_string(CM_CWISE_MAX_LEAVES) + "]; const int* idx; void* dst; long long n; };\n"
            "#define EL(e0, k) ((e0) + ((k) >> 2) * 256 + ((k) & 3))\n"
@@ -488,6 +543,7 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
            "        } else {\n" +
            loads_tail +
            "        }\n" +
+           coords + strided_loads +
            gathers +
            "        R res[8];\n"
            "#pragma unroll\n"
@@ -712,7 +768,8 @@ static int cwise_dense(cm_ctx_t ctx, const cm_leaf* leaves, int nleaves, const c
         int k = 0;
         if (cm::match_lincomb(P, A, k)) return cm::lincomb_run(ctx, A, k, (float*)dst, n);
     }
-    if (flat && P.dst_direct && cm::jit::want(n)) {
+    // (also programs with strided or broadcast leaves, coordinates compiled in)
+    if (P.dst_direct && cm::jit::want(n)) {
         const int s = cm::jit_cwise_flat(ctx, P, 0, nullptr, wide);
         if (s != cm::JIT_DECLINED) return s;
     }

@@ -558,3 +558,74 @@ def test_reduce_expression_non_flat_stays_on_interpreter(gpu, data):
     assert cm.counters()["JitLaunch"] == 0
     a = np.abs(data["f"].arr[:R * Cc].reshape(R, Cc)).astype(np.float64).sum(axis=0)
     np.testing.assert_allclose(got, a, rtol=1e-5)
+
+
+# ------------------------------------------- programs with strided leaves
+def _st_same(cm, dst_factory, e):
+    """dst = e by the interpreter and by the generated kernel (coordinates
+    compiled in), bit for bit; returns the logical (batches, rows, cols)
+    result."""
+    outs = []
+    for jit in (0, 2):
+        dst = dst_factory()
+        old = cm.jit_mode(jit)
+        try:
+            cm.reset_counters()
+            cm.assign(dst, e)
+            launched = cm.counters()["JitLaunch"]
+        finally:
+            cm.jit_mode(old)
+        assert launched == (1 if jit else 0), (jit, launched)
+        outs.append(dst.numpy(logical=True))
+    assert outs[0].tobytes() == outs[1].tobytes()
+    return outs[1]
+
+
+@pytest.mark.parametrize("shape", [(512, 300, 3), (3, 2000, 5), (1000, 3, 7), (1, 5000, 2), (777, 1, 9)])
+def test_strided_broadcast_programs(gpu, shape):
+    """Column, row and batch broadcasts, a row-major leaf in a column-major
+    destination and a row-major destination, odd leading dimensions: the
+    generated kernel reads each leaf at the element's coordinates."""
+    cm = gpu
+    R, C, B = shape
+    rng = np.random.default_rng(R * 7 + C)
+    a = rng.standard_normal((B, R, C)).astype(np.float32)
+    ar = rng.standard_normal((B, R, C)).astype(np.float32)
+    col = rng.standard_normal((1, R, 1)).astype(np.float32)
+    row = rng.standard_normal((1, 1, C)).astype(np.float32)
+    mat = rng.standard_normal((1, R, C)).astype(np.float32)
+    A = cm.DeviceArray.from_numpy(a)
+    Ar = cm.DeviceArray.from_numpy(ar, row_major=True)
+    V = cm.DeviceArray.from_numpy(col)
+    W = cm.DeviceArray.from_numpy(row)
+    M = cm.DeviceArray.from_numpy(mat)
+    colmajor = lambda: cm.DeviceArray(R, C, B, dtype=np.float32)
+    rowmajor = lambda: cm.DeviceArray(R, C, B, dtype=np.float32, row_major=True)
+    got = _st_same(cm, colmajor, A.expr() * V.expr())
+    assert np.array_equal(got, a * col)
+    got = _st_same(cm, colmajor, A.expr() * W.expr() + V.expr())
+    assert np.array_equal(got, a * row + col)
+    got = _st_same(cm, colmajor, A.expr() - M.expr())
+    assert np.array_equal(got, a - mat)
+    got = _st_same(cm, colmajor, A.expr() + Ar.expr())
+    assert np.array_equal(got, a + ar)
+    got = _st_same(cm, rowmajor, A.expr() * V.expr() + Ar.expr())
+    assert np.array_equal(got, a * col + ar)
+
+
+def test_strided_programs_f64_and_int(gpu):
+    cm = gpu
+    R, C, B = 300, 257, 4
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal((B, R, C))
+    row = rng.standard_normal((1, 1, C))
+    A = cm.DeviceArray.from_numpy(a)
+    W = cm.DeviceArray.from_numpy(row)
+    got = _st_same(cm, lambda: cm.DeviceArray(R, C, B, dtype=np.float64), A.expr() * W.expr() - 0.5)
+    assert np.array_equal(got, a * row - 0.5)
+    ai = rng.integers(-1000, 1000, (B, R, C)).astype(np.int32)
+    ci = rng.integers(1, 50, (1, R, 1)).astype(np.int32)
+    Ai = cm.DeviceArray.from_numpy(ai, row_major=True)
+    Ci = cm.DeviceArray.from_numpy(ci)
+    got = _st_same(cm, lambda: cm.DeviceArray(R, C, B, dtype=np.int32), Ai.expr() % Ci.expr())
+    assert np.array_equal(got, np.fmod(ai, ci))
