@@ -84,7 +84,7 @@ __device__ __forceinline__ Coords rx_coords(const RxShape& S, long long s, long 
 template <typename Op, typename T, typename S, bool W, int NL, bool FLAT>
 __global__ __launch_bounds__(RX_BS) void reduce_expr_kernel(const CwProgram P, const RxShape SH, int nchunk,
                                                             long long chunk, T* __restrict__ partials,
-                                                            S* __restrict__ out)
+                                                            S* __restrict__ out, int sp)
 {
     constexpr int V = CwVec<W>::v;
     __shared__ T lds[RX_BS / 64];
@@ -93,11 +93,11 @@ __global__ __launch_bounds__(RX_BS) void reduce_expr_kernel(const CwProgram P, c
     const long long beg = c * chunk;
     const long long end = min(SH.L, beg + chunk);
     T acc = RxIdentity<Op, T>::v();
-    // element of slot k in the pass starting at p0: 32-bit FLAT reductions
-    // give a thread two groups of 4 that lie 256 elements apart (one 16-byte
-    // load of a wavefront then covers 1 KB contiguously); otherwise V
-    // consecutive elements
-    constexpr bool SPREAD = FLAT && !W;
+    // element of slot k in the pass starting at p0: 32-bit reductions with a
+    // leaf in the virtual order (sp) give a thread two groups of 4 that lie
+    // 256 elements apart (one 16-byte load of a wavefront then covers 1 KB
+    // of that leaf); others V consecutive elements
+    const bool SPREAD = (FLAT || sp) && !W;
     const long long t0 = SPREAD ? (long long)(threadIdx.x >> 6) * 512 + (threadIdx.x & 63) * 4
                                 : (long long)threadIdx.x * V;
     for (long long p0 = beg; p0 < end; p0 += (long long)RX_BS * V) {
@@ -184,6 +184,30 @@ static bool rx_flat(CwProgram& P, const RxShape& SH)
         else if (!flat) return false;
     }
     return true;
+}
+
+// Whether some leaf of P that is not a broadcast scalar lies in the virtual
+// order of SH.
+static bool rx_any_virtual(const CwProgram& P, const RxShape& SH)
+{
+    const long long dim[3] = {P.rows, P.cols, P.batches};
+    long long es[3] = {0, 0, 0};
+    long long st = 1;
+    for (int i = 0; i < SH.nr; ++i) es[SH.raxis[i]] = st, st *= SH.rdim[i];
+    for (int i = 0; i < SH.nk; ++i) es[SH.kaxis[i]] = st, st *= SH.kdim[i];
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        if (L.kind != CM_LEAF_STRIDED) continue;
+        const long long ls[3] = {L.sr, L.sc, L.sb};
+        bool scalar = true, virt = true;
+        for (int a = 0; a < 3; ++a) {
+            if (dim[a] <= 1) continue;
+            scalar = scalar && ls[a] == 0;
+            virt = virt && ls[a] == es[a];
+        }
+        if (virt && !scalar) return true;
+    }
+    return false;
 }
 
 constexpr int RX_JIT_DECLINED = 1;
@@ -323,16 +347,178 @@ int jit_reduce_flat_source(const CwProgram& P, std::string& key, std::string& bo
     return CM_OK;
 }
 
+// Source and key of the generated reduction for programs with strided or
+// broadcast leaves (32-bit programs): the kernel of
+// reduce_expr_kernel<Op, T, S, false, NL, false> -- a thread's two groups of
+// 4 elements 256 apart, the same fold order and block reduction -- with the
+// program inlined. Leaves in the virtual order are loaded as in the flat
+// kernel (16-byte loads where a thread's groups are whole and aligned); the
+// others at the element's coordinates, with the virtual order's shape and
+// their strides compiled in as constants (rx_coords() without run-time
+// divisions).
+template <typename Op, typename T, typename S>
+int jit_reduce_strided_source(const CwProgram& P, const RxShape& SH, std::string& key, std::string& body,
+                              bool spread)
+{
+    if (!std::is_same<T, float>::value && !std::is_same<T, int>::value) return RX_JIT_DECLINED;
+    char buf[256];
+    const char* tname = std::is_same<T, float>::value ? "float" : "int";
+    const char* sname = std::is_same<S, float>::value ? "float" : (std::is_same<S, int>::value ? "int" : "unsigned char");
+    snprintf(buf, sizeof(buf), "rxst32%c|%s|%s|%s|k", spread ? 's' : 'c', RxOpName<Op>::get(), tname, sname);
+    key = buf;
+    const bool narrow = (double)SH.L * (double)SH.nseg < 4294967295.0;
+    const char* sfx = narrow ? "u" : "ull";
+    const char* ix = narrow ? "unsigned" : "unsigned long long";
+    std::string coords = std::string("            ") + ix + " s_ = (" + ix + ")s, j_ = (" + ix +
+                         ")(EL(j0, k) < end ? EL(j0, k) : beg);\n"
+                         "            long long q3[3] = {0, 0, 0};\n";
+    for (int a = 0; a < SH.nk; ++a) {
+        snprintf(buf, sizeof(buf), "%lld@%d,", SH.kdim[a], SH.kaxis[a]);
+        key += buf;
+        snprintf(buf, sizeof(buf), "            { const auto q = s_ / %lld%s; q3[%d] = (long long)(s_ - q * %lld%s); s_ = q; }\n",
+                 SH.kdim[a], sfx, SH.kaxis[a], SH.kdim[a], sfx);
+        coords += buf;
+    }
+    key += "r";
+    for (int a = 0; a < SH.nr; ++a) {
+        snprintf(buf, sizeof(buf), "%lld@%d,", SH.rdim[a], SH.raxis[a]);
+        key += buf;
+        snprintf(buf, sizeof(buf), "            { const auto q = j_ / %lld%s; q3[%d] = (long long)(j_ - q * %lld%s); j_ = q; }\n",
+                 SH.rdim[a], sfx, SH.raxis[a], SH.rdim[a], sfx);
+        coords += buf;
+    }
+    key += "|";
+    // element strides of the virtual order (as rx_flat)
+    const long long dim[3] = {P.rows, P.cols, P.batches};
+    long long es[3] = {0, 0, 0};
+    long long stv = 1;
+    for (int i = 0; i < SH.nr; ++i) es[SH.raxis[i]] = stv, stv *= SH.rdim[i];
+    for (int i = 0; i < SH.nk; ++i) es[SH.kaxis[i]] = stv, stv *= SH.kdim[i];
+    std::string decls, vload, sload, xload, scalars, names[CM_CWISE_MAX_LEAVES];
+    const char* namep[CM_CWISE_MAX_LEAVES];
+    bool all_al = true, any_v = false;
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL) return RX_JIT_DECLINED;
+        if (L.kind != CM_LEAF_STRIDED) return RX_JIT_DECLINED;
+        const long long ls[3] = {L.sr, L.sc, L.sb};
+        bool scalar = true, virt = true;
+        for (int a = 0; a < 3; ++a) {
+            if (dim[a] <= 1) continue;
+            scalar = scalar && ls[a] == 0;
+            virt = virt && ls[a] == es[a];
+        }
+        const bool al = L.dtype != CM_BOOL && ((uintptr_t)L.ptr & 15) == 0;
+        snprintf(buf, sizeof(buf), "L%d%c%c%lld,%lld,%lld|", L.dtype, scalar ? 's' : (virt ? 'v' : 'x'), al ? 'a' : 'u',
+                 ls[0], ls[1], ls[2]);
+        key += buf;
+        const std::string lsn = std::to_string(l);
+        const char* rd = L.dtype == CM_BOOL ? "(R)((const unsigned char*)A.leaf[" : "((const uint32_t*)A.leaf[";
+        if (scalar) {
+            names[l] = "c" + lsn;
+            scalars += "    const R c" + lsn + " = " + rd + lsn + "])[0];\n";
+        } else if (virt) {
+            any_v = true;
+            names[l] = "v" + lsn + "[k]";
+            decls += "        R v" + lsn + "[8];\n";
+            if (!al) all_al = false;
+            vload += "            { const u4v a = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[" + lsn +
+                     "] + e0)); const u4v b = __builtin_nontemporal_load((const u4v*)((const uint32_t*)A.leaf[" + lsn +
+                     "] + e0 + 256));\n              v" + lsn + "[0] = a.x; v" + lsn + "[1] = a.y; v" + lsn + "[2] = a.z; v" +
+                     lsn + "[3] = a.w; v" + lsn + "[4] = b.x; v" + lsn + "[5] = b.y; v" + lsn + "[6] = b.z; v" + lsn +
+                     "[7] = b.w; }\n";
+            sload += "#pragma unroll\n            for (int k = 0; k < 8; ++k) v" + lsn + "[k] = " + rd + lsn +
+                     "])[s * A.L + (EL(j0, k) < end ? EL(j0, k) : beg)];\n";
+        } else {
+            names[l] = "v" + lsn + "[k]";
+            decls += "        R v" + lsn + "[8];\n";
+            std::string off;
+            for (int a = 0; a < 3; ++a)
+                if (ls[a] != 0 && dim[a] > 1) {
+                    snprintf(buf, sizeof(buf), "%sq3[%d] * %lldll", off.empty() ? "" : " + ", a, ls[a]);
+                    off += buf;
+                }
+            if (off.empty()) off = "0";
+            xload += "            v" + lsn + "[k] = " + rd + lsn + "])[" + off + "];\n";
+        }
+        namep[l] = names[l].c_str();
+    }
+    for (int i = 0; i < P.ncode; ++i) {
+        const cm_instr& I = P.code[i];
+        uint64_t fb;
+        memcpy(&fb, &I.fimm, 8);
+        snprintf(buf, sizeof(buf), "%d,%d,%d,%d,%d,%d,%llx,%llx;", I.opcode, I.dtype, I.fn, I.src, I.arg, I.flags,
+                 (unsigned long long)fb, (unsigned long long)I.iimm);
+        key += buf;
+    }
+    std::string prog;
+    if (!jit::program_body(P, namep, prog)) return RX_JIT_DECLINED;
+    const char* get = std::is_same<T, float>::value ? "getf" : "geti";
+    std::string vpart;
+    if (any_v)
+        vpart = std::string("        const long long e0 = s * A.L + j0;\n") +
+                (all_al && spread ? "        if (j0 + 259 < end && (e0 & 3) == 0) {\n" + vload + "        } else {\n"
+                                  : std::string("        {\n")) +
+                sload + "        }\n";
+    std::string xpart;
+    if (!xload.empty())
+        xpart = "#pragma unroll\n"
+                "        for (int k = 0; k < 8; ++k) {\n" + coords + xload + "        }\n";
+    body = std::string("typedef unsigned int u4v __attribute__((ext_vector_type(4)));\n") + "typedef " + tname +
+           " TT;\ntypedef " + sname + " SS;\ntypedef " + RxOpName<Op>::get() +
+           " OP;\n"
+           "struct JitRxArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) +
+           "]; void* partials; void* out; long long L, chunk; int nchunk, pad; };\n"
+           + (spread ? "#define EL(j0, k) ((j0) + ((k) >> 2) * 256 + ((k) & 3))\n" : "#define EL(j0, k) ((j0) + (k))\n") +
+           "extern \"C\" __global__ __launch_bounds__(256) void cm_jit_reduce(const JitRxArgs A)\n{\n"
+           "    typedef uint32_t R;\n"
+           "    __shared__ TT lds[4];\n"
+           "    const long long s = blockIdx.x / A.nchunk;\n"
+           "    const int c = blockIdx.x % A.nchunk;\n"
+           "    const long long beg = c * A.chunk;\n"
+           "    const long long end = min(A.L, beg + A.chunk);\n" +
+           scalars +
+           "    TT acc = " + rx_identity_src<Op, T>() + ";\n" +
+           (spread ? "    const long long t0 = (long long)(threadIdx.x >> 6) * 512 + (threadIdx.x & 63) * 4;\n"
+                   : "    const long long t0 = (long long)threadIdx.x * 8;\n") +
+           "    for (long long p0 = beg; p0 < end; p0 += 256 * 8) {\n"
+           "        const long long j0 = p0 + t0;\n" + (spread ? "" : "        if (j0 >= end) break;\n") + decls +
+           vpart + xpart +
+           "#pragma unroll\n"
+           "        for (int k = 0; k < 8; ++k) {\n"
+           "            R acc_r = 0, t = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;\n"
+           "            (void)t; (void)s0; (void)s1; (void)s2; (void)s3;\n"
+           "#define acc acc_r\n" +
+           prog +
+           "#undef acc\n"
+           "            if (EL(j0, k) < end) acc = OP::apply(acc, (TT)" + get + "(acc_r));\n"
+           "        }\n"
+           "    }\n"
+           "    acc = cm::block_reduce<OP, 256>(acc, lds);\n"
+           "    if (threadIdx.x == 0) {\n"
+           "        if (A.nchunk == 1) ((SS*)A.out)[s] = (SS)acc;\n"
+           "        else ((TT*)A.partials)[blockIdx.x] = acc;\n"
+           "    }\n"
+           "}\n";
+    return CM_OK;
+}
+
 // instances with external linkage, so the generator can be exercised on a
 // host without a GPU (source generation and compilation only)
 template int jit_reduce_flat_source<OpSum, float, float>(const CwProgram&, std::string&, std::string&);
 template int jit_reduce_flat_source<OpAll, int, unsigned char>(const CwProgram&, std::string&, std::string&);
+template int jit_reduce_strided_source<OpSum, float, float>(const CwProgram&, const RxShape&, std::string&,
+                                                            std::string&, bool);
 
 template <typename Op, typename T, typename S, bool W, int NL>
 static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P0, const RxShape& SH, S* out)
 {
     CwProgram P = P0;
     const bool flat = rx_flat(P, SH);
+    // the spread element mapping pays where a leaf lies in the virtual order
+    // (its loads are then 16 bytes wide); a program whose leaves are all
+    // strided keeps consecutive elements per thread
+    const bool spread = flat || rx_any_virtual(P0, SH);
     constexpr int V = CwVec<W>::v;
     const long long cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
     // chunks of at least one pass of the block; about 8 blocks per CU overall
@@ -350,10 +536,12 @@ static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P0, const RxShape& 
     if (nchunk > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * (size_t)blocks, &part));
     bool done = false;
     if constexpr (!W) {
-        if (flat && jit::want(SH.L * SH.nseg)) {
+        if (jit::want(SH.L * SH.nseg)) {
             std::string key, body;
             jit::Kernel k;
-            if (jit_reduce_flat_source<Op, T, S>(P, key, body) == CM_OK && jit::get(key, body, "cm_jit_reduce", &k) == 0) {
+            const int gen = flat ? jit_reduce_flat_source<Op, T, S>(P, key, body)
+                                 : jit_reduce_strided_source<Op, T, S>(P, SH, key, body, spread);
+            if (gen == CM_OK && jit::get(key, body, "cm_jit_reduce", &k) == 0) {
                 JitRxArgs A;
                 memset(&A, 0, sizeof(A));
                 for (int l = 0; l < P.nleaves; ++l) A.leaf[l] = P.leaves[l].ptr;
@@ -372,10 +560,10 @@ static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P0, const RxShape& 
     if (!done) {
         if (flat)
             reduce_expr_kernel<Op, T, S, W, NL, true><<<(unsigned)blocks, RX_BS, 0, ctx->stream>>>(
-                P, SH, (int)nchunk, chunk, (T*)part, out);
+                P, SH, (int)nchunk, chunk, (T*)part, out, 1);
         else
             reduce_expr_kernel<Op, T, S, W, NL, false><<<(unsigned)blocks, RX_BS, 0, ctx->stream>>>(
-                P, SH, (int)nchunk, chunk, (T*)part, out);
+                P, SH, (int)nchunk, chunk, (T*)part, out, spread ? 1 : 0);
         CM_LAUNCH_CHECK();
     }
     if (nchunk > 1) {

@@ -545,19 +545,58 @@ def test_reduce_expression_generated_int_and_bool(gpu, data, op):
         assert int(got[0]) == int(exp)
 
 
-def test_reduce_expression_non_flat_stays_on_interpreter(gpu, data):
+def test_reduce_expression_non_flat_generated(gpu, data):
     """A row-major leaf reduced along rows walks a virtual order that is not
-    its memory order: the interpreter runs it (no generated kernel)."""
+    its memory order: the generated reduction reads it at the element's
+    coordinates (shape and strides compiled in), with the interpreter's
+    elements per thread and fold order."""
     from cumat_amd import _capi as capi
     cm = gpu
     R, Cc = 1024, 1000
     code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0), _instr(capi.OP_UNARY, capi.F32, capi.UN_ABS)]
     leaves = [(data["f"], (Cc, 1, R * Cc))]
+    ref = _rx(cm, leaves, code, capi.F32, capi.RED_SUM, 1, (R, Cc, 1), np.float32, jit=0)
     cm.reset_counters()
     got = _rx(cm, leaves, code, capi.F32, capi.RED_SUM, 1, (R, Cc, 1), np.float32, jit=2)
-    assert cm.counters()["JitLaunch"] == 0
+    assert cm.counters()["JitLaunch"] == 1
+    assert ref.tobytes() == got.tobytes()
     a = np.abs(data["f"].arr[:R * Cc].reshape(R, Cc)).astype(np.float64).sum(axis=0)
     np.testing.assert_allclose(got, a, rtol=1e-5)
+
+
+@pytest.mark.parametrize("axis", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("op", ["sum", "max"])
+def test_reduce_broadcast_expressions_generated(gpu, data, axis, op):
+    """(A * colvec - rowvec) over every axis subset of a 300 x 257 x 7
+    tensor, and an int32 program with a batch-broadcast leaf: generated and
+    interpreted reductions give the same bits."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    ops = {"sum": capi.RED_SUM, "max": capi.RED_MAX}
+    R, Cc, B = 300, 257, 7
+    st = (1, R, R * Cc)
+    code = [_instr(capi.OP_LOAD, capi.F32, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_MUL, capi.SRC_LEAF, 1),
+            _instr(capi.OP_BINARY, capi.F32, capi.BIN_SUB, capi.SRC_LEAF, 2)]
+    leaves = [(data["f"], st), (data["g"], (1, 0, 0)), (data["g"], (0, 1, 0))]
+    ref = _rx(cm, leaves, code, capi.F32, ops[op], axis, (R, Cc, B), np.float32, jit=0)
+    cm.reset_counters()
+    got = _rx(cm, leaves, code, capi.F32, ops[op], axis, (R, Cc, B), np.float32, jit=2)
+    assert cm.counters()["JitLaunch"] == 1
+    assert ref.tobytes() == got.tobytes()
+    codei = [_instr(capi.OP_LOAD, capi.I32, src=capi.SRC_LEAF, arg=0),
+             _instr(capi.OP_BINARY, capi.I32, capi.BIN_ADD, capi.SRC_LEAF, 1)]
+    leavesi = [(data["i"], (Cc, 1, R * Cc)), (data["j"], (1, R, 0))]
+    refi = _rx(cm, leavesi, codei, capi.I32, ops[op], axis, (R, Cc, B), np.int32, jit=0)
+    goti = _rx(cm, leavesi, codei, capi.I32, ops[op], axis, (R, Cc, B), np.int32, jit=2)
+    assert refi.tobytes() == goti.tobytes()
+    if op == "sum":
+        i3 = data["i"].arr[:R * Cc * B].astype(np.int64).reshape(B, R, Cc)
+        j2 = data["j"].arr[:R * Cc].astype(np.int64).reshape(Cc, R).T
+        tot = i3 + j2[None]
+        red = tuple(a for a, bit in ((1, 1), (2, 2), (0, 4)) if axis & bit)
+        exp = tot.sum(axis=red)
+        assert np.array_equal(np.sort(goti.astype(np.int64)), np.sort((exp.reshape(-1) + 2**31) % 2**32 - 2**31))
 
 
 # ------------------------------------------- programs with strided leaves
