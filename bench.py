@@ -345,6 +345,24 @@ def run_extras(cm, capi, ctx, stream, timer, rng):
         cm.jit_mode(old_mode)
         out[key] = res
         log(key, res)
+    # reductions of broadcast expressions over the same tensor (4 bytes read
+    # per element)
+    for key, jm in (("reduce_broadcast_generated_4096x4096x4", 2), ("reduce_broadcast_interpreter_4096x4096x4", 0)):
+        old_mode = cm.jit_mode(jm)
+        res = {}
+        for label, fn in (("(A*colvec).sum()", lambda: cm.reduce(TA.expr() * TV.expr(), "sum", 7, ctx=ctx)),
+                          ("(A-rowvec).sum<Row>()", lambda: cm.reduce(TA.expr() - TW.expr(), "sum", 1, ctx=ctx))):
+            fn()
+            ctx.sync()
+            timer.start(stream)
+            for _ in range(10):
+                fn()
+            ms = timer.stop_ms(stream) / 10
+            res[label] = {"us": round(ms * 1000, 1), "GBps": round(4 * nb / (ms * 1e-3) / 1e9, 1),
+                          "frac": round(4 * nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        cm.jit_mode(old_mode)
+        out[key] = res
+        log(key, res)
     del TA, TV, TW, TO
     # ---- axis sums of 1024 x 1024 x 64 (two rotating tensors of 256 MiB)
     R, Cc, B = 1024, 1024, 64
