@@ -567,10 +567,52 @@ int jit_cwise_flat_source(const CwProgram& P, std::string& key, std::string& bod
 static int jit_cwise_flat(cm_ctx_t ctx, const CwProgram& P, unsigned gmask = 0, const int* idx = nullptr,
                           bool wide = false)
 {
-    std::string key, body;
-    if (jit_cwise_flat_source(P, key, body, gmask, idx, wide) != CM_OK) return JIT_DECLINED;
+    // everything jit_cwise_flat_source reads: flags, destination, shape,
+    // leaves (strides and 16-byte alignment, not addresses) and the code
+    std::string sig("cw");
+    jit::sig_add(sig, (int)wide);
+    jit::sig_add(sig, gmask);
+    jit::sig_add(sig, (int)(((uintptr_t)idx & 15) == 0));
+    jit::sig_add(sig, P.dst_dtype);
+    jit::sig_add(sig, (int)(((uintptr_t)P.dst & 15) == 0));
+    jit::sig_add(sig, P.mode);
+    jit::sig_add(sig, P.dst_direct);
+    jit::sig_add(sig, P.row_order);
+    jit::sig_add(sig, P.rows);
+    jit::sig_add(sig, P.cols);
+    jit::sig_add(sig, P.batches);
+    jit::sig_add(sig, (int)(P.n <= 0xffffffffll));
+    jit::sig_add(sig, P.leaf_scalar);
+    jit::sig_add(sig, P.leaf_direct);
+    jit::sig_add(sig, P.nleaves);
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        jit::sig_add(sig, L.dtype);
+        jit::sig_add(sig, L.kind);
+        jit::sig_add(sig, L.sr);
+        jit::sig_add(sig, L.sc);
+        jit::sig_add(sig, L.sb);
+        jit::sig_add(sig, (int)(((uintptr_t)L.ptr & 15) == 0));
+    }
+    jit::sig_add(sig, P.ncode);
+    for (int i = 0; i < P.ncode; ++i) {
+        const cm_instr& I = P.code[i];
+        jit::sig_add(sig, I.opcode);
+        jit::sig_add(sig, I.dtype);
+        jit::sig_add(sig, I.fn);
+        jit::sig_add(sig, I.src);
+        jit::sig_add(sig, I.arg);
+        jit::sig_add(sig, I.flags);
+        jit::sig_add(sig, I.fimm);
+        jit::sig_add(sig, I.iimm);
+    }
     jit::Kernel k;
-    if (jit::get(key, body, "cm_jit_cwise", &k) != 0) return JIT_DECLINED;
+    if (jit::find_sig(sig, &k) != 0) {
+        std::string key, body;
+        if (jit_cwise_flat_source(P, key, body, gmask, idx, wide) != CM_OK) return JIT_DECLINED;
+        if (jit::get(key, body, "cm_jit_cwise", &k) != 0) return JIT_DECLINED;
+        jit::put_sig(sig, k);
+    }
     JitCwiseArgs A;
     memset(&A, 0, sizeof(A));
     for (int l = 0; l < P.nleaves; ++l) {

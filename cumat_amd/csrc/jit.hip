@@ -15,6 +15,8 @@
 // cache leaves the call on the interpreter.
 #include "jit.hpp"
 
+#include <unordered_map>
+
 #include <hip/hiprtc.h>
 
 #include <cinttypes>
@@ -308,6 +310,38 @@ int get(const std::string& key, const std::string& body, const char* name, Kerne
     g_cache[full_key] = k;  // failures are cached too: the interpreter keeps the program
     *out = k;
     return ok ? 0 : -1;
+}
+
+// front cache: a byte signature of everything a generator reads (more
+// specific than its key) -> the kernel, so a repeated evaluation skips
+// building the source and the key
+namespace {
+std::unordered_map<std::string, Kernel> g_sig;
+constexpr size_t SIG_MAX = 8192;
+}  // namespace
+
+int find_sig(const std::string& sig, Kernel* out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    std::string full = sig;
+    full.append((const char*)&dev, sizeof(dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_sig.find(full);
+    if (it == g_sig.end()) return -1;
+    *out = it->second;
+    return 0;
+}
+
+void put_sig(const std::string& sig, const Kernel& k)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || !k.fn) return;
+    std::string full = sig;
+    full.append((const char*)&dev, sizeof(dev));
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_sig.size() >= SIG_MAX) g_sig.clear();  // the kernels stay in the main cache
+    g_sig[full] = k;
 }
 
 int launch(const Kernel& k, unsigned grid, unsigned block, hipStream_t stream, const void* args, size_t args_bytes,

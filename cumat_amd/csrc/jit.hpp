@@ -39,6 +39,16 @@ int compile(const std::string& body, const std::string& arch, std::vector<char>&
 // Compiled kernel 'name' of 'body' (cached by key on the current device);
 // 0 on success.
 int get(const std::string& key, const std::string& body, const char* name, Kernel* out);
+// Front cache by a byte signature that determines the generated source
+// (find_sig: 0 and the kernel on a hit; put_sig after a successful get).
+int find_sig(const std::string& sig, Kernel* out);
+void put_sig(const std::string& sig, const Kernel& k);
+// Append the bytes of a value to a signature.
+template <typename V>
+inline void sig_add(std::string& sig, const V& v)
+{
+    sig.append(reinterpret_cast<const char*>(&v), sizeof(V));
+}
 // Launch (grid x grid_y blocks) with the kernel arguments packed in one
 // buffer; 0 on success.
 int launch(const Kernel& k, unsigned grid, unsigned block, hipStream_t stream, const void* args, size_t args_bytes,

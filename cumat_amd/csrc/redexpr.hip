@@ -537,11 +537,60 @@ static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P0, const RxShape& 
     bool done = false;
     if constexpr (!W) {
         if (jit::want(SH.L * SH.nseg)) {
-            std::string key, body;
+            // everything the generators read (the front cache's signature)
+            std::string sig("rx");
+            sig += RxOpName<Op>::get();
+            jit::sig_add(sig, (int)sizeof(T));
+            jit::sig_add(sig, (int)std::is_same<T, float>::value);
+            jit::sig_add(sig, (int)sizeof(S));
+            jit::sig_add(sig, (int)std::is_same<S, float>::value);
+            jit::sig_add(sig, (int)flat);
+            jit::sig_add(sig, (int)spread);
+            jit::sig_add(sig, SH.nk);
+            jit::sig_add(sig, SH.nr);
+            for (int a = 0; a < 3; ++a) {
+                jit::sig_add(sig, a < SH.nk ? SH.kdim[a] : 0ll);
+                jit::sig_add(sig, a < SH.nk ? SH.kaxis[a] : -1);
+                jit::sig_add(sig, a < SH.nr ? SH.rdim[a] : 0ll);
+                jit::sig_add(sig, a < SH.nr ? SH.raxis[a] : -1);
+            }
+            jit::sig_add(sig, (int)((double)SH.L * (double)SH.nseg < 4294967295.0));
+            jit::sig_add(sig, P.rows);
+            jit::sig_add(sig, P.cols);
+            jit::sig_add(sig, P.batches);
+            jit::sig_add(sig, P.leaf_scalar);
+            jit::sig_add(sig, P.nleaves);
+            for (int l = 0; l < P.nleaves; ++l) {
+                const cm_leaf& L = P.leaves[l];
+                jit::sig_add(sig, L.dtype);
+                jit::sig_add(sig, L.kind);
+                jit::sig_add(sig, L.sr);
+                jit::sig_add(sig, L.sc);
+                jit::sig_add(sig, L.sb);
+                jit::sig_add(sig, (int)(((uintptr_t)L.ptr & 15) == 0));
+            }
+            jit::sig_add(sig, P.ncode);
+            for (int i = 0; i < P.ncode; ++i) {
+                const cm_instr& I = P.code[i];
+                jit::sig_add(sig, I.opcode);
+                jit::sig_add(sig, I.dtype);
+                jit::sig_add(sig, I.fn);
+                jit::sig_add(sig, I.src);
+                jit::sig_add(sig, I.arg);
+                jit::sig_add(sig, I.flags);
+                jit::sig_add(sig, I.fimm);
+                jit::sig_add(sig, I.iimm);
+            }
             jit::Kernel k;
-            const int gen = flat ? jit_reduce_flat_source<Op, T, S>(P, key, body)
-                                 : jit_reduce_strided_source<Op, T, S>(P, SH, key, body, spread);
-            if (gen == CM_OK && jit::get(key, body, "cm_jit_reduce", &k) == 0) {
+            int gen = CM_OK;
+            if (jit::find_sig(sig, &k) != 0) {
+                std::string key, body;
+                gen = flat ? jit_reduce_flat_source<Op, T, S>(P, key, body)
+                           : jit_reduce_strided_source<Op, T, S>(P, SH, key, body, spread);
+                if (gen == CM_OK) gen = jit::get(key, body, "cm_jit_reduce", &k) == 0 ? CM_OK : RX_JIT_DECLINED;
+                if (gen == CM_OK) jit::put_sig(sig, k);
+            }
+            if (gen == CM_OK) {
                 JitRxArgs A;
                 memset(&A, 0, sizeof(A));
                 for (int l = 0; l < P.nleaves; ++l) A.leaf[l] = P.leaves[l].ptr;
