@@ -503,12 +503,157 @@ int jit_reduce_strided_source(const CwProgram& P, const RxShape& SH, std::string
     return CM_OK;
 }
 
+// Source and key of the generated reduction of a 64-bit program (FP64
+// leaves or instructions): the kernel of reduce_expr_kernel<Op, T, S, true,
+// NL, FLAT> -- 4 consecutive elements of the virtual order per thread and
+// pass in 64-bit registers, the same fold order and block reduction -- with
+// the program inlined; leaves in the virtual order are read at s*L + j, the
+// others at the element's coordinates (shape and strides compiled in).
+template <typename Op, typename T, typename S>
+int jit_reduce_wide_source(const CwProgram& P, const RxShape& SH, std::string& key, std::string& body)
+{
+    char buf[256];
+    const char* tname = std::is_same<T, float>::value ? "float" : (std::is_same<T, int>::value ? "int" : "double");
+    const char* sname = std::is_same<S, float>::value
+                            ? "float"
+                            : (std::is_same<S, int>::value ? "int" : (std::is_same<S, double>::value ? "double" : "unsigned char"));
+    snprintf(buf, sizeof(buf), "rxw64|%s|%s|%s|k", RxOpName<Op>::get(), tname, sname);
+    key = buf;
+    const bool narrow = (double)SH.L * (double)SH.nseg < 4294967295.0;
+    const char* sfx = narrow ? "u" : "ull";
+    const char* ix = narrow ? "unsigned" : "unsigned long long";
+    std::string coords = std::string("            ") + ix + " s_ = (" + ix + ")s, j_ = (" + ix +
+                         ")(j0 + k < end ? j0 + k : beg);\n"
+                         "            long long q3[3] = {0, 0, 0};\n";
+    for (int a = 0; a < SH.nk; ++a) {
+        snprintf(buf, sizeof(buf), "%lld@%d,", SH.kdim[a], SH.kaxis[a]);
+        key += buf;
+        snprintf(buf, sizeof(buf), "            { const auto q = s_ / %lld%s; q3[%d] = (long long)(s_ - q * %lld%s); s_ = q; }\n",
+                 SH.kdim[a], sfx, SH.kaxis[a], SH.kdim[a], sfx);
+        coords += buf;
+    }
+    key += "r";
+    for (int a = 0; a < SH.nr; ++a) {
+        snprintf(buf, sizeof(buf), "%lld@%d,", SH.rdim[a], SH.raxis[a]);
+        key += buf;
+        snprintf(buf, sizeof(buf), "            { const auto q = j_ / %lld%s; q3[%d] = (long long)(j_ - q * %lld%s); j_ = q; }\n",
+                 SH.rdim[a], sfx, SH.raxis[a], SH.rdim[a], sfx);
+        coords += buf;
+    }
+    key += "|";
+    const long long dim[3] = {P.rows, P.cols, P.batches};
+    long long es[3] = {0, 0, 0};
+    long long stv = 1;
+    for (int i = 0; i < SH.nr; ++i) es[SH.raxis[i]] = stv, stv *= SH.rdim[i];
+    for (int i = 0; i < SH.nk; ++i) es[SH.kaxis[i]] = stv, stv *= SH.kdim[i];
+    std::string decls, loads, scalars, names[CM_CWISE_MAX_LEAVES];
+    const char* namep[CM_CWISE_MAX_LEAVES];
+    bool any_x = false;
+    for (int l = 0; l < P.nleaves; ++l) {
+        const cm_leaf& L = P.leaves[l];
+        if (L.dtype != CM_F32 && L.dtype != CM_I32 && L.dtype != CM_BOOL && L.dtype != CM_F64) return RX_JIT_DECLINED;
+        if (L.kind != CM_LEAF_STRIDED) return RX_JIT_DECLINED;
+        const long long ls[3] = {L.sr, L.sc, L.sb};
+        bool scalar = true, virt = true;
+        for (int a = 0; a < 3; ++a) {
+            if (dim[a] <= 1) continue;
+            scalar = scalar && ls[a] == 0;
+            virt = virt && ls[a] == es[a];
+        }
+        snprintf(buf, sizeof(buf), "L%d%c%lld,%lld,%lld|", L.dtype, scalar ? 's' : (virt ? 'v' : 'x'), ls[0], ls[1], ls[2]);
+        key += buf;
+        const std::string lsn = std::to_string(l);
+        const char* rd = L.dtype == CM_BOOL ? "(R)((const unsigned char*)A.leaf["
+                                            : (L.dtype == CM_F64 ? "(R)((const uint64_t*)A.leaf[" : "(R)((const uint32_t*)A.leaf[");
+        if (scalar) {
+            names[l] = "c" + lsn;
+            scalars += "    const R c" + lsn + " = " + rd + lsn + "])[0];\n";
+        } else {
+            names[l] = "v" + lsn + "[k]";
+            decls += "        R v" + lsn + "[4];\n";
+            std::string off;
+            if (virt) {
+                off = "s * A.L + (j0 + k < end ? j0 + k : beg)";
+            } else {
+                any_x = true;
+                for (int a = 0; a < 3; ++a)
+                    if (ls[a] != 0 && dim[a] > 1) {
+                        snprintf(buf, sizeof(buf), "%sq3[%d] * %lldll", off.empty() ? "" : " + ", a, ls[a]);
+                        off += buf;
+                    }
+                if (off.empty()) off = "0";
+            }
+            loads += "            v" + lsn + "[k] = " + rd + lsn + "])[" + off + "];\n";
+        }
+        namep[l] = names[l].c_str();
+    }
+    for (int i = 0; i < P.ncode; ++i) {
+        const cm_instr& I = P.code[i];
+        uint64_t fb;
+        memcpy(&fb, &I.fimm, 8);
+        snprintf(buf, sizeof(buf), "%d,%d,%d,%d,%d,%d,%llx,%llx;", I.opcode, I.dtype, I.fn, I.src, I.arg, I.flags,
+                 (unsigned long long)fb, (unsigned long long)I.iimm);
+        key += buf;
+    }
+    std::string prog;
+    if (!jit::program_body(P, namep, prog)) return RX_JIT_DECLINED;
+    const char* get = std::is_same<T, float>::value ? "getf" : (std::is_same<T, int>::value ? "geti" : "getd");
+    std::string ident;
+    if (std::is_same<T, double>::value) {
+        if (std::is_same<Op, OpMin>::value) ident = "__longlong_as_double(0x7fefffffffffffffll)";
+        else if (std::is_same<Op, OpMax>::value) ident = "__longlong_as_double((long long)0xffefffffffffffffull)";
+        else if (std::is_same<Op, OpProd>::value || std::is_same<Op, OpAll>::value) ident = "1.0";
+        else ident = "0.0";
+    } else {
+        ident = rx_identity_src<Op, T>();
+    }
+    body = std::string("typedef ") + tname + " TT;\ntypedef " + sname + " SS;\ntypedef " + RxOpName<Op>::get() +
+           " OP;\n"
+           "struct JitRxArgs { const void* leaf[" + std::to_string(CM_CWISE_MAX_LEAVES) +
+           "]; void* partials; void* out; long long L, chunk; int nchunk, pad; };\n"
+           "extern \"C\" __global__ __launch_bounds__(256) void cm_jit_reduce(const JitRxArgs A)\n{\n"
+           "    typedef uint64_t R;\n"
+           "    __shared__ TT lds[4];\n"
+           "    const long long s = blockIdx.x / A.nchunk;\n"
+           "    const int c = blockIdx.x % A.nchunk;\n"
+           "    const long long beg = c * A.chunk;\n"
+           "    const long long end = min(A.L, beg + A.chunk);\n" +
+           scalars +
+           "    TT acc = " + ident + ";\n"
+           "    const long long t0 = (long long)threadIdx.x * 4;\n"
+           "    for (long long p0 = beg; p0 < end; p0 += 256 * 4) {\n"
+           "        const long long j0 = p0 + t0;\n"
+           "        if (j0 >= end) break;\n" + decls +
+           "#pragma unroll\n"
+           "        for (int k = 0; k < 4; ++k) {\n" + (any_x ? coords : std::string()) + loads +
+           "        }\n"
+           "#pragma unroll\n"
+           "        for (int k = 0; k < 4; ++k) {\n"
+           "            R acc_r = 0, t = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;\n"
+           "            (void)t; (void)s0; (void)s1; (void)s2; (void)s3;\n"
+           "#define acc acc_r\n" +
+           prog +
+           "#undef acc\n"
+           "            if (j0 + k < end) acc = OP::apply(acc, (TT)" + get + "(acc_r));\n"
+           "        }\n"
+           "    }\n"
+           "    acc = cm::block_reduce<OP, 256>(acc, lds);\n"
+           "    if (threadIdx.x == 0) {\n"
+           "        if (A.nchunk == 1) ((SS*)A.out)[s] = (SS)acc;\n"
+           "        else ((TT*)A.partials)[blockIdx.x] = acc;\n"
+           "    }\n"
+           "}\n";
+    return CM_OK;
+}
+
 // instances with external linkage, so the generator can be exercised on a
 // host without a GPU (source generation and compilation only)
 template int jit_reduce_flat_source<OpSum, float, float>(const CwProgram&, std::string&, std::string&);
 template int jit_reduce_flat_source<OpAll, int, unsigned char>(const CwProgram&, std::string&, std::string&);
 template int jit_reduce_strided_source<OpSum, float, float>(const CwProgram&, const RxShape&, std::string&,
                                                             std::string&, bool);
+template int jit_reduce_wide_source<OpSum, double, double>(const CwProgram&, const RxShape&, std::string&,
+                                                           std::string&);
 
 template <typename Op, typename T, typename S, bool W, int NL>
 static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P0, const RxShape& SH, S* out)
@@ -535,10 +680,10 @@ static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P0, const RxShape& 
     void* part = nullptr;
     if (nchunk > 1) CM_TRY(scratch(ctx, 1, sizeof(T) * (size_t)blocks, &part));
     bool done = false;
-    if constexpr (!W) {
+    {
         if (jit::want(SH.L * SH.nseg)) {
             // everything the generators read (the front cache's signature)
-            std::string sig("rx");
+            std::string sig(W ? "rxw" : "rx");
             sig += RxOpName<Op>::get();
             jit::sig_add(sig, (int)sizeof(T));
             jit::sig_add(sig, (int)std::is_same<T, float>::value);
@@ -585,8 +730,11 @@ static int launch_reduce_expr(cm_ctx_t ctx, const CwProgram& P0, const RxShape& 
             int gen = CM_OK;
             if (jit::find_sig(sig, &k) != 0) {
                 std::string key, body;
-                gen = flat ? jit_reduce_flat_source<Op, T, S>(P, key, body)
-                           : jit_reduce_strided_source<Op, T, S>(P, SH, key, body, spread);
+                if constexpr (W)
+                    gen = jit_reduce_wide_source<Op, T, S>(P, SH, key, body);
+                else
+                    gen = flat ? jit_reduce_flat_source<Op, T, S>(P, key, body)
+                               : jit_reduce_strided_source<Op, T, S>(P, SH, key, body, spread);
                 if (gen == CM_OK) gen = jit::get(key, body, "cm_jit_reduce", &k) == 0 ? CM_OK : RX_JIT_DECLINED;
                 if (gen == CM_OK) jit::put_sig(sig, k);
             }

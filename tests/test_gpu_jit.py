@@ -668,3 +668,52 @@ def test_strided_programs_f64_and_int(gpu):
     Ci = cm.DeviceArray.from_numpy(ci)
     got = _st_same(cm, lambda: cm.DeviceArray(R, C, B, dtype=np.int32), Ai.expr() % Ci.expr())
     assert np.array_equal(got, np.fmod(ai, ci))
+
+
+@pytest.mark.parametrize("op", ["sum", "prod", "min", "max"])
+@pytest.mark.parametrize("axis", [7, 3, 1])
+def test_reduce_expression_generated_f64(gpu, data64, op, axis):
+    """64-bit programs (4 consecutive elements per thread in 64-bit
+    registers): FP64 a*b + c over every element, per batch, per column, with
+    a broadcast column vector -- the same bits as the interpreter."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    ops = {"sum": capi.RED_SUM, "prod": capi.RED_PROD, "min": capi.RED_MIN, "max": capi.RED_MAX}
+    R, Cc, B = 1000, 250, 4
+    st = (1, R, R * Cc)
+    e = data64["e"] if op != "prod" else _Leaf(gpu, 1.0 + data64["d"].arr * 1e-6)
+    code = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F64, capi.BIN_MUL, capi.SRC_LEAF, 1),
+            _instr(capi.OP_BINARY, capi.F64, capi.BIN_ADD, capi.SRC_LEAF, 2)]
+    for leaves in ([(data64["d"], st), (e, st), (data64["e"], st)],
+                   [(data64["d"], st), (e, (1, 0, 0)), (data64["e"], (0, 1, 0))]):
+        ref = _rx(cm, leaves, code, capi.F64, ops[op], axis, (R, Cc, B), np.float64, jit=0)
+        cm.reset_counters()
+        got = _rx(cm, leaves, code, capi.F64, ops[op], axis, (R, Cc, B), np.float64, jit=2)
+        assert cm.counters()["JitLaunch"] == 1
+        assert ref.tobytes() == got.tobytes()
+
+
+def test_reduce_expression_generated_mixed_width(gpu, data, data64):
+    """An FP32 result of a program with an FP64 leaf (64-bit registers, FP32
+    accumulation) and a bool result of an FP64 comparison."""
+    from cumat_amd import _capi as capi
+    cm = gpu
+    R, Cc, B = 700, 300, 3
+    st = (1, R, R * Cc)
+    code = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),
+            _instr(capi.OP_BINARY, capi.F64, capi.BIN_SUB, capi.SRC_LEAF, 1),
+            _instr(capi.OP_CAST, capi.F32, arg=capi.F64)]
+    leaves = [(data["f"], st), (data64["d"], (Cc, 1, R * Cc))]
+    for op in (capi.RED_SUM, capi.RED_MAX):
+        ref = _rx(cm, leaves, code, capi.F32, op, 6, (R, Cc, B), np.float32, jit=0)
+        got = _rx(cm, leaves, code, capi.F32, op, 6, (R, Cc, B), np.float32, jit=2)
+        assert ref.tobytes() == got.tobytes()
+    codeb = [_instr(capi.OP_LOAD, capi.F64, src=capi.SRC_LEAF, arg=0),
+             _instr(capi.OP_BINARY, capi.F64, capi.BIN_LT, capi.SRC_LEAF, 1)]
+    for op in (capi.RED_SUM, capi.RED_ALL, capi.RED_ANY):
+        ref = _rx(cm, leaves, codeb, capi.BOOL, op, 7, (R, Cc, B), np.bool_, jit=0)
+        cm.reset_counters()
+        got = _rx(cm, leaves, codeb, capi.BOOL, op, 7, (R, Cc, B), np.bool_, jit=2)
+        assert cm.counters()["JitLaunch"] == 1
+        assert ref.tobytes() == got.tobytes()
