@@ -422,8 +422,9 @@ int cm_csr_inverse_diagonal(cm_ctx_t ctx, int dtype, int rows, int nnz,
  * then computed). x holds the guess on entry if use_guess, else is zeroed.
  * iterations_out / error_out are host pointers.
  * ====================================================================== */
-/* Launch shapes: AUTO picks RESIDENT for one small system (n <= 6144) and
- * MULTIKERNEL otherwise. MULTIKERNEL: three launches per iteration (SpMV,
+/* Launch shapes: AUTO picks RESIDENT for one small system (n <= 6144),
+ * FUSED up to 2^17 unknowns (n x batches, where an iteration is bound by
+ * the launches) and MULTIKERNEL beyond. MULTIKERNEL: three launches per iteration (SpMV,
  * x/r update, p update). FUSED: two launches per iteration (the p update is
  * folded into the SpMV launch, each gathered p value being recomputed from
  * r, M^-1 and the previous p). RESIDENT: the whole solve in one workgroup
