@@ -11,7 +11,9 @@ successive launches cannot be served from the 256 MB last-level cache.
 Algorithmic bytes per launch = 8*nnz + 12*n.
 
 Also reported (bounded, after the timed region): the linear combination of 8
-vectors of 2^26 floats, dot of 2^26, general cwise programs over 2^26,
+vectors of 2^26 floats, dot of 2^26, general cwise programs over 2^26
+(generated kernels beside the interpreter; also FP64 over 2^25, broadcasts
+and reductions of broadcasts over 4096x4096x4),
 row/column/batch sums of 1024x1024x64, the 16384^2 dense product (both
 layouts), ELLPACK SpMV and cwise evaluation into CSR values on the 2048^2
 matrix, the block-scalar (3x3 blocks) SpMV and CG on a 96^3 grid, the 128^2
